@@ -1,0 +1,928 @@
+// runtime.hip — host runtime of the engine: device residency of an index
+// (upload, layout conversion), per-index workspace, kernel selection and
+// launch, and the C ABI of include/hnsw_amd.h for the device side.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "brute_kernels.h"
+#include "hh_internal.h"
+#include "search_kernels.h"
+
+using hh::fail;
+
+namespace {
+
+#define HH_HIP(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(e_ == hipErrorOutOfMemory ? HH_OUT_OF_MEMORY : HH_DEVICE_ERROR,       \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+  } while (0)
+
+template <class T>
+int dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return HH_OK;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(HH_OUT_OF_MEMORY, "device allocation of " + std::to_string(count * sizeof(T)) +
+                                      " bytes failed: " + hipGetErrorString(e));
+  }
+  return HH_OK;
+}
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+// natural row-major rows -> accumulator-major rows (RowLayout::pos)
+__global__ void permute_rows_kernel(const float *src, float *dst, uint64_t n, uint32_t dim,
+                                    uint32_t stride, int T) {
+  const uint64_t total = n * dim;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = i / dim;
+    const uint32_t e = (uint32_t)(i % dim);
+    uint32_t pos = e;
+    if (e < 32u * (uint32_t)T) {
+      uint32_t t = e >> 5, j = e & 31, b = t >> 2;
+      uint32_t tw = (uint32_t)T - 4 * b < 4 ? (uint32_t)T - 4 * b : 4;
+      pos = b * 128 + j * tw + (t & 3);
+    }
+    dst[r * stride + pos] = src[i];
+  }
+}
+
+// f32 -> bf16, round to nearest even (rows are finite).
+__global__ void to_bf16_kernel(const float *src, uint16_t *dst, uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t u = __float_as_uint(src[i]);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    dst[i] = (uint16_t)(u >> 16);
+  }
+}
+
+int grid_for(uint64_t work) {
+  uint64_t g = (work + 255) / 256;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, 65536));
+}
+
+int upload_rows(const float *host_rows, uint64_t n, uint32_t dim, const hh::RowLayout &lay,
+                float **out) {
+  *out = nullptr;
+  if (n == 0) return HH_OK;
+  float *tmp = nullptr, *rows = nullptr;
+  int st = dalloc(&tmp, n * dim);
+  if (st) return st;
+  st = dalloc(&rows, n * lay.stride);
+  if (st) {
+    dfree(tmp);
+    return st;
+  }
+  hipError_t e = hipMemcpy(tmp, host_rows, n * dim * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(rows, 0, n * lay.stride * sizeof(float));
+  if (e == hipSuccess) {
+    permute_rows_kernel<<<grid_for(n * dim), 256>>>(tmp, rows, n, dim, lay.stride, (int)lay.T);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  dfree(tmp);
+  if (e != hipSuccess) {
+    dfree(rows);
+    return fail(HH_DEVICE_ERROR, std::string("row upload: ") + hipGetErrorString(e));
+  }
+  *out = rows;
+  return HH_OK;
+}
+
+int tmax_for(const hh::RowLayout &lay) {
+  if (lay.T <= 4) return 4;
+  if (lay.T <= 8) return 8;
+  if (lay.T <= 32) return 32;
+  return 0;
+}
+int slots_for(uint32_t ef) {
+  if (ef <= 64) return 1;
+  if (ef <= 128) return 2;
+  if (ef <= 256) return 4;
+  if (ef <= 512) return 8;
+  return 0;
+}
+
+// Per-index workspace; one search runs in it at a time (ordered by `done`).
+struct Workspace {
+  std::mutex mu;
+  hipEvent_t done = nullptr;
+  float *qprep = nullptr;
+  int8_t *qcode = nullptr;
+  int32_t *status = nullptr;
+  uint64_t cap_q = 0;
+  uint32_t *counter = nullptr;
+  unsigned long long *stats = nullptr;
+  uint32_t *spill = nullptr;
+  uint64_t spill_slots = 0;
+  // brute-force partial lists
+  float *part_d = nullptr;
+  double *part_d64 = nullptr;
+  uint64_t *part_k = nullptr;
+  uint32_t *part_n = nullptr;
+  uint64_t cap_part = 0, cap_partn = 0;
+
+  int init() {
+    HH_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    int st = dalloc(&counter, 4);
+    if (!st) st = dalloc(&stats, 8);
+    if (st) return st;
+    HH_HIP(hipMemset(stats, 0, 8 * sizeof(unsigned long long)));
+    HH_HIP(hipMemset(counter, 0, 4 * sizeof(uint32_t)));
+    return HH_OK;
+  }
+  void release() {
+    if (done) (void)hipEventSynchronize(done);
+    dfree(qprep);
+    dfree(qcode);
+    dfree(status);
+    dfree(counter);
+    dfree(stats);
+    dfree(spill);
+    dfree(part_d);
+    dfree(part_d64);
+    dfree(part_k);
+    dfree(part_n);
+    if (done) (void)hipEventDestroy(done);
+    done = nullptr;
+  }
+  int ensure_queries(uint64_t nq, uint32_t stride, uint32_t cstride) {
+    if (nq <= cap_q) return HH_OK;
+    HH_HIP(hipEventSynchronize(done));
+    dfree(qprep);
+    dfree(qcode);
+    dfree(status);
+    uint64_t cap = std::max<uint64_t>(nq, 1024);
+    int st = dalloc(&qprep, cap * stride);
+    if (!st) st = dalloc(&qcode, cap * std::max<uint32_t>(cstride, 1));
+    if (!st) st = dalloc(&status, cap);
+    if (st) {
+      cap_q = 0;
+      return st;
+    }
+    cap_q = cap;
+    return HH_OK;
+  }
+  int ensure_spill(uint64_t slots, uint64_t words) {
+    if (slots <= spill_slots) return HH_OK;
+    HH_HIP(hipEventSynchronize(done));
+    dfree(spill);
+    int st = dalloc(&spill, slots * words);
+    if (st) {
+      spill_slots = 0;
+      return st;
+    }
+    HH_HIP(hipMemset(spill, 0, slots * words * sizeof(uint32_t)));
+    spill_slots = slots;
+    return HH_OK;
+  }
+  int ensure_part(uint64_t entries, uint64_t lists, bool f64) {
+    if (entries <= cap_part && lists <= cap_partn && (!f64 || part_d64)) return HH_OK;
+    HH_HIP(hipEventSynchronize(done));
+    dfree(part_d);
+    dfree(part_d64);
+    dfree(part_k);
+    dfree(part_n);
+    uint64_t ce = std::max(entries, cap_part), cl = std::max(lists, cap_partn);
+    int st = dalloc(&part_d, ce);
+    if (!st) st = dalloc(&part_d64, ce);
+    if (!st) st = dalloc(&part_k, ce);
+    if (!st) st = dalloc(&part_n, cl);
+    if (st) {
+      cap_part = cap_partn = 0;
+      return st;
+    }
+    cap_part = ce;
+    cap_partn = cl;
+    return HH_OK;
+  }
+};
+
+int num_cus(int device) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  return cus;
+}
+
+}  // namespace
+
+struct hh_index {
+  int device = 0;
+  uint32_t dim = 0, metric = 0, m = 0, m0 = 0;
+  uint64_t n = 0, live = 0;
+  uint32_t entry = 0;
+  int32_t top = -1;
+  hh::RowLayout lay;
+  uint32_t cstride = 0;
+  float *rows = nullptr;
+  uint16_t *bf16 = nullptr;
+  int8_t *codes = nullptr;
+  uint32_t *adj0 = nullptr;
+  uint64_t *upper_off = nullptr;
+  uint32_t *upper = nullptr;
+  uint64_t *keys = nullptr;
+  uint32_t *deleted_bits = nullptr;
+  float *sq_offset = nullptr;
+  float sq_scale = 1.0f;
+  Workspace ws;
+  hipStream_t stream = nullptr;  // used by the synchronous host-memory API
+
+  ~hh_index() {
+    (void)hipSetDevice(device);
+    ws.release();
+    dfree(rows);
+    dfree(bf16);
+    dfree(codes);
+    dfree(adj0);
+    dfree(upper_off);
+    dfree(upper);
+    dfree(keys);
+    dfree(deleted_bits);
+    dfree(sq_offset);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct hh_brute {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t dim = 0, metric = 0;
+  uint64_t n = 0;
+  hh::RowLayout lay;
+  float *rows = nullptr;
+  uint64_t *keys = nullptr;
+  Workspace ws;
+  ~hh_brute() {
+    (void)hipSetDevice(device);
+    ws.release();
+    dfree(rows);
+    dfree(keys);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int pack_deleted(const uint8_t *deleted, uint64_t n, std::vector<uint32_t> &bits) {
+  bits.assign((n + 31) / 32, 0u);
+  if (deleted)
+    for (uint64_t i = 0; i < n; i++)
+      if (deleted[i]) bits[i >> 5] |= 1u << (i & 31);
+  return HH_OK;
+}
+
+template <int TMAX, int SW>
+void *search_kernel_ptr() {
+  return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, SW>);
+}
+
+void *pick_search_kernel(int tmax, int sw) {
+#define HH_PICK(TM)                                     \
+  if (tmax == TM) {                                     \
+    if (sw == 1) return search_kernel_ptr<TM, 1>();     \
+    if (sw == 2) return search_kernel_ptr<TM, 2>();     \
+    if (sw == 4) return search_kernel_ptr<TM, 4>();     \
+    if (sw == 8) return search_kernel_ptr<TM, 8>();     \
+  }
+  HH_PICK(4)
+  HH_PICK(8)
+  HH_PICK(32)
+#undef HH_PICK
+  return nullptr;
+}
+
+// Reports the lowest-index failing query of a batch, as an error.
+int first_failure(const std::vector<int32_t> &status, uint64_t *first_bad) {
+  for (uint64_t i = 0; i < status.size(); i++)
+    if (status[i]) {
+      if (first_bad) *first_bad = i;
+      static const char *msgs[] = {"", "dimension mismatch", "invalid dimension",
+                                   "invalid parameter", "vector contains NaN or infinite values",
+                                   "zero vector has no direction under cosine distance",
+                                   "duplicate key", "capacity exceeded",
+                                   "quantized search requested but quantization is not enabled"};
+      std::string m = status[i] < 9 ? msgs[status[i]] : "invalid query parameters";
+      return fail(status[i], m + " (query " + std::to_string(i) + ")");
+    }
+  return HH_OK;
+}
+
+// Shared launch path of the uniform and the per-query-spec searches.
+int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
+                  uint32_t precision, const uint32_t *allow, const hh_query_spec *specs,
+                  uint32_t k_stride, const uint32_t *filters, uint32_t nfilters,
+                  uint64_t *out_keys, float *out_dist, uint32_t *out_count, int32_t *status,
+                  hipStream_t stream) {
+  if (nq == 0) return HH_OK;
+  if (!queries || !out_count || (k_stride > 0 && (!out_keys || !out_dist)))
+    return fail(HH_INVALID_ARGUMENT, "null device pointer");
+  const int tmax = tmax_for(idx->lay);
+  if (!tmax) return fail(HH_UNSUPPORTED, "search kernels support dim <= 1024");
+  uint32_t efmax = specs ? 512u : std::max(ef, k);
+  if (!specs && precision > 2) return fail(HH_INVALID_ARGUMENT, "unknown precision");
+  const int sw = slots_for(std::max<uint32_t>(efmax, 1));
+  if (!sw) return fail(HH_UNSUPPORTED, "ef (raised to k) above 512 is not supported");
+  if (!specs && precision == 2 && !idx->bf16)
+    return fail(HH_UNSUPPORTED, "bf16 rows not resident: call hh_index_enable_bf16");
+  HH_HIP(hipSetDevice(idx->device));
+  Workspace &ws = idx->ws;
+  std::lock_guard<std::mutex> lock(ws.mu);
+  int st = ws.ensure_queries(nq, idx->lay.stride, idx->cstride);
+  if (st) return st;
+  const uint32_t hash_cap = std::max<uint32_t>(efmax, 1) <= 128 ? 4096u : 8192u;
+  const uint32_t lds_words = hash_cap + 128;
+  const int wpb = std::max(1, std::min(4, (int)(65536 / (lds_words * 4))));
+  void *kern = pick_search_kernel(tmax, sw);
+  int blocks_per_cu = 0;
+  HH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kern, wpb * 64,
+                                                      lds_words * 4 * wpb));
+  blocks_per_cu = std::max(1, blocks_per_cu);
+  const uint64_t want = (nq + wpb - 1) / wpb;
+  const uint64_t blocks = std::min<uint64_t>(want, (uint64_t)blocks_per_cu * num_cus(idx->device));
+  const uint64_t spill_words = std::max<uint64_t>(1, (idx->n + 31) / 32);
+  st = ws.ensure_spill(blocks * wpb, spill_words);
+  if (st) return st;
+  HH_HIP(hipStreamWaitEvent(stream, ws.done, 0));
+
+  hhk::PrepArgs pa{};
+  pa.queries = queries;
+  pa.nq = nq;
+  pa.dim = idx->dim;
+  pa.metric = idx->metric;
+  pa.T = (int)idx->lay.T;
+  pa.stride = idx->lay.stride;
+  pa.cstride = idx->cstride;
+  pa.qprep = ws.qprep;
+  pa.qcode = idx->codes ? ws.qcode : nullptr;
+  pa.sq_offset = idx->sq_offset;
+  pa.sq_scale = idx->sq_scale;
+  pa.precision = precision;
+  pa.specs = specs;
+  pa.k_stride = k_stride;
+  pa.nfilters = nfilters;
+  pa.ef_cap = 512;
+  pa.has_codes = idx->codes != nullptr;
+  pa.has_bf16 = idx->bf16 != nullptr;
+  pa.status = status ? status : ws.status;
+  pa.counter = ws.counter;
+  hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(grid_for(nq)), dim3(256), 0, stream, pa);
+  HH_HIP(hipGetLastError());
+
+  hhk::SearchArgs a{};
+  a.rows = idx->rows;
+  a.bf16 = idx->bf16;
+  a.codes = idx->codes;
+  a.adj0 = idx->adj0;
+  a.upper_off = idx->upper_off;
+  a.upper = idx->upper;
+  a.keys = idx->keys;
+  a.deleted_bits = idx->deleted_bits;
+  a.n = (uint32_t)idx->n;
+  a.m = idx->m;
+  a.m0 = idx->m0;
+  a.metric = idx->metric;
+  a.entry = idx->entry;
+  a.top = idx->top;
+  a.live = idx->live;
+  a.T = (int)idx->lay.T;
+  a.U = (int)idx->lay.U;
+  a.R = (int)idx->lay.R;
+  a.stride = idx->lay.stride;
+  a.cstride = idx->cstride;
+  a.sq_scale = idx->sq_scale;
+  a.qprep = ws.qprep;
+  a.qcode = ws.qcode;
+  a.status = pa.status;
+  a.nq = nq;
+  a.specs = specs;
+  a.k = k;
+  a.ef = ef;
+  a.precision = precision;
+  a.allow = allow;
+  a.filters = filters;
+  a.filter_words = (uint32_t)((idx->n + 31) / 32);
+  a.k_stride = k_stride;
+  a.out_keys = out_keys;
+  a.out_dist = out_dist;
+  a.out_count = out_count;
+  a.counter = ws.counter;
+  a.spill = ws.spill;
+  a.spill_words = spill_words;
+  a.stats = ws.stats;
+  a.hash_cap = hash_cap;
+  a.hash_limit = hash_cap / 2 + hash_cap / 4;
+  a.lds_words = lds_words;
+  void *args[] = {&a};
+  HH_HIP(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(wpb * 64), args, lds_words * 4 * wpb,
+                         stream));
+  HH_HIP(hipEventRecord(ws.done, stream));
+  return HH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int hh_index_upload(const hh_graph_desc *d, int device, hh_index **out) {
+  if (!d || !out) return fail(HH_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (d->dim == 0 || d->dim > 65536) return fail(HH_INVALID_DIMENSION, "invalid dimension");
+  if (d->metric > 2) return fail(HH_INVALID_ARGUMENT, "unknown metric");
+  if (d->n > 0xFFFFFFFEull) return fail(HH_CAPACITY_EXCEEDED, "too many nodes");
+  if (d->m < 1 || d->m0 < 1) return fail(HH_INVALID_PARAMETER, "m and m0 must be positive");
+  if (d->n > 0 && (!d->vectors || !d->keys || !d->levels || !d->layer0 || !d->upper_off))
+    return fail(HH_INVALID_ARGUMENT, "null graph array");
+  if (d->top_level >= 0 && d->entry_id >= d->n)
+    return fail(HH_INVALID_ARGUMENT, "entry point out of range");
+  HH_HIP(hipSetDevice(device));
+  std::unique_ptr<hh_index> idx(new hh_index());
+  idx->device = device;
+  idx->dim = d->dim;
+  idx->metric = d->metric;
+  idx->m = d->m;
+  idx->m0 = d->m0;
+  idx->n = d->n;
+  idx->live = d->live;
+  idx->entry = d->entry_id;
+  idx->top = d->top_level;
+  idx->lay = hh::RowLayout(d->dim);
+  int st = idx->ws.init();
+  if (st) return st;
+  HH_HIP(hipStreamCreateWithFlags(&idx->stream, hipStreamNonBlocking));
+  if (d->n == 0) {
+    *out = idx.release();
+    return HH_OK;
+  }
+  const uint64_t n = d->n;
+  st = upload_rows(d->vectors, n, d->dim, idx->lay, &idx->rows);
+  if (st) return st;
+  // layer 0: crate blocks (count, ids) -> n x m0 ids with unused slots
+  // 0xFFFFFFFF (the count is implied by the first unused slot).
+  {
+    std::vector<uint32_t> ell((size_t)n * d->m0, 0xFFFFFFFFu);
+    for (uint64_t i = 0; i < n; i++) {
+      const uint32_t *blk = d->layer0 + i * (d->m0 + 1);
+      uint32_t c = std::min(blk[0], d->m0);
+      for (uint32_t s = 0; s < c; s++) {
+        if (blk[1 + s] >= n) return fail(HH_INVALID_ARGUMENT, "layer-0 neighbour out of range");
+        ell[i * d->m0 + s] = blk[1 + s];
+      }
+    }
+    st = dalloc(&idx->adj0, ell.size());
+    if (st) return st;
+    HH_HIP(hipMemcpy(idx->adj0, ell.data(), ell.size() * 4, hipMemcpyHostToDevice));
+  }
+  {
+    uint64_t words = 0;
+    for (uint64_t i = 0; i < n; i++)
+      words = std::max<uint64_t>(words, d->upper_off[i] + (uint64_t)d->levels[i] * (d->m + 1));
+    st = dalloc(&idx->upper_off, n);
+    if (!st) st = dalloc(&idx->upper, std::max<uint64_t>(words, 1));
+    if (st) return st;
+    HH_HIP(hipMemcpy(idx->upper_off, d->upper_off, n * 8, hipMemcpyHostToDevice));
+    if (words) HH_HIP(hipMemcpy(idx->upper, d->upper, words * 4, hipMemcpyHostToDevice));
+  }
+  st = dalloc(&idx->keys, n);
+  if (st) return st;
+  HH_HIP(hipMemcpy(idx->keys, d->keys, n * 8, hipMemcpyHostToDevice));
+  {
+    std::vector<uint32_t> bits;
+    pack_deleted(d->deleted, n, bits);
+    st = dalloc(&idx->deleted_bits, bits.size());
+    if (st) return st;
+    HH_HIP(hipMemcpy(idx->deleted_bits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice));
+  }
+  if (d->codes) {
+    if (!d->sq_offset) return fail(HH_INVALID_ARGUMENT, "codes without quantizer offsets");
+    idx->cstride = (d->dim + 127) / 128 * 128;
+    std::vector<int8_t> packed((size_t)n * idx->cstride, 0);
+    for (uint64_t i = 0; i < n; i++)
+      std::memcpy(packed.data() + i * idx->cstride, d->codes + i * d->dim, d->dim);
+    st = dalloc(&idx->codes, packed.size());
+    if (!st) st = dalloc(&idx->sq_offset, d->dim);
+    if (st) return st;
+    HH_HIP(hipMemcpy(idx->codes, packed.data(), packed.size(), hipMemcpyHostToDevice));
+    HH_HIP(hipMemcpy(idx->sq_offset, d->sq_offset, d->dim * 4, hipMemcpyHostToDevice));
+    idx->sq_scale = d->sq_scale;
+  }
+  *out = idx.release();
+  return HH_OK;
+}
+
+void hh_index_free(hh_index *idx) { delete idx; }
+
+int hh_index_update_deleted(hh_index *idx, const uint8_t *deleted, uint64_t live) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (idx->n == 0) return HH_OK;
+  HH_HIP(hipSetDevice(idx->device));
+  std::vector<uint32_t> bits;
+  pack_deleted(deleted, idx->n, bits);
+  std::lock_guard<std::mutex> lock(idx->ws.mu);
+  HH_HIP(hipEventSynchronize(idx->ws.done));
+  HH_HIP(hipMemcpy(idx->deleted_bits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice));
+  idx->live = live;
+  return HH_OK;
+}
+
+int hh_index_enable_bf16(hh_index *idx) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (idx->bf16 || idx->n == 0) return HH_OK;
+  HH_HIP(hipSetDevice(idx->device));
+  const uint64_t count = idx->n * idx->lay.stride;
+  int st = dalloc(&idx->bf16, count);
+  if (st) return st;
+  to_bf16_kernel<<<grid_for(count), 256>>>(idx->rows, idx->bf16, count);
+  HH_HIP(hipGetLastError());
+  HH_HIP(hipDeviceSynchronize());
+  return HH_OK;
+}
+
+int hh_index_stats(hh_index *idx, hh_search_stats *out) {
+  if (!idx || !out) return fail(HH_INVALID_ARGUMENT, "null argument");
+  HH_HIP(hipSetDevice(idx->device));
+  std::lock_guard<std::mutex> lock(idx->ws.mu);
+  HH_HIP(hipEventSynchronize(idx->ws.done));
+  unsigned long long v[8] = {0};
+  HH_HIP(hipMemcpy(v, idx->ws.stats, sizeof v, hipMemcpyDeviceToHost));
+  out->queries = v[0];
+  out->n_dist = v[1];
+  out->n_upper = v[2];
+  out->n_expand = v[3];
+  out->n_rerank = v[4];
+  out->n_overflow = v[5];
+  out->n_visited_spill = v[6];
+  return HH_OK;
+}
+
+int hh_index_reset_stats(hh_index *idx) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  HH_HIP(hipSetDevice(idx->device));
+  std::lock_guard<std::mutex> lock(idx->ws.mu);
+  HH_HIP(hipEventSynchronize(idx->ws.done));
+  HH_HIP(hipMemset(idx->ws.stats, 0, 8 * sizeof(unsigned long long)));
+  return HH_OK;
+}
+
+int hh_search_batch_device(hh_index *idx, const float *queries, uint64_t nq, uint32_t k,
+                           uint32_t ef, uint32_t precision, const uint32_t *allow_bits,
+                           uint64_t *out_keys, float *out_dist, uint32_t *out_count,
+                           int32_t *status, void *stream) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  return launch_search(idx, queries, nq, k, ef, precision, allow_bits, nullptr, k, nullptr, 0,
+                       out_keys, out_dist, out_count, status, (hipStream_t)stream);
+}
+
+int hh_search_batch_spec_device(hh_index *idx, const float *queries, uint64_t nq,
+                                const hh_query_spec *specs, uint32_t k_stride,
+                                const uint32_t *filters, uint32_t nfilters, uint64_t *out_keys,
+                                float *out_dist, uint32_t *out_count, int32_t *status,
+                                void *stream) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (!specs) return fail(HH_INVALID_ARGUMENT, "null specs");
+  return launch_search(idx, queries, nq, 0, 0, 0, nullptr, specs, k_stride, filters, nfilters,
+                       out_keys, out_dist, out_count, status, (hipStream_t)stream);
+}
+
+int hh_search_batch(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
+                    uint32_t precision, const uint32_t *allow_bits, uint64_t *out_keys,
+                    float *out_dist, uint32_t *out_count, uint64_t *first_bad) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (nq == 0) return HH_OK;
+  if (!queries || !out_count || (k && (!out_keys || !out_dist)))
+    return fail(HH_INVALID_ARGUMENT, "null argument");
+  HH_HIP(hipSetDevice(idx->device));
+  float *dq = nullptr, *dd = nullptr;
+  uint64_t *dk = nullptr;
+  uint32_t *dc = nullptr, *da = nullptr;
+  int32_t *ds = nullptr;
+  auto cleanup = [&]() {
+    dfree(dq);
+    dfree(dd);
+    dfree(dk);
+    dfree(dc);
+    dfree(da);
+    dfree(ds);
+  };
+  const uint64_t kk = std::max<uint32_t>(k, 1);
+  int st = dalloc(&dq, nq * idx->dim);
+  if (!st) st = dalloc(&dd, nq * kk);
+  if (!st) st = dalloc(&dk, nq * kk);
+  if (!st) st = dalloc(&dc, nq);
+  if (!st) st = dalloc(&ds, nq);
+  if (!st && allow_bits) st = dalloc(&da, std::max<uint64_t>(1, (idx->n + 31) / 32));
+  if (st) {
+    cleanup();
+    return st;
+  }
+  hipStream_t s = idx->stream;
+  hipError_t e = hipMemcpyAsync(dq, queries, nq * idx->dim * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && allow_bits)
+    e = hipMemcpyAsync(da, allow_bits, ((idx->n + 31) / 32) * 4, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(HH_DEVICE_ERROR, hipGetErrorString(e));
+  }
+  st = launch_search(idx, dq, nq, k, ef, precision, da, nullptr, k, nullptr, 0, dk, dd, dc, ds, s);
+  if (st) {
+    (void)hipStreamSynchronize(s);
+    cleanup();
+    return st;
+  }
+  std::vector<int32_t> status(nq);
+  e = hipMemcpyAsync(status.data(), ds, nq * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(out_count, dc, nq * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && k) e = hipMemcpyAsync(out_keys, dk, nq * k * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && k) e = hipMemcpyAsync(out_dist, dd, nq * k * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  cleanup();
+  if (e != hipSuccess) return fail(HH_DEVICE_ERROR, hipGetErrorString(e));
+  return first_failure(status, first_bad);
+}
+
+// ---------------------------------------------------------------- brute
+int hh_brute_upload(uint32_t dim, uint32_t metric, const uint64_t *keys, const float *vectors,
+                    uint64_t n, int device, hh_brute **out, uint64_t *bad_key) {
+  if (!out) return fail(HH_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (dim == 0 || dim > 65536)
+    return fail(HH_INVALID_DIMENSION, "invalid dimension " + std::to_string(dim) +
+                                          ": must be between 1 and 65536");
+  if (metric > 2) return fail(HH_INVALID_ARGUMENT, "unknown metric");
+  if (n && (!keys || !vectors)) return fail(HH_INVALID_ARGUMENT, "null argument");
+  // prepare_batch then the duplicate check (src/brute.rs:102-114)
+  std::vector<float> prepared((size_t)n * dim);
+  for (uint64_t i = 0; i < n; i++) {
+    int st = hh::prepare_vector(metric, dim, vectors + i * dim, prepared.data() + i * dim);
+    if (st) return st;
+  }
+  std::unordered_set<uint64_t> seen;
+  seen.reserve(n);
+  for (uint64_t i = 0; i < n; i++)
+    if (!seen.insert(keys[i]).second) {
+      if (bad_key) *bad_key = keys[i];
+      return fail(HH_DUPLICATE_KEY, "key " + std::to_string(keys[i]) + " is already present");
+    }
+  HH_HIP(hipSetDevice(device));
+  std::unique_ptr<hh_brute> b(new hh_brute());
+  b->device = device;
+  b->dim = dim;
+  b->metric = metric;
+  b->n = n;
+  b->lay = hh::RowLayout(dim);
+  int st = b->ws.init();
+  if (st) return st;
+  HH_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  st = upload_rows(prepared.data(), n, dim, b->lay, &b->rows);
+  if (!st) st = dalloc(&b->keys, n);
+  if (st) return st;
+  if (n) HH_HIP(hipMemcpy(b->keys, keys, n * 8, hipMemcpyHostToDevice));
+  *out = b.release();
+  return HH_OK;
+}
+
+void hh_brute_free(hh_brute *b) { delete b; }
+
+}  // extern "C"
+
+namespace {
+
+template <int TM, int QB>
+void *brute_partial_ptr(int s) {
+  if (s == 1) return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, QB, 1>);
+  if (s == 2) return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, QB, 2>);
+  if (s == 4) return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, QB, 4>);
+  return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, QB, 8>);
+}
+
+int brute_launch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                 const uint32_t *allow, uint64_t *out_keys, float *out_dist, double *out_dist64,
+                 uint32_t *out_count, int32_t *status, hipStream_t stream, bool f64) {
+  if (!b) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (nq == 0) return HH_OK;
+  if (!queries || !out_count) return fail(HH_INVALID_ARGUMENT, "null device pointer");
+  const int tmax = tmax_for(b->lay);
+  if (!tmax) return fail(HH_UNSUPPORTED, "brute-force kernels support dim <= 1024");
+  const uint32_t kc = (uint32_t)std::min<uint64_t>(k, b->n);  // src/brute.rs:141-142
+  if (kc > 512) return fail(HH_UNSUPPORTED, "k above 512 (after clamping to len) is not supported");
+  if (kc && (!out_keys || (!out_dist && !out_dist64)))
+    return fail(HH_INVALID_ARGUMENT, "null device pointer");
+  const int s = slots_for(std::max<uint32_t>(kc, 1));
+  HH_HIP(hipSetDevice(b->device));
+  Workspace &ws = b->ws;
+  std::lock_guard<std::mutex> lock(ws.mu);
+  int st = ws.ensure_queries(nq, b->lay.stride, 0);
+  if (st) return st;
+  const int qb = f64 ? 1 : (tmax == 4 ? 4 : (tmax == 8 ? 2 : 1));
+  const uint64_t groups = (nq + qb - 1) / qb;
+  const uint64_t target_waves = 8192;
+  uint64_t nslices = std::max<uint64_t>(1, (target_waves + groups - 1) / groups);
+  nslices = std::min<uint64_t>(nslices, std::max<uint64_t>(1, (b->n + 1) / 2));
+  nslices = std::min<uint64_t>(nslices, 1024);
+  uint64_t rps = (b->n + nslices - 1) / nslices;
+  rps = (rps + 1) / 2 * 2;
+  if (rps == 0) rps = 2;
+  nslices = std::max<uint64_t>(1, (b->n + rps - 1) / rps);
+  st = ws.ensure_part(nq * nslices * std::max<uint32_t>(kc, 1), nq * nslices, f64);
+  if (st) return st;
+  HH_HIP(hipStreamWaitEvent(stream, ws.done, 0));
+  hhk::PrepArgs pa{};
+  pa.queries = queries;
+  pa.nq = nq;
+  pa.dim = b->dim;
+  pa.metric = b->metric;
+  pa.T = (int)b->lay.T;
+  pa.stride = b->lay.stride;
+  pa.qprep = ws.qprep;
+  pa.sq_scale = 1.0f;
+  pa.status = status ? status : ws.status;
+  pa.counter = nullptr;
+  pa.ef_cap = 512;
+  hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(grid_for(nq)), dim3(256), 0, stream, pa);
+  HH_HIP(hipGetLastError());
+  hhk::BruteArgs a{};
+  a.rows = b->rows;
+  a.keys = b->keys;
+  a.allow = allow;
+  a.n = b->n;
+  a.dim = b->dim;
+  a.metric = b->metric;
+  a.T = (int)b->lay.T;
+  a.U = (int)b->lay.U;
+  a.R = (int)b->lay.R;
+  a.stride = b->lay.stride;
+  a.qprep = ws.qprep;
+  a.status = pa.status;
+  a.nq = nq;
+  a.k = kc;
+  a.k_stride = k;
+  a.nslices = (uint32_t)nslices;
+  a.rows_per_slice = rps;
+  a.part_d = ws.part_d;
+  a.part_d64 = ws.part_d64;
+  a.part_k = ws.part_k;
+  a.part_n = ws.part_n;
+  a.out_keys = out_keys;
+  a.out_dist = out_dist;
+  a.out_dist64 = out_dist64;
+  a.out_count = out_count;
+  void *args[] = {&a};
+  if (kc == 0 || b->n == 0) {
+    // nothing to rank: every valid query gets zero results
+    a.nslices = 0;
+    void *mk = reinterpret_cast<void *>(&hhk::brute_merge_kernel<1>);
+    HH_HIP(hipLaunchKernel(mk, dim3((unsigned)((nq + 3) / 4)), dim3(256), args, 0, stream));
+    HH_HIP(hipEventRecord(ws.done, stream));
+    return HH_OK;
+  }
+  const uint64_t work = f64 ? nq * nslices : groups * nslices;
+  void *pk;
+  void *mk;
+  if (f64) {
+    pk = s == 1 ? (void *)&hhk::brute_f64_kernel<1>
+                : s == 2 ? (void *)&hhk::brute_f64_kernel<2>
+                         : s == 4 ? (void *)&hhk::brute_f64_kernel<4> : (void *)&hhk::brute_f64_kernel<8>;
+    mk = s == 1 ? (void *)&hhk::brute_merge_f64_kernel<1>
+                : s == 2 ? (void *)&hhk::brute_merge_f64_kernel<2>
+                         : s == 4 ? (void *)&hhk::brute_merge_f64_kernel<4>
+                                  : (void *)&hhk::brute_merge_f64_kernel<8>;
+  } else {
+    pk = tmax == 4 ? brute_partial_ptr<4, 4>(s) : tmax == 8 ? brute_partial_ptr<8, 2>(s) : brute_partial_ptr<32, 1>(s);
+    mk = s == 1 ? (void *)&hhk::brute_merge_kernel<1>
+                : s == 2 ? (void *)&hhk::brute_merge_kernel<2>
+                         : s == 4 ? (void *)&hhk::brute_merge_kernel<4> : (void *)&hhk::brute_merge_kernel<8>;
+  }
+  HH_HIP(hipLaunchKernel(pk, dim3((unsigned)((work + 3) / 4)), dim3(256), args, 0, stream));
+  HH_HIP(hipLaunchKernel(mk, dim3((unsigned)((nq + 3) / 4)), dim3(256), args, 0, stream));
+  HH_HIP(hipEventRecord(ws.done, stream));
+  return HH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hh_brute_search_batch_device(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                                 const uint32_t *allow_bits, uint64_t *out_keys, float *out_dist,
+                                 uint32_t *out_count, int32_t *status, void *stream) {
+  return brute_launch(b, queries, nq, k, allow_bits, out_keys, out_dist, nullptr, out_count,
+                      status, (hipStream_t)stream, false);
+}
+
+int hh_brute_search_batch_f64_device(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                                     uint64_t *out_keys, double *out_dist, uint32_t *out_count,
+                                     int32_t *status, void *stream) {
+  return brute_launch(b, queries, nq, k, nullptr, out_keys, nullptr, out_dist, out_count, status,
+                      (hipStream_t)stream, true);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+uint64_t hh_brute_len(const hh_brute *b) { return b ? b->n : 0; }
+
+static int brute_host(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                      const uint32_t *allow_bits, uint64_t *out_keys, float *out_dist,
+                      double *out_dist64, uint32_t *out_count, uint64_t *first_bad, bool f64) {
+  if (!b) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (nq == 0) return HH_OK;
+  if (!queries || !out_count || (k && (!out_keys || (!out_dist && !out_dist64))))
+    return fail(HH_INVALID_ARGUMENT, "null argument");
+  HH_HIP(hipSetDevice(b->device));
+  const uint64_t kk = std::max<uint32_t>(k, 1);
+  float *dq = nullptr, *dd = nullptr;
+  double *dd64 = nullptr;
+  uint64_t *dk = nullptr;
+  uint32_t *dc = nullptr, *da = nullptr;
+  int32_t *ds = nullptr;
+  auto cleanup = [&]() {
+    dfree(dq);
+    dfree(dd);
+    dfree(dd64);
+    dfree(dk);
+    dfree(dc);
+    dfree(da);
+    dfree(ds);
+  };
+  const uint64_t words = std::max<uint64_t>(1, (b->n + 31) / 32);
+  int st = dalloc(&dq, nq * b->dim);
+  if (!st) st = f64 ? dalloc(&dd64, nq * kk) : dalloc(&dd, nq * kk);
+  if (!st) st = dalloc(&dk, nq * kk);
+  if (!st) st = dalloc(&dc, nq);
+  if (!st) st = dalloc(&ds, nq);
+  if (!st && allow_bits) st = dalloc(&da, words);
+  if (st) {
+    cleanup();
+    return st;
+  }
+  hipStream_t s = b->stream;
+  hipError_t e = hipMemcpyAsync(dq, queries, nq * b->dim * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && allow_bits)
+    e = hipMemcpyAsync(da, allow_bits, words * 4, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(HH_DEVICE_ERROR, hipGetErrorString(e));
+  }
+  st = brute_launch(b, dq, nq, k, da, dk, dd, dd64, dc, ds, s, f64);
+  if (st) {
+    (void)hipStreamSynchronize(s);
+    cleanup();
+    return st;
+  }
+  std::vector<int32_t> status(nq);
+  e = hipMemcpyAsync(status.data(), ds, nq * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(out_count, dc, nq * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && k) e = hipMemcpyAsync(out_keys, dk, nq * k * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && k && !f64)
+    e = hipMemcpyAsync(out_dist, dd, nq * k * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && k && f64)
+    e = hipMemcpyAsync(out_dist64, dd64, nq * k * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  cleanup();
+  if (e != hipSuccess) return fail(HH_DEVICE_ERROR, hipGetErrorString(e));
+  return first_failure(status, first_bad);
+}
+
+int hh_brute_search_batch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                          const uint32_t *allow_bits, uint64_t *out_keys, float *out_dist,
+                          uint32_t *out_count, uint64_t *first_bad) {
+  return brute_host(b, queries, nq, k, allow_bits, out_keys, out_dist, nullptr, out_count,
+                    first_bad, false);
+}
+
+int hh_brute_search_batch_f64(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                              uint64_t *out_keys, double *out_dist, uint32_t *out_count,
+                              uint64_t *first_bad) {
+  return brute_host(b, queries, nq, k, nullptr, out_keys, nullptr, out_dist, out_count, first_bad,
+                    true);
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// search_kernels.h — argument blocks of the gfx950 kernels and their
+// template instances (shared by the kernel and host translation units).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hnsw_amd.h"
+
+namespace hhk {
+
+struct SearchArgs {
+  // index (HBM)
+  const float *rows;            // n x stride, accumulator-major f32 rows
+  const uint16_t *bf16;         // n x stride bf16 rows, or null
+  const int8_t *codes;          // n x cstride int8 codes (natural order), or null
+  const uint32_t *adj0;         // n x m0 layer-0 ids, unused slots 0xFFFFFFFF
+  const uint64_t *upper_off;    // n
+  const uint32_t *upper;        // crate layout: count then m ids per layer
+  const uint64_t *keys;         // n
+  const uint32_t *deleted_bits; // ceil(n/32) words, or null
+  uint32_t n, m, m0, metric;
+  uint32_t entry;
+  int32_t top;
+  uint64_t live;
+  int T, U, R;
+  uint32_t stride, cstride;
+  float sq_scale;
+  // queries (prepared by prepare_queries_kernel)
+  const float *qprep;
+  const int8_t *qcode;
+  const int32_t *status;
+  uint64_t nq;
+  const hh_query_spec *specs;   // null = uniform k/ef/precision/allow
+  uint32_t k, ef, precision;
+  const uint32_t *allow;
+  const uint32_t *filters;
+  uint32_t filter_words;
+  uint32_t k_stride;
+  // outputs
+  uint64_t *out_keys;
+  float *out_dist;
+  uint32_t *out_count;
+  // workspace
+  uint32_t *counter;
+  uint32_t *spill;              // per-wave visited bitsets (spill_words each)
+  uint64_t spill_words;
+  unsigned long long *stats;    // 7 counters
+  uint32_t hash_cap, hash_limit, lds_words;
+};
+
+struct PrepArgs {
+  const float *queries;
+  uint64_t nq;
+  uint32_t dim, metric;
+  int T;
+  uint32_t stride, cstride;
+  float *qprep;
+  int8_t *qcode;
+  const float *sq_offset;
+  float sq_scale;
+  uint32_t precision;
+  const hh_query_spec *specs;
+  uint32_t k_stride, nfilters, ef_cap;
+  int has_codes, has_bf16;
+  int32_t *status;
+  uint32_t *counter;
+};
+
+template <int TMAX, int SW>
+__global__ void hnsw_search_kernel(const SearchArgs P);
+__global__ void prepare_queries_kernel(const PrepArgs A);
+
+}  // namespace hhk

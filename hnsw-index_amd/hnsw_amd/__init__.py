@@ -1,0 +1,507 @@
+"""Python mirror of the hnsw-index crate's public API over the MI355X engine.
+
+Names, argument meaning and error behaviour follow the crate
+(aozmekik/hnsw-index): `HnswIndex` (src/hnsw/mod.rs), `BruteForceIndex`
+(src/brute.rs), `SearchOptions`, `Metric`, `Neighbor`, `HnswParams`,
+`NeighborSelection`, `Error` variants (src/error.rs), `eval.recall_at_k`,
+`synthetic.Dataset`, and `kernels.Backend` with the new `Backend.HIP`.
+
+Graph construction (`insert`, `insert_batch`) runs on the CPU, as in the
+crate; every search runs on the GPU through libhnsw_amd.so.  There is no CPU
+search path: without a GPU, searching raises `DeviceError`.
+"""
+import ctypes
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import lib
+
+__all__ = [
+    "Backend", "BruteForceIndex", "Error", "HnswIndex", "HnswParams", "Metric", "Neighbor",
+    "NeighborSelection", "SearchOptions", "Precision", "eval", "synthetic", "MAX_DIM",
+    "MAX_NODES",
+]
+
+MAX_DIM = 1 << 16           # src/lib.rs:70
+MAX_NODES = (1 << 32) - 2   # src/lib.rs:74
+
+
+# ---------------------------------------------------------------- errors
+class Error(Exception):
+    """Base of every error (src/error.rs:9-51); `.status` is the C ABI code."""
+    status = -1
+
+    def __init__(self, msg="", **fields):
+        super().__init__(msg)
+        for k, v in fields.items():
+            setattr(self, k, v)
+
+
+class DimensionMismatch(Error): status = 1
+class InvalidDimension(Error): status = 2
+class InvalidParameter(Error): status = 3
+class NonFiniteVector(Error): status = 4
+class ZeroVector(Error): status = 5
+class DuplicateKey(Error): status = 6
+class CapacityExceeded(Error): status = 7
+class QuantizationDisabled(Error): status = 8
+class DeviceError(Error): status = 100
+class OutOfMemory(Error): status = 101
+class Unsupported(Error): status = 102
+class InvalidArgument(Error): status = 103
+
+
+_BY_STATUS = {c.status: c for c in (DimensionMismatch, InvalidDimension, InvalidParameter,
+                                     NonFiniteVector, ZeroVector, DuplicateKey,
+                                     CapacityExceeded, QuantizationDisabled, DeviceError,
+                                     OutOfMemory, Unsupported, InvalidArgument)}
+
+
+def _check(st, **fields):
+    if st:
+        raise _BY_STATUS.get(st, Error)(L.last_error(), **fields)
+
+
+# ------------------------------------------------------------ plain types
+class Metric(enum.IntEnum):
+    """src/metric.rs:11-30 (tags as :44-50)."""
+    L2 = 0
+    InnerProduct = 1
+    Cosine = 2
+
+
+class Backend(enum.Enum):
+    """kernels::Backend (src/kernels/mod.rs:55-66) plus the HIP engine."""
+    Scalar = "scalar"
+    Portable = "portable"
+    Neon = "neon"
+    Avx2 = "avx2"
+    Hip = "hip"
+
+    def is_available(self):
+        if self is Backend.Hip:
+            return lib.hh_device_count() > 0
+        if self is Backend.Neon:
+            return False
+        return True
+
+
+class Precision(enum.IntEnum):
+    F32 = 0
+    INT8 = 1
+    BF16 = 2
+
+
+@dataclass(frozen=True)
+class Neighbor:
+    """src/neighbor.rs:7-12."""
+    key: int
+    distance: float
+
+
+@dataclass(frozen=True)
+class NeighborSelection:
+    """src/params.rs:19-46."""
+    heuristic: bool = True
+    extend_candidates: bool = False
+    keep_pruned: bool = False
+
+    @staticmethod
+    def simple():
+        return NeighborSelection(False)
+
+    @staticmethod
+    def heuristic_with(extend_candidates=False, keep_pruned=False):
+        return NeighborSelection(True, extend_candidates, keep_pruned)
+
+
+class HnswParams:
+    """src/params.rs:61-161."""
+
+    def __init__(self, m=16):
+        p = L.Params()
+        lib.hh_params_default(m, ctypes.byref(p))
+        self._p = p
+        if m < 2:  # keep what the crate keeps; validation happens at HnswIndex()
+            self._p.m0 = m * 2
+
+    def with_m0(self, m0):
+        self._p.m0 = m0
+        return self
+
+    def with_ef_construction(self, ef):
+        self._p.ef_construction = ef
+        return self
+
+    def with_level_multiplier(self, ml):
+        self._p.level_multiplier = ml
+        return self
+
+    def with_selection(self, sel):
+        self._p.selection = 1 if sel.heuristic else 0
+        self._p.extend_candidates = int(sel.extend_candidates)
+        self._p.keep_pruned = int(sel.keep_pruned)
+        return self
+
+    def with_seed(self, seed):
+        self._p.seed = seed
+        return self
+
+    def m(self): return self._p.m
+    def m0(self): return self._p.m0
+    def ef_construction(self): return self._p.ef_construction
+    def level_multiplier(self): return self._p.level_multiplier
+    def seed(self): return self._p.seed
+
+
+class SearchOptions:
+    """src/hnsw/mod.rs:56-98, plus `precision` for the bf16 traversal mode."""
+
+    def __init__(self, ef):
+        self._ef = ef
+        self._filter = None
+        self._precision = Precision.F32
+
+    def filter(self, f):
+        self._filter = f
+        return self
+
+    def quantized(self, q):
+        self._precision = Precision.INT8 if q else Precision.F32
+        return self
+
+    def precision(self, p):
+        self._precision = Precision(p)
+        return self
+
+    def ef(self):
+        return self._ef
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _bits_from_mask(mask):
+    """bool array over ids -> uint32 words, bit i of word i/32 = mask[i]."""
+    mask = np.ascontiguousarray(mask, dtype=bool)
+    n = mask.size
+    padded = np.zeros(((n + 31) // 32) * 32, dtype=bool)
+    padded[:n] = mask
+    return np.packbits(padded, bitorder="little").view(np.uint32).copy()
+
+
+def _results(keys, dists, counts):
+    return [[Neighbor(int(keys[i, j]), float(dists[i, j])) for j in range(int(counts[i]))]
+            for i in range(len(counts))]
+
+
+# ----------------------------------------------------------------- index
+class HnswIndex:
+    """The HNSW index (src/hnsw/mod.rs:148-172): built on the CPU, searched
+    on the GPU.  The device copy is uploaded on the first search after a
+    mutation."""
+
+    def __init__(self, dim, metric=Metric.L2, params=None, device=0):
+        params = params or HnswParams()
+        h = ctypes.c_void_p()
+        _check(lib.hh_graph_new(dim, int(metric), ctypes.byref(params._p), ctypes.byref(h)))
+        self._g = h
+        self._dim = dim
+        self._metric = Metric(metric)
+        self._params = params
+        self._device = device
+        self._idx = None
+        self._dirty = True
+        self._bf16 = False
+
+    def __del__(self):
+        try:
+            if getattr(self, "_idx", None):
+                lib.hh_index_free(self._idx)
+            if getattr(self, "_g", None):
+                lib.hh_graph_free(self._g)
+        except Exception:
+            pass
+
+    # -- properties (src/hnsw/mod.rs:210-264)
+    def dim(self): return self._dim
+    def metric(self): return self._metric
+    def params(self): return self._params
+    def backend(self): return Backend.Hip
+    def __len__(self): return int(lib.hh_graph_len(self._g))
+    def len(self): return len(self)
+    def is_empty(self): return len(self) == 0
+    def node_count(self): return int(lib.hh_graph_node_count(self._g))
+    def deleted_len(self): return self.node_count() - len(self)
+
+    def contains(self, key):
+        return key in self._key_map()
+
+    def _key_map(self):
+        v = self.graph_arrays()
+        live = ~v["deleted"].astype(bool)
+        return set(v["keys"][live].tolist())
+
+    # -- mutation (CPU, src/hnsw/mod.rs:290-400)
+    def insert(self, key, vector):
+        self.insert_batch([key], np.asarray(vector, np.float32).reshape(1, -1), threads=1)
+
+    def insert_batch(self, keys, vectors, threads=None):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        vectors = _f32(vectors)
+        if vectors.size != keys.size * self._dim:
+            raise DimensionMismatch(
+                f"dimension mismatch: expected {keys.size * self._dim}, got {vectors.size}",
+                expected=keys.size * self._dim, actual=vectors.size)
+        if threads is None:
+            threads = min(16, os.cpu_count() or 1)
+        bad = ctypes.c_uint64(0)
+        st = lib.hh_graph_insert_batch(self._g, _ptr(keys), _ptr(vectors), keys.size, threads,
+                                       ctypes.byref(bad))
+        _check(st, key=bad.value)
+        self._dirty = True
+
+    def remove(self, key):
+        r = ctypes.c_int(0)
+        _check(lib.hh_graph_remove(self._g, key, ctypes.byref(r)))
+        if r.value and self._idx and not self._dirty:
+            v = self.graph_arrays()
+            _check(lib.hh_index_update_deleted(self._idx, _ptr(v["deleted"]), len(self)))
+        return bool(r.value)
+
+    def enable_quantization(self):
+        _check(lib.hh_graph_enable_quantization(self._g))
+        self._dirty = True
+
+    def disable_quantization(self):
+        _check(lib.hh_graph_disable_quantization(self._g))
+        self._dirty = True
+
+    def is_quantized(self):
+        return bool(self._view().codes)
+
+    # -- graph access
+    def _view(self):
+        d = L.GraphDesc()
+        _check(lib.hh_graph_view(self._g, ctypes.byref(d)))
+        return d
+
+    def graph_arrays(self):
+        """Borrowed numpy views of the host graph in the crate's layout."""
+        d = self._view()
+        n, dim, m, m0 = d.n, d.dim, d.m, d.m0
+
+        def arr(ptr, ctype, count):
+            if not ptr or count == 0:
+                return np.zeros(0, dtype=np.ctypeslib.as_array(ctypes.pointer(ctype())).dtype)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), (count,))
+
+        levels = arr(d.levels, ctypes.c_uint8, n)
+        upper_off = arr(d.upper_off, ctypes.c_uint64, n)
+        words = int((upper_off.astype(np.int64) + levels.astype(np.int64) * (m + 1)).max()) if n else 0
+        return dict(
+            dim=dim, metric=d.metric, m=m, m0=m0, n=n, entry_id=d.entry_id, top=d.top_level,
+            vectors=arr(d.vectors, ctypes.c_float, n * dim).reshape(n, dim),
+            keys=arr(d.keys, ctypes.c_uint64, n), levels=levels,
+            layer0=arr(d.layer0, ctypes.c_uint32, n * (m0 + 1)), upper_off=upper_off,
+            upper=arr(d.upper, ctypes.c_uint32, words), deleted=arr(d.deleted, ctypes.c_uint8, n),
+            codes=arr(d.codes, ctypes.c_int8, n * dim).reshape(-1, dim) if d.codes else None,
+            sq_offset=arr(d.sq_offset, ctypes.c_float, dim) if d.codes else None,
+            sq_scale=d.sq_scale, live=d.live)
+
+    # -- device residency
+    def device_index(self):
+        """The uploaded hh_index handle (uploading first if needed)."""
+        if self._dirty or self._idx is None:
+            if self._idx:
+                lib.hh_index_free(self._idx)
+                self._idx = None
+            d = self._view()
+            h = ctypes.c_void_p()
+            _check(lib.hh_index_upload(ctypes.byref(d), self._device, ctypes.byref(h)))
+            self._idx = h
+            self._dirty = False
+            self._bf16 = False
+        return self._idx
+
+    def enable_bf16(self):
+        _check(lib.hh_index_enable_bf16(self.device_index()))
+        self._bf16 = True
+
+    def search_stats(self, reset=False):
+        s = L.SearchStats()
+        _check(lib.hh_index_stats(self.device_index(), ctypes.byref(s)))
+        if reset:
+            _check(lib.hh_index_reset_stats(self.device_index()))
+        return {f: getattr(s, f) for f, _ in L.SearchStats._fields_}
+
+    # -- search (src/hnsw/mod.rs:402-536), on the GPU
+    def _allow_bits(self, flt):
+        if flt is None:
+            return None
+        v = self.graph_arrays()
+        if isinstance(flt, np.ndarray) and flt.dtype == bool:
+            mask = flt  # by node id
+        else:
+            keys = v["keys"]
+            mask = np.fromiter((bool(flt(int(kk))) for kk in keys), dtype=bool, count=keys.size)
+        return _bits_from_mask(mask)
+
+    def search_arrays(self, queries, k, ef, precision=Precision.F32, filter=None):
+        """Batched search into padded arrays: (keys nq x k, dists nq x k, counts nq)."""
+        queries = _f32(queries)
+        if queries.size % self._dim:
+            raise DimensionMismatch(
+                f"dimension mismatch: expected {self._dim}, got {queries.size}",
+                expected=self._dim, actual=queries.size)
+        nq = queries.size // self._dim
+        if precision == Precision.BF16 and not self._bf16:
+            self.device_index()
+            self.enable_bf16()
+        idx = self.device_index()
+        kk = max(int(k), 1)
+        ok = np.zeros((nq, kk), np.uint64)
+        od = np.zeros((nq, kk), np.float32)
+        oc = np.zeros(nq, np.uint32)
+        allow = self._allow_bits(filter)
+        bad = ctypes.c_uint64(0)
+        st = lib.hh_search_batch(idx, _ptr(queries), nq, int(k), int(ef), int(precision),
+                                 _ptr(allow), _ptr(ok), _ptr(od), _ptr(oc), ctypes.byref(bad))
+        _check(st, query=bad.value)
+        return ok[:, :k], od[:, :k], oc
+
+    def search_batch(self, queries, k, ef):
+        queries = _f32(queries)
+        if queries.size % self._dim:
+            raise DimensionMismatch(
+                f"dimension mismatch: expected {self._dim}, got {queries.size}",
+                expected=self._dim, actual=queries.size)
+        return _results(*self.search_arrays(queries, k, ef))
+
+    def _single(self, query, k, ef, precision, flt):
+        q = _f32(query).ravel()
+        if q.size != self._dim:
+            raise DimensionMismatch(f"dimension mismatch: expected {self._dim}, got {q.size}",
+                                    expected=self._dim, actual=q.size)
+        return _results(*self.search_arrays(q.reshape(1, -1), k, ef, precision, flt))[0]
+
+    def search(self, query, k, ef):
+        return self._single(query, k, ef, Precision.F32, None)
+
+    def search_filtered(self, query, k, ef, filter):
+        return self._single(query, k, ef, Precision.F32, filter)
+
+    def search_with(self, query, k, options):
+        return self._single(query, k, options._ef, options._precision, options._filter)
+
+
+class BruteForceIndex:
+    """Exact search (src/brute.rs:28-205), on the GPU."""
+
+    def __init__(self, dim, metric=Metric.L2, device=0):
+        if dim == 0 or dim > MAX_DIM:
+            raise InvalidDimension(f"invalid dimension {dim}: must be between 1 and {MAX_DIM}")
+        self._dim = dim
+        self._metric = Metric(metric)
+        self._device = device
+        self._keys = np.zeros(0, np.uint64)
+        self._vecs = np.zeros((0, dim), np.float32)
+        self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib.hh_brute_free(self._h)
+        except Exception:
+            pass
+
+    def dim(self): return self._dim
+    def metric(self): return self._metric
+    def __len__(self): return int(self._keys.size)
+    def len(self): return len(self)
+    def is_empty(self): return len(self) == 0
+
+    def insert(self, key, vector):
+        self.insert_batch([key], np.asarray(vector, np.float32).reshape(1, -1))
+
+    def insert_batch(self, keys, vectors):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        vectors = _f32(vectors)
+        if vectors.size != keys.size * self._dim:
+            raise DimensionMismatch(
+                f"dimension mismatch: expected {keys.size * self._dim}, got {vectors.size}",
+                expected=keys.size * self._dim, actual=vectors.size)
+        new_keys = np.concatenate([self._keys, keys])
+        new_vecs = np.concatenate([self._vecs, vectors.reshape(-1, self._dim)])
+        h = ctypes.c_void_p()
+        bad = ctypes.c_uint64(0)
+        # validation, duplicate checks and upload happen in the library; the
+        # batch is rejected as a whole on error (src/brute.rs:102-114)
+        st = lib.hh_brute_upload(self._dim, int(self._metric), _ptr(new_keys), _ptr(new_vecs),
+                                 new_keys.size, self._device, ctypes.byref(h), ctypes.byref(bad))
+        _check(st, key=bad.value)
+        if self._h:
+            lib.hh_brute_free(self._h)
+        self._h = h
+        self._keys, self._vecs = new_keys, new_vecs
+
+    def _handle(self):
+        if self._h is None:
+            h = ctypes.c_void_p()
+            bad = ctypes.c_uint64(0)
+            _check(lib.hh_brute_upload(self._dim, int(self._metric), None, None, 0,
+                                       self._device, ctypes.byref(h), ctypes.byref(bad)))
+            self._h = h
+        return self._h
+
+    def search_arrays(self, queries, k, filter=None, f64=False):
+        queries = _f32(queries)
+        if queries.size % self._dim:
+            raise DimensionMismatch(
+                f"dimension mismatch: expected {self._dim}, got {queries.size}",
+                expected=self._dim, actual=queries.size)
+        nq = queries.size // self._dim
+        k = int(min(k, max(len(self), 0), 1 << 31))
+        kk = max(k, 1)
+        ok = np.zeros((nq, kk), np.uint64)
+        od = np.zeros((nq, kk), np.float64 if f64 else np.float32)
+        oc = np.zeros(nq, np.uint32)
+        allow = None
+        if filter is not None:
+            mask = filter if isinstance(filter, np.ndarray) else np.fromiter(
+                (bool(filter(int(x))) for x in self._keys), dtype=bool, count=self._keys.size)
+            allow = _bits_from_mask(mask)
+        bad = ctypes.c_uint64(0)
+        if f64:
+            st = lib.hh_brute_search_batch_f64(self._handle(), _ptr(queries), nq, k, _ptr(ok),
+                                               _ptr(od), _ptr(oc), ctypes.byref(bad))
+        else:
+            st = lib.hh_brute_search_batch(self._handle(), _ptr(queries), nq, k, _ptr(allow),
+                                           _ptr(ok), _ptr(od), _ptr(oc), ctypes.byref(bad))
+        _check(st, query=bad.value)
+        return ok[:, :k], od[:, :k], oc
+
+    def search(self, query, k):
+        return self.search_filtered(query, k, None)
+
+    def search_filtered(self, query, k, filter):
+        q = _f32(query).ravel()
+        if q.size != self._dim:
+            raise DimensionMismatch(f"dimension mismatch: expected {self._dim}, got {q.size}",
+                                    expected=self._dim, actual=q.size)
+        return _results(*self.search_arrays(q.reshape(1, -1), k, filter))[0]
+
+    def search_batch(self, queries, k):
+        return _results(*self.search_arrays(queries, k))
+
+
+from . import eval, synthetic  # noqa: E402  (public submodules, as in the crate)

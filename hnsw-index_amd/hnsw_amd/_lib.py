@@ -1,0 +1,89 @@
+"""Loads libhnsw_amd.so (the C ABI of include/hnsw_amd.h) and declares it.
+
+The library is built in-tree by `make -C hnsw-index_amd` (or
+__graft_entry__.build()).  There is no fallback: if the library is missing,
+importing the package fails with an ImportError that says how to build it.
+"""
+import ctypes
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libhnsw_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `make -C {PKG_ROOT}` (hipcc, gfx950) first")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+I32 = ctypes.c_int32
+INT = ctypes.c_int
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("m", U32), ("m0", U32), ("ef_construction", U32), ("selection", U32),
+                ("extend_candidates", U32), ("keep_pruned", U32),
+                ("level_multiplier", ctypes.c_double), ("seed", U64)]
+
+
+class GraphDesc(ctypes.Structure):
+    _fields_ = [("dim", U32), ("metric", U32), ("m", U32), ("m0", U32), ("n", U64),
+                ("entry_id", U32), ("top_level", I32), ("vectors", P), ("keys", P),
+                ("levels", P), ("layer0", P), ("upper_off", P), ("upper", P), ("deleted", P),
+                ("codes", P), ("sq_offset", P), ("sq_scale", ctypes.c_float), ("live", U64)]
+
+
+class SearchStats(ctypes.Structure):
+    _fields_ = [("queries", U64), ("n_dist", U64), ("n_upper", U64), ("n_expand", U64),
+                ("n_rerank", U64), ("n_overflow", U64), ("n_visited_spill", U64)]
+
+
+class QuerySpec(ctypes.Structure):
+    _fields_ = [("k", U32), ("ef", U32), ("precision", U32), ("filter", U32)]
+
+
+def _sig(name, restype, *args):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = list(args)
+
+
+_sig("hh_last_error", ctypes.c_char_p)
+_sig("hh_device_count", INT)
+_sig("hh_abi_version", U32)
+_sig("hh_params_default", None, U32, ctypes.POINTER(Params))
+_sig("hh_graph_new", INT, U32, U32, ctypes.POINTER(Params), ctypes.POINTER(P))
+_sig("hh_graph_free", None, P)
+_sig("hh_graph_insert_batch", INT, P, P, P, U64, INT, ctypes.POINTER(U64))
+_sig("hh_graph_remove", INT, P, U64, ctypes.POINTER(INT))
+_sig("hh_graph_enable_quantization", INT, P)
+_sig("hh_graph_disable_quantization", INT, P)
+_sig("hh_graph_len", U64, P)
+_sig("hh_graph_node_count", U64, P)
+_sig("hh_graph_view", INT, P, ctypes.POINTER(GraphDesc))
+_sig("hh_index_upload", INT, ctypes.POINTER(GraphDesc), INT, ctypes.POINTER(P))
+_sig("hh_index_free", None, P)
+_sig("hh_index_update_deleted", INT, P, P, U64)
+_sig("hh_index_enable_bf16", INT, P)
+_sig("hh_index_stats", INT, P, ctypes.POINTER(SearchStats))
+_sig("hh_index_reset_stats", INT, P)
+_sig("hh_search_batch_device", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, P)
+_sig("hh_search_batch_spec_device", INT, P, P, U64, P, U32, P, U32, P, P, P, P, P)
+_sig("hh_search_batch", INT, P, P, U64, U32, U32, U32, P, P, P, P, ctypes.POINTER(U64))
+_sig("hh_brute_upload", INT, U32, U32, P, P, U64, INT, ctypes.POINTER(P), ctypes.POINTER(U64))
+_sig("hh_brute_free", None, P)
+_sig("hh_brute_len", U64, P)
+_sig("hh_brute_search_batch_device", INT, P, P, U64, U32, P, P, P, P, P, P)
+_sig("hh_brute_search_batch_f64_device", INT, P, P, U64, U32, P, P, P, P, P)
+_sig("hh_brute_search_batch", INT, P, P, U64, U32, P, P, P, P, ctypes.POINTER(U64))
+_sig("hh_brute_search_batch_f64", INT, P, P, U64, U32, P, P, P, ctypes.POINTER(U64))
+_sig("hh_synthetic", INT, U32, U64, U64, U32, U32, ctypes.c_double, ctypes.c_double, U64, INT,
+     P, P, P)
+_sig("hh_recall_at_k", ctypes.c_double, U64, P, P, U32, P, P, U32, U32)
+
+
+def last_error():
+    return lib.hh_last_error().decode("utf-8", "replace")

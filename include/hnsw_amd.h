@@ -1,0 +1,260 @@
+/*
+ * hnsw_amd.h — C ABI of the MI355X (gfx950) query engine for hnsw-index
+ * graphs.
+ *
+ * This is the boundary a host binding calls (the crate's Rust FFI, or the
+ * ctypes mirror in hnsw-index_amd/hnsw_amd).  Plain pointers and sizes only.
+ * Functions whose pointer arguments are marked "device" take HBM pointers
+ * (hipMalloc'd or from a framework allocator) and are asynchronous on the
+ * given stream; every other pointer is host memory.
+ *
+ * Each entry point cites the reference interface it replaces
+ * (aozmekik/hnsw-index, file:line).
+ *
+ * Status codes: 0 on success, otherwise one of hh_status.  Codes 1..11 map
+ * one-to-one onto the variants of the crate's `Error` (src/error.rs:9-51);
+ * codes >= 100 are device-side conditions the crate has no variant for.
+ * hh_last_error() returns a message for the calling thread's last failure.
+ */
+#ifndef HNSW_AMD_H
+#define HNSW_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HH_ABI_VERSION 1u
+
+typedef enum hh_status {
+  HH_OK = 0,
+  HH_DIMENSION_MISMATCH = 1,     /* Error::DimensionMismatch */
+  HH_INVALID_DIMENSION = 2,      /* Error::InvalidDimension */
+  HH_INVALID_PARAMETER = 3,      /* Error::InvalidParameter */
+  HH_NON_FINITE_VECTOR = 4,      /* Error::NonFiniteVector */
+  HH_ZERO_VECTOR = 5,            /* Error::ZeroVector */
+  HH_DUPLICATE_KEY = 6,          /* Error::DuplicateKey */
+  HH_CAPACITY_EXCEEDED = 7,      /* Error::CapacityExceeded */
+  HH_QUANTIZATION_DISABLED = 8,  /* Error::QuantizationDisabled */
+  HH_DEVICE_ERROR = 100,         /* a HIP call failed */
+  HH_OUT_OF_MEMORY = 101,        /* device allocation refused */
+  HH_UNSUPPORTED = 102,          /* shape outside what the kernels handle */
+  HH_INVALID_ARGUMENT = 103,     /* null pointer, bad enum, ... */
+} hh_status;
+
+/* Metric tags: src/metric.rs:44-50. */
+typedef enum hh_metric { HH_L2 = 0, HH_INNER_PRODUCT = 1, HH_COSINE = 2 } hh_metric;
+
+/* Traversal precision of a search.  Every non-F32 mode re-ranks the final
+ * ef candidates with exact f32 distances, as src/hnsw/mod.rs:483-500 does
+ * for its int8 mode. */
+typedef enum hh_precision {
+  HH_PREC_F32 = 0,          /* exact f32 traversal (SearchOptions::new) */
+  HH_PREC_INT8 = 1,         /* crate quantizer codes (SearchOptions::quantized) */
+  HH_PREC_BF16 = 2,         /* bf16 rows, f32 accumulation (no crate counterpart) */
+} hh_precision;
+
+/* Last error message of the calling thread (never NULL). */
+const char *hh_last_error(void);
+/* Number of visible GPUs (0 when none, never an error). */
+int hh_device_count(void);
+uint32_t hh_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Host-side graph construction.                                        */
+/* The crate builds its graph on the CPU (HnswIndex::insert /          */
+/* insert_batch, src/hnsw/mod.rs:290-349, src/hnsw/build.rs:41-222).   */
+/* Where the crate itself is linked, its graph is handed over through  */
+/* hh_graph_desc directly; these functions restate that construction   */
+/* so the engine can be used, tested and benchmarked without it.        */
+/* ------------------------------------------------------------------ */
+
+/* HnswParams (src/params.rs:61-88). selection: 0 Simple, 1 Heuristic. */
+typedef struct hh_params {
+  uint32_t m, m0, ef_construction;
+  uint32_t selection, extend_candidates, keep_pruned;
+  double level_multiplier;
+  uint64_t seed;
+} hh_params;
+
+/* HnswParams::new(m) defaults (src/params.rs:79-88). */
+void hh_params_default(uint32_t m, hh_params *out);
+
+typedef struct hh_graph hh_graph;
+
+/* HnswIndex::new (src/hnsw/mod.rs:181-187). */
+int hh_graph_new(uint32_t dim, uint32_t metric, const hh_params *params, hh_graph **out);
+void hh_graph_free(hh_graph *g);
+/* HnswIndex::insert_batch (src/hnsw/mod.rs:318-331).  threads <= 1 links
+ * in id order on the calling thread, which equals repeated
+ * HnswIndex::insert (src/hnsw/mod.rs:290-302) and is deterministic.
+ * On error *bad_key receives the offending key (DuplicateKey). */
+int hh_graph_insert_batch(hh_graph *g, const uint64_t *keys, const float *vectors, uint64_t n,
+                          int threads, uint64_t *bad_key);
+/* HnswIndex::remove (src/hnsw/mod.rs:357-365): *removed = 1 if it was live. */
+int hh_graph_remove(hh_graph *g, uint64_t key, int *removed);
+/* HnswIndex::enable_quantization (src/hnsw/mod.rs:544-547). */
+int hh_graph_enable_quantization(hh_graph *g);
+int hh_graph_disable_quantization(hh_graph *g);
+/* HnswIndex::len / node_count / stats (src/hnsw/mod.rs:236,705,562-583). */
+uint64_t hh_graph_len(const hh_graph *g);
+uint64_t hh_graph_node_count(const hh_graph *g);
+
+/* ------------------------------------------------------------------ */
+/* The graph as the crate stores it (src/hnsw/graph.rs:26-33,           */
+/* src/hnsw/mod.rs:148-172).  Filled by hh_graph_view or by a binding.  */
+/* ------------------------------------------------------------------ */
+typedef struct hh_graph_desc {
+  uint32_t dim, metric, m, m0;
+  uint64_t n;               /* nodes, tombstones included */
+  uint32_t entry_id;        /* entry point (src/hnsw/mod.rs:715-717) */
+  int32_t top_level;        /* -1 for an empty graph */
+  const float *vectors;     /* n*dim, prepared (unit length for cosine) */
+  const uint64_t *keys;     /* n */
+  const uint8_t *levels;    /* n */
+  const uint32_t *layer0;   /* n*(m0+1): count, then ids (Links::layer0) */
+  const uint64_t *upper_off;/* n: first word of the node's upper blocks */
+  const uint32_t *upper;    /* levels[i]*(m+1) words per node at upper_off[i] */
+  const uint8_t *deleted;   /* n tombstone flags, or NULL */
+  const int8_t *codes;      /* n*dim int8 codes, or NULL (no quantizer) */
+  const float *sq_offset;   /* dim quantizer offsets (src/quantization.rs:25-28) */
+  float sq_scale;
+  uint64_t live;            /* live (non-deleted) nodes */
+} hh_graph_desc;
+
+/* Borrowed view of a host graph; valid until the graph is mutated or freed. */
+int hh_graph_view(const hh_graph *g, hh_graph_desc *out);
+
+/* ------------------------------------------------------------------ */
+/* Device-resident index: upload once, search many times.              */
+/* ------------------------------------------------------------------ */
+typedef struct hh_index hh_index;
+
+/* Copies the graph into HBM as: ELL layer 0 (n x m0 ids, unused slots
+ * 0xFFFFFFFF), upper blocks, permuted f32 rows, keys, tombstone bits and,
+ * when present, int8 codes.  Replaces the crate's in-memory HnswIndex for
+ * the query path.  device: HIP ordinal. */
+int hh_index_upload(const hh_graph_desc *desc, int device, hh_index **out);
+void hh_index_free(hh_index *idx);
+/* Re-uploads the tombstone flags (after HnswIndex::remove). */
+int hh_index_update_deleted(hh_index *idx, const uint8_t *deleted, uint64_t live);
+/* Converts the resident f32 rows to bf16 (round to nearest even) on the
+ * device, enabling HH_PREC_BF16. */
+int hh_index_enable_bf16(hh_index *idx);
+
+/* Search statistics accumulated by the kernels since the last reset. */
+typedef struct hh_search_stats {
+  uint64_t queries;
+  uint64_t n_dist;          /* traversal distance evaluations, layer 0 */
+  uint64_t n_upper;         /* distance evaluations in the upper descent */
+  uint64_t n_expand;        /* layer-0 expansions */
+  uint64_t n_rerank;        /* exact re-scores after approximate traversal */
+  uint64_t n_overflow;      /* candidates dropped from a full candidate list */
+  uint64_t n_visited_spill; /* queries whose visited set left LDS */
+} hh_search_stats;
+int hh_index_stats(hh_index *idx, hh_search_stats *out);
+int hh_index_reset_stats(hh_index *idx);
+
+/*
+ * Batched HNSW search (HnswIndex::search_batch / search_with /
+ * search_filtered, src/hnsw/mod.rs:412-536).
+ *
+ * device: queries (nq*dim f32, raw: validation and cosine normalisation
+ * happen on the device exactly as src/vector.rs:22-45), out_keys/out_dist
+ * (nq*k), out_count (nq), status (nq int32, the per-query hh_status).
+ * allow_bits: device bitmap over node ids (bit i of word i/32 set = the
+ * filter accepts node i), or NULL for no filter; tombstones are applied on
+ * top (src/hnsw/mod.rs:478-480).
+ * Results per query: up to k neighbours sorted by (distance, key), fewer
+ * only when fewer admissible nodes are reachable.  ef is raised to k.
+ * Asynchronous on stream (a hipStream_t, NULL = default stream).
+ */
+int hh_search_batch_device(hh_index *idx, const float *queries, uint64_t nq, uint32_t k,
+                           uint32_t ef, uint32_t precision, const uint32_t *allow_bits,
+                           uint64_t *out_keys, float *out_dist, uint32_t *out_count,
+                           int32_t *status, void *stream);
+
+/* Per-query parameters for one mixed launch. filter: index of a bitmap in
+ * the filter table passed alongside, or 0xFFFFFFFF for none. */
+typedef struct hh_query_spec {
+  uint32_t k, ef, precision, filter;
+} hh_query_spec;
+
+/* Mixed batch: query i uses specs[i] (device array); its results start at
+ * out_keys + i*k_stride (k_stride >= every spec's k).  filters: device
+ * array of nfilters bitmaps of ceil(n/32) words each, or NULL. */
+int hh_search_batch_spec_device(hh_index *idx, const float *queries, uint64_t nq,
+                                const hh_query_spec *specs, uint32_t k_stride,
+                                const uint32_t *filters, uint32_t nfilters, uint64_t *out_keys,
+                                float *out_dist, uint32_t *out_count, int32_t *status,
+                                void *stream);
+
+/* Synchronous host-memory form of hh_search_batch_device with the crate's
+ * error behaviour: on an invalid query returns its status and writes its
+ * index to *first_bad (src/hnsw/mod.rs:513-536); allow_bits is a host
+ * bitmap or NULL. */
+int hh_search_batch(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
+                    uint32_t precision, const uint32_t *allow_bits, uint64_t *out_keys,
+                    float *out_dist, uint32_t *out_count, uint64_t *first_bad);
+
+/* ------------------------------------------------------------------ */
+/* Exact brute force (BruteForceIndex, src/brute.rs:28-180).            */
+/* ------------------------------------------------------------------ */
+typedef struct hh_brute hh_brute;
+
+/* BruteForceIndex::new + insert_batch: vectors are raw; they are validated
+ * and (cosine) normalised on the host as src/vector.rs:69-87.  On a
+ * duplicate key *bad_key receives it. */
+int hh_brute_upload(uint32_t dim, uint32_t metric, const uint64_t *keys, const float *vectors,
+                    uint64_t n, int device, hh_brute **out, uint64_t *bad_key);
+void hh_brute_free(hh_brute *b);
+
+/* BruteForceIndex::search_batch / search_filtered (src/brute.rs:131-180):
+ * exact top-k under (distance, key), distances bit-identical to the
+ * crate's AVX2 kernels.  k is clamped to n.  device pointers as in
+ * hh_search_batch_device; allow_bits is over row positions. */
+int hh_brute_search_batch_device(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                                 const uint32_t *allow_bits, uint64_t *out_keys, float *out_dist,
+                                 uint32_t *out_count, int32_t *status, void *stream);
+/* The FP64 oracle: same ranking with every distance evaluated in f64. */
+int hh_brute_search_batch_f64_device(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                                     uint64_t *out_keys, double *out_dist, uint32_t *out_count,
+                                     int32_t *status, void *stream);
+
+/* Synchronous host-memory forms (src/brute.rs:162-180 error behaviour: an
+ * invalid query's status is returned and its index written to *first_bad).
+ * out arrays hold nq*k entries; allow_bits is a host bitmap over rows. */
+int hh_brute_search_batch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                          const uint32_t *allow_bits, uint64_t *out_keys, float *out_dist,
+                          uint32_t *out_count, uint64_t *first_bad);
+int hh_brute_search_batch_f64(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
+                              uint64_t *out_keys, double *out_dist, uint32_t *out_count,
+                              uint64_t *first_bad);
+uint64_t hh_brute_len(const hh_brute *b);
+
+/* ------------------------------------------------------------------ */
+/* Host helpers that mirror public crate items.                         */
+/* ------------------------------------------------------------------ */
+
+/* synthetic::Dataset::gaussian / clustered (src/synthetic.rs:81-124) and
+ * the generators of the benchmark workloads (kind >= 2, no crate
+ * counterpart): 2 = uniform-centre clusters (centres U[-1,1], sigma for
+ * base rows, sigma_q for queries), 3 = U(0,1) rows then L2-normalised,
+ * 4 = N(0,1) rows scaled by exp(sigma*N(0,1)), 5 = N(0,1) centres with
+ * sigma spread.  categories (kind 4, optional): n category ids drawn
+ * uniformly below `clusters`. */
+int hh_synthetic(uint32_t kind, uint64_t n, uint64_t nq, uint32_t dim, uint32_t clusters,
+                 double sigma, double sigma_q, uint64_t seed, int threads, float *out_base,
+                 float *out_queries, uint32_t *categories);
+
+/* eval::recall_at_k (src/eval.rs:25-47) over padded result arrays. */
+double hh_recall_at_k(uint64_t nq, const uint64_t *truth, const uint32_t *truth_count,
+                      uint32_t truth_stride, const uint64_t *found, const uint32_t *found_count,
+                      uint32_t found_stride, uint32_t k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
