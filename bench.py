@@ -1,0 +1,255 @@
+"""Benchmark of the HNSW query path on one MI355X.
+
+Workload (the GPU configuration of the headline metric): 1,000,000 f32
+vectors, dim 128, clustered (1,000 centres uniform in [-1,1]^128, rows =
+centre + N(0, 0.1^2)), L2, M=16, efConstruction=200, graph built on the CPU
+by the host library and uploaded; 10,000 queries = random centre +
+N(0, 0.12^2); k=10; ef swept over {16,32,64,128,256}.  Ground truth is the
+exact GPU brute force (bit-identical to the crate's BruteForceIndex),
+spot-checked against the FP64 oracle kernel.
+
+A step is one batched search of all queries, inputs resident in HBM, at the
+smallest swept ef whose recall@10 reaches 0.95.  The JSON line reports
+QPS over the timed steps, the int8 and bf16 traversal modes at that ef, the
+search kernel's roofline position (algorithmic gathered bytes over the
+kernel time measured with HIP events on the launch stream), and the CPU
+baseline: the oracle's C restatement of search_batch on the host cores, on
+a bounded sample of the queries.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hnsw-index_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--dim", type=int, default=128)
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--efs", default="16,32,64,128,256")
+    ap.add_argument("--target", type=float, default=0.95)
+    ap.add_argument("--cpu-sample", type=int, default=10000)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--f64-check", type=int, default=500)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each query is "
+              "independent, so N GPUs would each hold a replica and take 1/N of the queries)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import hnsw_amd as H
+    from hnsw_amd import _lib as L
+
+    if H.lib.hh_device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    threads = args.threads or min(16, os.cpu_count() or 1)
+    efs = [int(x) for x in args.efs.split(",")]
+    n, nq, dim, k = args.n, args.nq, args.dim, args.k
+
+    log(f"generating {n} x {dim} clustered rows + {nq} queries")
+    data = H.synthetic.Dataset.uniform_clusters(n, nq, dim, 1000, 0.1, 0.12, 42, threads=threads)
+    idx = H.HnswIndex(dim, H.Metric.L2, H.HnswParams(16))
+    keys = np.arange(n, dtype=np.uint64)
+    chunk = 100_000
+    t0 = time.time()
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        idx.insert_batch(keys[s:e], data.base()[s:e], threads=threads)
+        log(f"graph build: {e}/{n} linked ({time.time() - t0:.1f} s, {threads} threads)")
+    build_s = time.time() - t0
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    sptr = ctypes.c_void_p(stream.cuda_stream)
+    q_dev = torch.from_numpy(data.queries()).to(dev)
+    out_k = torch.zeros((nq, k), dtype=torch.int64, device=dev)
+    out_d = torch.zeros((nq, k), dtype=torch.float32, device=dev)
+    out_c = torch.zeros(nq, dtype=torch.int32, device=dev)
+    status = torch.zeros(nq, dtype=torch.int32, device=dev)
+    log("uploading the index")
+    h = idx.device_index()
+
+    # ground truth: exact GPU brute force, checked against the FP64 oracle
+    log("ground truth: GPU brute force over all rows")
+    bf = H.BruteForceIndex(dim, H.Metric.L2)
+    bf.insert_batch(keys, data.base())
+    t = time.time()
+    tk, td, tc = bf.search_arrays(data.queries(), k)
+    bf_s = time.time() - t
+    f64n = min(args.f64_check, nq)
+    fk, fd, fc = bf.search_arrays(data.queries()[:f64n], k, f64=True)
+    f64_agree = float(np.mean([len(set(tk[i].tolist()) & set(fk[i].tolist())) / k
+                               for i in range(f64n)]))
+    log(f"brute force {nq} queries in {bf_s:.2f} s; recall of f32 truth vs FP64 oracle "
+        f"on {f64n} queries = {f64_agree:.5f}")
+    del bf
+
+    def run(ef, precision):
+        st = H.lib.hh_search_batch_device(
+            h, ctypes.c_void_p(q_dev.data_ptr()), nq, k, ef, int(precision), None,
+            ctypes.c_void_p(out_k.data_ptr()), ctypes.c_void_p(out_d.data_ptr()),
+            ctypes.c_void_p(out_c.data_ptr()), ctypes.c_void_p(status.data_ptr()), sptr)
+        if st:
+            raise RuntimeError(L.last_error())
+
+    def recall():
+        gk = out_k.cpu().numpy().view(np.uint64)
+        gc = out_c.cpu().numpy().view(np.uint32)
+        return H.lib.hh_recall_at_k(nq, tk.ctypes.data, tc.ctypes.data, k, gk.ctypes.data,
+                                    gc.ctypes.data, k, k)
+
+    def timed(ef, precision, steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            run(ef, precision)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    sweep = []
+    for ef in efs:
+        run(ef, H.Precision.F32)
+        r = recall()
+        sec = timed(ef, H.Precision.F32, 3)
+        sweep.append({"ef": ef, "recall": round(r, 4), "qps": round(nq / sec)})
+        log(f"f32 ef={ef}: recall@{k}={r:.4f} qps={nq / sec:,.0f}")
+    chosen = next((s for s in sweep if s["recall"] >= args.target), sweep[-1])
+    ef = chosen["ef"]
+
+    # timed region
+    for _ in range(args.warmup):
+        run(ef, H.Precision.F32)
+    torch.cuda.synchronize()
+    step_ms = []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        run(ef, H.Precision.F32)
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - t) * 1e3)
+    total = time.perf_counter() - t_all
+    qps = nq * args.steps / total
+    rec = recall()
+
+    # kernel time (HIP events on the launch stream) and algorithmic bytes
+    H.lib.hh_index_set_timing(h, 1)
+    kms = []
+    for _ in range(5):
+        run(ef, H.Precision.F32)
+        ms = ctypes.c_float()
+        H.lib.hh_index_last_kernel_ms(h, ctypes.byref(ms))
+        kms.append(ms.value)
+    H.lib.hh_index_set_timing(h, 0)
+    idx.search_stats(reset=True)
+    run(ef, H.Precision.F32)
+    torch.cuda.synchronize()
+    st = idx.search_stats()
+    n_dist = st["n_dist"] + st["n_upper"]
+    alg_bytes = n_dist * dim * 4 + st["n_rerank"] * dim * 4
+    adj_bytes = st["n_expand"] * 32 * 4
+    kern_ms = float(np.median(kms))
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    log(f"search kernel {kern_ms:.3f} ms/launch; {n_dist / nq:.0f} distance evaluations/query; "
+        f"{achieved:.0f} GB/s of gathered rows")
+
+    # precision variants at the same ef
+    variants = {}
+    idx.enable_bf16()
+    for name, prec in (("bf16", H.Precision.BF16),):
+        run(ef, prec)
+        r = recall()
+        sec = timed(ef, prec, max(3, args.steps // 2))
+        variants[name] = {"recall": round(r, 4), "qps": round(nq / sec)}
+        log(f"{name} ef={ef}: recall={r:.4f} qps={nq / sec:,.0f}")
+    log("int8: training the quantizer and re-uploading")
+    idx.enable_quantization()
+    h = idx.device_index()
+    run(ef, H.Precision.INT8)
+    r = recall()
+    sec = timed(ef, H.Precision.INT8, max(3, args.steps // 2))
+    variants["int8"] = {"recall": round(r, 4), "qps": round(nq / sec)}
+    log(f"int8 ef={ef}: recall={r:.4f} qps={nq / sec:,.0f}")
+
+    # CPU baseline: the oracle's restatement of search_batch on host cores
+    cpu = None
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        v = idx.graph_arrays()
+        g = O.OracleGraph(v["dim"], v["metric"], v["m"], v["m0"], v["vectors"], v["keys"],
+                          v["levels"], v["layer0"], v["upper_off"], v["upper"], v["entry_id"],
+                          v["top"])
+        ns = min(args.cpu_sample, nq)
+        sample = data.queries()[:ns]
+        g.search_batch(sample[:64], k, ef, threads=threads)
+        t = time.perf_counter()
+        ok, od, oc, _ = g.search_batch(sample, k, ef, threads=threads)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": round(ns / cpu_s), "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"{ns} of the {nq} queries, same graph, k={k}, ef={ef}, "
+                         f"{cpu_s:.1f} s on {threads} threads (oracle C restatement of "
+                         "search_batch with AVX2-order kernels)"}
+        log(f"cpu baseline {ns / cpu_s:,.0f} qps on {threads} threads")
+    except Exception as e:  # the baseline is reported, never required
+        log(f"cpu baseline unavailable: {e}")
+
+    line = {
+        "metric": "QPS at recall@10>=0.95, 1Mx128 L2 (GPU FP32 search_batch)",
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "hnsw search_batch, 1M x 128 clustered L2, M=16, efC=200, "
+                               f"{nq} queries, k={k}",
+                   "ef": ef, "recall_at_10": round(rec, 4), "n": n, "dim": dim,
+                   "queries": nq, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+        "step_ms_p50": round(float(np.percentile(step_ms, 50)), 3),
+        "step_ms_p99": round(float(np.percentile(step_ms, 99)), 3),
+        "search_kernel_ms": round(kern_ms, 4),
+        "dist_evals_per_query": round(n_dist / nq, 1),
+        "gathered_bytes_per_launch": alg_bytes,
+        "adjacency_bytes_per_launch": adj_bytes,
+        "sweep_f32": sweep,
+        "variants": variants,
+        "ground_truth": {"brute_force_s": round(bf_s, 3), "f64_agreement": round(f64_agree, 5)},
+        "graph_build_s": round(build_s, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

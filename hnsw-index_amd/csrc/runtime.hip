@@ -245,6 +245,9 @@ struct hh_index {
   float sq_scale = 1.0f;
   Workspace ws;
   hipStream_t stream = nullptr;  // used by the synchronous host-memory API
+  // optional timing of the search kernel (hh_index_set_timing)
+  bool timing = false;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
 
   ~hh_index() {
     (void)hipSetDevice(device);
@@ -258,6 +261,8 @@ struct hh_index {
     dfree(keys);
     dfree(deleted_bits);
     dfree(sq_offset);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -432,8 +437,10 @@ int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, 
   a.hash_limit = hash_cap / 2 + hash_cap / 4;
   a.lds_words = lds_words;
   void *args[] = {&a};
+  if (idx->timing) HH_HIP(hipEventRecord(idx->t0, stream));
   HH_HIP(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(wpb * 64), args, lds_words * 4 * wpb,
                          stream));
+  if (idx->timing) HH_HIP(hipEventRecord(idx->t1, stream));
   HH_HIP(hipEventRecord(ws.done, stream));
   return HH_OK;
 }
@@ -579,6 +586,26 @@ int hh_index_stats(hh_index *idx, hh_search_stats *out) {
   out->n_rerank = v[4];
   out->n_overflow = v[5];
   out->n_visited_spill = v[6];
+  return HH_OK;
+}
+
+int hh_index_set_timing(hh_index *idx, int on) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  HH_HIP(hipSetDevice(idx->device));
+  if (on && !idx->t0) {
+    HH_HIP(hipEventCreate(&idx->t0));
+    HH_HIP(hipEventCreate(&idx->t1));
+  }
+  idx->timing = on != 0;
+  return HH_OK;
+}
+
+int hh_index_last_kernel_ms(hh_index *idx, float *ms) {
+  if (!idx || !ms) return fail(HH_INVALID_ARGUMENT, "null argument");
+  if (!idx->timing) return fail(HH_INVALID_ARGUMENT, "timing is off (hh_index_set_timing)");
+  HH_HIP(hipSetDevice(idx->device));
+  HH_HIP(hipEventSynchronize(idx->t1));
+  HH_HIP(hipEventElapsedTime(ms, idx->t0, idx->t1));
   return HH_OK;
 }
 

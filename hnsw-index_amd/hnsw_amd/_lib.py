@@ -70,6 +70,8 @@ _sig("hh_index_update_deleted", INT, P, P, U64)
 _sig("hh_index_enable_bf16", INT, P)
 _sig("hh_index_stats", INT, P, ctypes.POINTER(SearchStats))
 _sig("hh_index_reset_stats", INT, P)
+_sig("hh_index_set_timing", INT, P, INT)
+_sig("hh_index_last_kernel_ms", INT, P, ctypes.POINTER(ctypes.c_float))
 _sig("hh_search_batch_device", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, P)
 _sig("hh_search_batch_spec_device", INT, P, P, U64, P, U32, P, U32, P, P, P, P, P)
 _sig("hh_search_batch", INT, P, P, U64, U32, U32, U32, P, P, P, P, ctypes.POINTER(U64))

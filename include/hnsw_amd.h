@@ -156,6 +156,11 @@ typedef struct hh_search_stats {
 } hh_search_stats;
 int hh_index_stats(hh_index *idx, hh_search_stats *out);
 int hh_index_reset_stats(hh_index *idx);
+/* Times the search kernel of every following launch with a pair of HIP
+ * events on the launch stream (no reference counterpart: measurement). */
+int hh_index_set_timing(hh_index *idx, int on);
+/* Milliseconds of the search kernel of the last timed launch (waits for it). */
+int hh_index_last_kernel_ms(hh_index *idx, float *ms);
 
 /*
  * Batched HNSW search (HnswIndex::search_batch / search_with /

@@ -1,0 +1,182 @@
+"""GPU HNSW search against the oracle on the same graph.
+
+The GPU kernel reproduces the crate's AVX2 summation order, so on graphs
+without distance ties every query must return the oracle's keys and
+bit-identical distances, and the kernel's distance-evaluation counters must
+equal the oracle's (src/hnsw/search.rs:81-142 evaluated step by step).
+"""
+import numpy as np
+import pytest
+
+import hnsw_amd as H
+import oracle as O
+from helpers import build, oracle_graph, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(idx, queries, k, ef, quantized=False, allow=None, precision=None):
+    g = oracle_graph(idx)
+    prec = precision if precision is not None else (H.Precision.INT8 if quantized else H.Precision.F32)
+    filt = None if allow is None else allow.astype(bool)
+    idx.search_stats(reset=True)
+    gk, gd, gc = idx.search_arrays(queries, k, ef, precision=prec, filter=filt)
+    stats = idx.search_stats()
+    ok, od, oc, cnt = g.search_batch(queries, k, ef, quantized=quantized,
+                                      allow=None if allow is None else allow.astype(np.uint8))
+    return (gk, gd, gc), (ok, od, oc), stats, cnt
+
+
+@pytest.mark.parametrize("metric", [H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine])
+@pytest.mark.parametrize("dim", [128, 96, 100, 37, 3, 256])
+def test_search_matches_oracle_bit_exact(metric, dim):
+    print(f"case metric={metric} dim={dim}", flush=True)
+    b, q = O.dataset_gaussian(3000, 64, dim, 7 + dim)
+    idx = build(b, metric)
+    for k, ef in [(10, 10), (10, 64), (1, 16), (32, 200)]:
+        (gk, gd, gc), (ok, od, oc), st, cnt = _compare(idx, q, k, ef)
+        assert np.array_equal(gc, oc)
+        assert np.array_equal(gk, ok), f"k={k} ef={ef}"
+        assert same_bits(gd, od), f"k={k} ef={ef}"
+        assert st["n_dist"] + st["n_upper"] == cnt.n_dist, (st, cnt.n_dist)
+        assert st["n_expand"] == cnt.n_expand
+        assert st["n_overflow"] == 0
+
+
+def test_search_batch_equals_individual_searches():
+    """tests/concurrency.rs:60-69"""
+    b, q = O.dataset_gaussian(2000, 50, 16, 5)
+    idx = build(b)
+    batch = idx.search_batch(q, 10, 40)
+    for i in range(len(q)):
+        assert batch[i] == idx.search(q[i], 10, 40)
+    with pytest.raises(H.DimensionMismatch):
+        idx.search_batch(q.ravel()[:15], 10, 40)
+
+
+def test_quantized_search_matches_oracle():
+    """int8 traversal + exact re-rank (src/hnsw/mod.rs:483-500)."""
+    for metric in (H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine):
+        b, q = O.dataset_gaussian(2000, 40, 32, 12)
+        idx = build(b, metric)
+        with pytest.raises(H.QuantizationDisabled):
+            idx.search_with(q[0], 10, H.SearchOptions(64).quantized(True))
+        idx.enable_quantization()
+        (gk, gd, gc), (ok, od, oc), st, cnt = _compare(idx, q, 10, 64, quantized=True)
+        assert np.array_equal(gk, ok) and same_bits(gd, od) and np.array_equal(gc, oc)
+        assert st["n_rerank"] == cnt.n_rerank
+
+
+def test_filter_matches_oracle_and_is_respected():
+    """src/hnsw/tests.rs:293-313"""
+    b, q = O.dataset_gaussian(2000, 10, 8, 8)
+    idx = build(b)
+    keys = np.arange(2000)
+    allow = (keys % 10 == 3)
+    (gk, gd, gc), (ok, od, oc), st, cnt = _compare(idx, q, 10, 64, allow=allow)
+    assert np.array_equal(gk, ok) and same_bits(gd, od) and np.array_equal(gc, oc)
+    assert (gk % 10 == 3).all() and (gc == 10).all()
+    bf = H.BruteForceIndex(8)
+    bf.insert_batch(keys, b)
+    for i in range(len(q)):
+        only = idx.search_filtered(q[i], 3, 10, lambda k: k == 1234)
+        assert [n.key for n in only] == [1234]
+        assert idx.search_filtered(q[i], 3, 10, lambda k: False) == []
+        assert only[0] == bf.search_filtered(q[i], 1, lambda k: k == 1234)[0]
+
+
+def test_tombstones_are_never_returned():
+    """src/hnsw/tests.rs:220-248, with the entry point deleted."""
+    b, q = O.dataset_gaussian(300, 5, 8, 9)
+    idx = build(b)
+    v = idx.graph_arrays()
+    entry_key = int(v["keys"][v["entry_id"]])
+    idx.search(q[0], 1, 10)  # uploads, so removal goes through the device update
+    for key in range(0, 300, 2):
+        assert idx.remove(key)
+    if idx.contains(entry_key):
+        idx.remove(entry_key)
+    g = oracle_graph(idx)
+    for i in range(len(q)):
+        hits = idx.search(q[i], 20, 50)
+        assert len(hits) == 20
+        assert all(h.key % 2 == 1 and h.key != entry_key for h in hits)
+        ok, od, _, _ = g.search(q[i], 20, 50)
+        assert [h.key for h in hits] == ok.tolist()
+
+
+def test_k_and_ef_edge_cases():
+    """src/hnsw/tests.rs:208-218, empty index (:88-95)."""
+    b, q = O.dataset_gaussian(100, 1, 4, 2)
+    idx = build(b)
+    assert idx.search(q[0], 0, 10) == []
+    assert len(idx.search(q[0], 10, 1)) == 10
+    assert len(idx.search(q[0], 500, 500)) == 100
+    empty = H.HnswIndex(3)
+    assert empty.search([0.0, 0.0, 0.0], 5, 10) == []
+
+
+def test_invalid_queries_report_the_crate_errors():
+    b, q = O.dataset_gaussian(100, 4, 4, 2)
+    idx = build(b, H.Metric.Cosine)
+    with pytest.raises(H.ZeroVector):
+        idx.search([0, 0, 0, 0], 1, 10)
+    with pytest.raises(H.NonFiniteVector):
+        idx.search([np.nan, 0, 0, 1], 1, 10)
+    with pytest.raises(H.NonFiniteVector):
+        idx.search([3e38, 3e38, 0, 1], 1, 10)
+    qq = q.copy()
+    qq[2, 1] = np.inf
+    with pytest.raises(H.NonFiniteVector) as e:
+        idx.search_batch(qq, 3, 10)
+    assert e.value.query == 2
+
+
+def test_complete_graph_with_simple_selection_is_exact():
+    """src/hnsw/tests.rs:113-127"""
+    b, q = O.dataset_gaussian(16, 20, 8, 3)
+    params = H.HnswParams(16).with_selection(H.NeighborSelection.simple())
+    idx = build(b, params=params)
+    bf = H.BruteForceIndex(8)
+    bf.insert_batch(np.arange(16), b)
+    for i in range(len(q)):
+        assert idx.search(q[i], 5, 5) == bf.search(q[i], 5)
+
+
+def test_inner_product_and_cosine_rank_correctly():
+    """src/hnsw/tests.rs:370-388"""
+    ip = H.HnswIndex(2, H.Metric.InnerProduct)
+    ip.insert_batch([1, 2, 3], np.array([1.0, 0.0, 3.0, 0.0, 0.0, 1.0]))
+    hits = ip.search([1.0, 0.0], 3, 10)
+    assert [h.key for h in hits] == [2, 1, 3]
+    assert hits[0].distance == -3.0
+    cos = H.HnswIndex(2, H.Metric.Cosine)
+    cos.insert_batch([1, 2], np.array([10.0, 0.0, 1.0, 1.0]))
+    hits = cos.search([0.1, 0.0], 2, 10)
+    assert hits[0].key == 1 and abs(hits[0].distance) < 1e-6
+    assert abs(hits[1].distance - (1.0 - 0.7071067811865476)) < 1e-6
+
+
+def test_bf16_traversal_reranks_exactly():
+    b, q = O.dataset_gaussian(3000, 64, 64, 21)
+    idx = build(b)
+    gk, gd, gc = idx.search_arrays(q, 10, 64, precision=H.Precision.BF16)
+    v = idx.graph_arrays()
+    key_row = {int(kk): i for i, kk in enumerate(v["keys"])}
+    for i in range(len(q)):
+        for j in range(int(gc[i])):
+            want = O.distance(O.L2, q[i], v["vectors"][key_row[int(gk[i, j])]])
+            assert same_bits([gd[i, j]], [want])
+    tk, td, tc = O.brute_search_batch(O.L2, np.arange(3000), b, q, 10)
+    fk, fd, fc = idx.search_arrays(q, 10, 64)
+    r32 = O.recall_at_k(tk, tc, fk, fc, 10)
+    r16 = O.recall_at_k(tk, tc, gk, gc, 10)
+    assert r16 >= r32 - 0.02, (r16, r32)
+
+
+def test_large_ef_spills_visited_set_and_stays_exact():
+    b, q = O.dataset_gaussian(20000, 8, 16, 31)
+    idx = build(b, threads=8)
+    (gk, gd, gc), (ok, od, oc), st, cnt = _compare(idx, q, 50, 512)
+    assert np.array_equal(gk, ok) and same_bits(gd, od)
+    assert st["n_overflow"] == 0
