@@ -818,6 +818,10 @@ int hh_synthetic(uint32_t kind, uint64_t n, uint64_t nq, uint32_t dim, uint32_t 
                        out_queries, categories);
 }
 
+float hh_cpu_distance(uint32_t metric, const float *a, const float *b, uint64_t n) {
+  return hh::distance(metric, a, b, n);
+}
+
 // eval::recall_at_k (src/eval.rs:25-47)
 double hh_recall_at_k(uint64_t nq, const uint64_t *truth, const uint32_t *truth_count,
                       uint32_t truth_stride, const uint64_t *found, const uint32_t *found_count,

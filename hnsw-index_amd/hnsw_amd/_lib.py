@@ -84,6 +84,7 @@ _sig("hh_brute_search_batch", INT, P, P, U64, U32, P, P, P, P, ctypes.POINTER(U6
 _sig("hh_brute_search_batch_f64", INT, P, P, U64, U32, P, P, P, ctypes.POINTER(U64))
 _sig("hh_synthetic", INT, U32, U64, U64, U32, U32, ctypes.c_double, ctypes.c_double, U64, INT,
      P, P, P)
+_sig("hh_cpu_distance", ctypes.c_float, U32, P, P, U64)
 _sig("hh_recall_at_k", ctypes.c_double, U64, P, P, U32, P, P, U32, U32)
 
 

@@ -254,6 +254,12 @@ int hh_synthetic(uint32_t kind, uint64_t n, uint64_t nq, uint32_t dim, uint32_t 
                  double sigma, double sigma_q, uint64_t seed, int threads, float *out_base,
                  float *out_queries, uint32_t *categories);
 
+/* Metric::distance with the CPU kernels Kernels::best() selects
+ * (src/metric.rs:35-41, src/kernels/mod.rs:174-186): the AVX2+FMA
+ * intrinsic sequence of src/kernels/x86.rs on x86 CPUs that have it, the
+ * portable kernels otherwise.  Used by the CPU graph build. */
+float hh_cpu_distance(uint32_t metric, const float *a, const float *b, uint64_t n);
+
 /* eval::recall_at_k (src/eval.rs:25-47) over padded result arrays. */
 double hh_recall_at_k(uint64_t nq, const uint64_t *truth, const uint32_t *truth_count,
                       uint32_t truth_stride, const uint64_t *found, const uint32_t *found_count,

@@ -1,0 +1,247 @@
+"""The oracle pinned against the reference crate's own known answers
+(tests/golden/reference_kats.json), closed forms and properties.  CPU only."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+KAT = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+
+def test_splitmix64_known_answers():
+    k = KAT["splitmix64"]
+    r = O.SplitMix64(k["seed"])
+    assert [r.next_u64(), r.next_u64()] == k["outputs"]
+
+
+def test_splitmix_is_vigna_closed_form():
+    # state += golden; output = mix(state) with Vigna's constants
+    s = 987654321
+    r = O.SplitMix64(s)
+    for _ in range(5):
+        s = (s + 0x9E3779B97F4A7C15) % 2**64
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) % 2**64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) % 2**64
+        assert r.next_u64() == z ^ (z >> 31)
+
+
+def test_gaussian_first_components():
+    k = KAT["gaussian_first_components"]
+    base, _ = O.dataset_gaussian(*k["args"])
+    assert np.allclose(base.ravel()[:3], k["values"], atol=k["tol"], rtol=0)
+
+
+def test_gaussian_is_box_muller_of_the_stream():
+    # closed form: u1 = 1 - next_f64, u2 = next_f64, sqrt(-2 ln u1) cos(2 pi u2)
+    r = O.SplitMix64(77)
+    s = O.SplitMix64(77)
+    for _ in range(50):
+        u1 = 1.0 - (s.next_u64() >> 11) * 2.0**-53
+        u2 = (s.next_u64() >> 11) * 2.0**-53
+        want = math.sqrt(-2.0 * math.log(u1)) * math.cos(2 * math.pi * u2)
+        assert abs(r.next_gaussian() - want) < 1e-12
+
+
+def test_gaussian_moments():
+    k = KAT["gaussian_moments"]
+    r = O.SplitMix64(k["seed"])
+    x = np.array([r.next_gaussian() for _ in range(k["n"])])
+    assert abs(x.mean()) < k["mean_tol"] and abs(x.var() - 1.0) < k["var_tol"]
+
+
+def test_next_below_stays_in_range_and_covers():
+    r = O.SplitMix64(3)
+    seen = {r.next_below(7) for _ in range(1000)}
+    assert seen == set(range(7))
+
+
+def test_clustered_is_deterministic_and_clustered():
+    a = O.dataset_clustered(100, 5, 8, 4, 1)
+    b = O.dataset_clustered(100, 5, 8, 4, 1)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    c = O.dataset_clustered(100, 5, 8, 4, 2)
+    assert not np.array_equal(a[0], c[0])
+    # 4 clusters with spread 0.35: every row is within ~5 spread of some other
+    # row of its cluster, far closer than unrelated N(0,1) centres
+    base = O.dataset_clustered(2000, 0, 16, 4, 9)[0]
+    d = ((base[:, None, :] - base[None, :200, :]) ** 2).sum(-1)
+    nearest = np.sort(d, axis=1)[:, 1]
+    assert np.median(nearest) < 16 * 0.35**2 * 2 * 1.5
+
+
+def test_scalar_and_doctest_kernels():
+    k = KAT["scalar_kernels"]
+    assert O.dot(k["a"], k["b"], O.SCALAR) == k["dot"]
+    assert O.l2(k["a"], k["b"], O.SCALAR) == k["l2"]
+    assert O.dot([], [], O.SCALAR) == 0.0
+    d = KAT["kernel_doctest"]
+    assert O.dot(d["a"], d["b"], O.SCALAR) == d["dot"]
+    assert O.l2(d["a"], d["b"], O.SCALAR) == d["l2"]
+    for be in (O.SCALAR, O.PORTABLE, O.AVX2):
+        assert O.dot([], [], be) == 0.0 and O.l2([], [], be) == 0.0
+
+
+def test_metric_definitions():
+    k = KAT["metric_definitions"]
+    assert O.distance(O.L2, k["a"], k["b"], O.SCALAR) == k["l2"]
+    ip = O.distance(O.IP, k["a"], k["b"], O.SCALAR)
+    assert ip == 0.0 and math.copysign(1.0, ip) == -1.0  # -0.0
+    assert O.distance(O.COSINE, k["a"], k["b"], O.SCALAR) == k["cos"]
+    assert O.distance(O.COSINE, k["a"], k["a"], O.SCALAR) == k["cos_self"]
+
+
+def test_exact_on_small_integers_every_backend():
+    """tests/kernels.rs:83-96"""
+    a = np.array([(i % 7) - 3 for i in range(133)], np.float32)
+    b = np.array([(i % 5) - 2 for i in range(133)], np.float32)
+    for be in (O.SCALAR, O.PORTABLE, O.AVX2):
+        assert O.dot(a, b, be) == float((a * b).sum())
+        assert O.l2(a, b, be) == float(((a - b) ** 2).sum())
+
+
+def _tol(terms, n):
+    return (n + 4) * np.finfo(np.float32).eps * float(np.abs(terms).sum()) + np.finfo(np.float32).tiny
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_backends_match_scalar_within_the_reference_tolerance(seed):
+    """tests/kernels.rs:11-71 (the proptest bound)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(0, 301))
+    a = rng.uniform(-100, 100, n).astype(np.float32)
+    b = rng.uniform(-100, 100, n).astype(np.float32)
+    wd, wl = O.dot(a, b, O.SCALAR), O.l2(a, b, O.SCALAR)
+    for be in (O.PORTABLE, O.AVX2):
+        assert abs(O.dot(a, b, be) - wd) <= _tol(a * b, n)
+        assert abs(O.l2(a, b, be) - wl) <= _tol((a - b) ** 2, n)
+        assert O.l2(a, b, be) >= 0
+
+
+def test_every_length_up_to_two_blocks():
+    """tests/kernels.rs:73-81"""
+    for n in range(81):
+        a = np.sin(np.arange(n) * 0.37).astype(np.float32)
+        b = np.cos(np.arange(n) * 0.11).astype(np.float32)
+        wd, wl = O.dot(a, b, O.SCALAR), O.l2(a, b, O.SCALAR)
+        for be in (O.PORTABLE, O.AVX2):
+            assert abs(O.dot(a, b, be) - wd) <= _tol(a * b, n)
+            assert abs(O.l2(a, b, be) - wl) <= _tol((a - b) ** 2, n)
+
+
+def test_int8_kernels_match_naive_sums():
+    """src/kernels/int8.rs:68-85"""
+    for n in range(80):
+        a = np.array([((i * 37) % 255) for i in range(n)], np.int64).astype(np.uint8).view(np.int8)
+        b = np.array([((i * 91 + 13) % 255) for i in range(n)], np.int64).astype(np.uint8).view(np.int8)
+        assert O.i8_dot(a, b) == int((a.astype(np.int64) * b.astype(np.int64)).sum())
+        assert O.i8_l2(a, b) == int(((a.astype(np.int64) - b.astype(np.int64)) ** 2).sum())
+    a = np.full(65536, 127, np.int8)
+    b = np.full(65536, -127, np.int8)
+    assert O.i8_dot(a, b) == -(127 * 127) * 65536
+    assert O.i8_l2(a, b) == 254 * 254 * 65536
+
+
+def test_prepare_known_answers_and_rejections():
+    """src/vector.rs:106-179"""
+    k = KAT["cosine_normalisation"]
+    st, v = O.prepare(O.COSINE, 2, k["input"])
+    assert st == 0 and np.allclose(v, k["expect"], atol=k["tol"], rtol=0)
+    st, v = O.prepare(O.COSINE, 2, k["input2"])
+    assert st == 0 and v.tolist() == k["expect2"]
+    cases = [([1.0], O.L2, 1), ([np.nan, 1.0], O.L2, 4), ([np.inf, 1.0], O.IP, 4),
+             ([3.4028235e38, 3.4028235e38], O.L2, 4), ([0.0, 0.0], O.COSINE, 5)]
+    for v, metric, want in cases:
+        assert O.prepare(metric, 2, v)[0] == want
+    assert O.prepare(O.L2, 2, [0.0, 0.0])[0] == 0
+    assert O.prepare(O.L2, 1, [1.3e19])[0] == 4
+
+
+def test_quantizer_known_answers():
+    """src/quantization.rs:125-145"""
+    k = KAT["quantizer"]
+    off, scale = O.sq_train(O.L2, np.array(k["vectors"], np.float32))
+    assert off.tolist() == k["offset"] and abs(scale - k["scale"]) < k["scale_tol"]
+    assert O.sq_encode(off, scale, k["encode"]).tolist() == k["codes"]
+    off, _ = O.sq_train(O.IP, np.array(k["vectors"], np.float32))
+    assert off.tolist() == [0.0, 0.0]
+    _, s = O.sq_train(O.L2, np.array(k["degenerate"], np.float32))
+    assert s == k["degenerate_scale"]
+
+
+def test_quantizer_tracks_exact_distances():
+    """src/quantization.rs:101-123"""
+    base, q = O.dataset_gaussian(500, 50, 32, 11)
+    for metric in (O.L2, O.IP):
+        off, sc = O.sq_train(metric, base)
+        codes = O.sq_encode_rows(off, sc, base)
+        qc = O.sq_encode(off, sc, q[0])
+        worst = 0.0
+        for i in range(500):
+            exact = O.distance(metric, q[0], base[i])
+            approx = O.sq_distance(metric, sc, qc, codes[i])
+            scale = exact if metric == O.L2 else 32.0
+            worst = max(worst, abs(exact - approx) / scale)
+        assert worst < 0.05
+
+
+def test_quantizer_rounds_half_away_from_zero():
+    off = np.zeros(3, np.float32)
+    assert O.sq_encode(off, 1.0, [0.5, -0.5, 2.5]).tolist() == [1, -1, 3]
+
+
+def test_brute_force_known_answers():
+    """src/brute.rs:211-262"""
+    k = KAT["brute_force"]
+    rows = np.array(k["rows"], np.float32).reshape(-1, 1)
+    keys, dists = O.brute_search(O.L2, k["keys"], rows, [k["query"]], 3)
+    assert keys.tolist() == k["top3"] and dists[0] == k["top_distance"]
+    allow = np.array([kk > 4 for kk in k["keys"]], np.uint8)
+    keys, _ = O.brute_search(O.L2, k["keys"], rows, [k["query"]], 2, allow=allow)
+    assert keys.tolist() == k["filtered_gt4_k2"]
+    assert len(O.brute_search(O.L2, k["keys"], rows, [k["query"]], 1 << 40)[0]) == 4
+    assert len(O.brute_search(O.L2, k["keys"], rows, [k["query"]], 0)[0]) == 0
+    tr = np.array(k["tie_rows"], np.float32).reshape(-1, 1)
+    assert O.brute_search(O.L2, k["tie_keys"], tr, [0.0], 3)[0].tolist() == k["tie_order"]
+    d = KAT["brute_doctest"]
+    keys, dists = O.brute_search(O.L2, d["keys"], np.array(d["rows"], np.float32), d["query"], 1)
+    assert keys[0] == d["key"] and dists[0] == d["distance"]
+
+
+def test_brute_f64_matches_numpy_f64():
+    base, q = O.dataset_gaussian(1000, 5, 24, 3)
+    keys = np.arange(1000)
+    for i in range(5):
+        k, d = O.brute_search_f64(O.L2, keys, base, q[i], 10)
+        ref = ((base.astype(np.float64) - q[i].astype(np.float64)) ** 2).sum(1)
+        order = np.lexsort((keys, ref))[:10]
+        assert k.tolist() == order.tolist()
+        assert np.allclose(d, ref[order], rtol=1e-12)
+
+
+def test_candidate_order_known_answer():
+    """src/hnsw/search.rs:169-178"""
+    import functools
+    k = KAT["candidate_order"]
+    items = [(float("nan") if d == "nan" else d, i) for d, i in k["pushed"]]
+    order = sorted(items, key=functools.cmp_to_key(
+        lambda a, b: O.candidate_cmp(a[0], a[1], b[0], b[1])))
+    assert [i for _, i in order] == k["order"]
+
+
+def test_recall_at_k_known_answers():
+    """src/eval.rs:59-72"""
+    k = KAT["recall_at_k"]
+    t = np.array(k["truth"], np.uint64)
+    f = np.array(k["found"], np.uint64)
+    c = np.array([3, 3], np.uint32)
+    assert abs(O.recall_at_k(t, c, f, c, 3) - k["k3"]) < 1e-12
+    assert O.recall_at_k(t, c, f, c, 1) == k["k1"]
+    one = np.array([[1]], np.uint64)
+    assert O.recall_at_k(one, [1], one, [1], 10) == 1.0
+    assert O.recall_at_k(np.zeros((0, 1), np.uint64), [], np.zeros((0, 1), np.uint64), [], 10) == 1.0
+    assert O.recall_at_k(np.array([[1, 2]], np.uint64), [2], np.zeros((1, 2), np.uint64), [0], 2) == 0.0
