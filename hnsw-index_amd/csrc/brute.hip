@@ -2,8 +2,8 @@
 // src/brute.rs:131-180).
 //
 // Stage 1 (brute_partial_kernel): a wavefront holds QB queries in
-// registers and streams one slice of the rows, two rows per step (one per
-// half-wavefront), computing every distance in the crate's AVX2 order and
+// registers and streams one slice of the rows, four rows per step (one per
+// 16-lane quarter), computing every distance in the crate's AVX2 order and
 // keeping a sorted (distance, key) top-k list per query in registers.
 // Stage 2 (brute_merge_kernel): one wavefront per query merges the slice
 // lists.  Ranking is the crate's (distance total order, key).
@@ -94,9 +94,9 @@ __device__ __forceinline__ void klist_offer(KList<D, S> &L, int &size, int kk, D
   }
 }
 
-template <int TMAX, int QB, int S>
+template <int TMAX, bool EXTRA, int QB, int S>
 __global__ void __launch_bounds__(256) brute_partial_kernel(const BruteArgs A) {
-  const int lane = lane_id(), j = lane & 31, half = lane >> 5;
+  const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
   const uint64_t work = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const uint64_t groups = (A.nq + QB - 1) / QB;
   if (work >= groups * A.nslices) return;
@@ -104,16 +104,17 @@ __global__ void __launch_bounds__(256) brute_partial_kernel(const BruteArgs A) {
   const uint64_t r0 = slice * A.rows_per_slice;
   const uint64_t r1 = r0 + A.rows_per_slice < A.n ? r0 + A.rows_per_slice : A.n;
   const Shape s{A.T, A.U, A.R, A.stride};
-  Lanes<TMAX> q[QB];
+  QLanes<TMAX, EXTRA> q[QB];
   int qvalid[QB];
 #pragma unroll
   for (int b = 0; b < QB; b++) {
 #pragma unroll
-    for (int t = 0; t < TMAX; t++) q[b].m[t] = 0.0f;
-    q[b].e[0] = q[b].e[1] = q[b].e[2] = q[b].r = 0.0f;
+    for (int t = 0; t < TMAX; t++) q[b].lo[t] = q[b].hi[t] = 0.0f;
+    q[b].e[0] = q[b].r = 0.0f;
+    if constexpr (EXTRA) q[b].e[1] = q[b].e[2] = 0.0f;
     const uint64_t qi = g * QB + b;
-    qvalid[b] = qi < A.nq && A.status[qi] == 0;
-    if (qi < A.nq) load_lanes<TMAX>(A.qprep + qi * A.stride, s, j, q[b]);
+    qvalid[b] = qi < A.nq && G(A.status)[qi] == 0;
+    if (qi < A.nq) load_q<TMAX, EXTRA, float>(A.qprep + qi * A.stride, s, i, q[b]);
   }
   KList<float, S> L[QB];
   int size[QB];
@@ -131,28 +132,33 @@ __global__ void __launch_bounds__(256) brute_partial_kernel(const BruteArgs A) {
     kth_k[b] = 0;
   }
   const int kk = (int)A.k;
-  for (uint64_t base = r0; base < r1; base += 2) {
-    const uint64_t row = base + half;
+  for (uint64_t base = r0; base < r1; base += 4) {
+    const uint64_t row = base + quarter;
     const bool rv = row < r1 && (!A.allow || bit_test(A.allow, (uint32_t)row));
-    Lanes<TMAX> x;
+    QLanes<TMAX, EXTRA> x;
     uint64_t key = 0;
     if (rv) {
-      load_lanes<TMAX>(A.rows + row * A.stride, s, j, x);
-      key = A.keys[row];
+      load_q<TMAX, EXTRA, float>(A.rows + row * A.stride, s, i, x);
+      key = G(A.keys)[row];
     }
-    const bool v0 = rl((uint32_t)rv, 0) != 0, v1 = rl((uint32_t)rv, 32) != 0;
-    const uint64_t k0 = rl64(key, 0), k1 = rl64(key, 32);
+    bool vq[4];
+    uint64_t kq[4];
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      vq[w] = rl((uint32_t)rv, 16 * w) != 0;
+      kq[w] = rl64(key, 16 * w);
+    }
 #pragma unroll
     for (int b = 0; b < QB; b++) {
-      float tail = 0.0f, acc = 0.0f;
+      float lo = 0.0f, hi = 0.0f, tail = 0.0f;
       if (rv) {
-        if (A.metric == 0) acc = lane_acc<TMAX, true>(q[b], x, s, j, tail);
-        else acc = lane_acc<TMAX, false>(q[b], x, s, j, tail);
+        if (A.metric == 0) quarter_acc<TMAX, EXTRA, true>(q[b], x, s, i, lo, hi, tail);
+        else quarter_acc<TMAX, EXTRA, false>(q[b], x, s, i, lo, hi, tail);
       }
-      const float h = finish_metric(half_reduce(acc, tail, s.R, half), A.metric);
-      const float d0 = rlf(h, 0), d1 = rlf(h, 32);
-      if (v0) klist_offer<float, S>(L[b], size[b], kk, kth_d[b], kth_k[b], d0, k0);
-      if (v1) klist_offer<float, S>(L[b], size[b], kk, kth_d[b], kth_k[b], d1, k1);
+      const float h = finish_metric(quarter_reduce<EXTRA>(lo, hi, tail, s.R, quarter), A.metric);
+#pragma unroll
+      for (int w = 0; w < 4; w++)
+        if (vq[w]) klist_offer<float, S>(L[b], size[b], kk, kth_d[b], kth_k[b], rlf(h, 16 * w), kq[w]);
     }
   }
 #pragma unroll
@@ -164,11 +170,11 @@ __global__ void __launch_bounds__(256) brute_partial_kernel(const BruteArgs A) {
     for (int t = 0; t < S; t++) {
       const int idx = lane * S + t;
       if (idx < size[b]) {
-        A.part_d[o + idx] = L[b].d[t];
-        A.part_k[o + idx] = L[b].key[t];
+        GM(A.part_d)[o + idx] = L[b].d[t];
+        GM(A.part_k)[o + idx] = L[b].key[t];
       }
     }
-    if (lane == 0) A.part_n[qi * A.nslices + slice] = (uint32_t)size[b];
+    if (lane == 0) GM(A.part_n)[qi * A.nslices + slice] = (uint32_t)size[b];
   }
 }
 
@@ -193,10 +199,10 @@ __global__ void __launch_bounds__(256) brute_merge_kernel(const BruteArgs A) {
   const int kk = (int)A.k;
   for (uint32_t sl = 0; sl < A.nslices; sl++) {
     const uint64_t o = (qi * A.nslices + sl) * (uint64_t)A.k;
-    const int cnt = (int)A.part_n[qi * A.nslices + sl];
+    const int cnt = (int)G(A.part_n)[qi * A.nslices + sl];
     for (int i = 0; i < cnt; i++) {
-      const float d = A.part_d[o + i];
-      const uint64_t key = A.part_k[o + i];
+      const float d = G(A.part_d)[o + i];
+      const uint64_t key = G(A.part_k)[o + i];
       if (!(size < kk || nless(d, key, kth_d, kth_k))) break;  // slice lists are sorted
       klist_offer<float, S>(L, size, kk, kth_d, kth_k, d, key);
     }
@@ -213,17 +219,17 @@ __global__ void __launch_bounds__(256) brute_merge_kernel(const BruteArgs A) {
   if (lane == 0) A.out_count[qi] = (uint32_t)size;
 }
 
-// FP64 oracle: one query per wave, all rows, f64 distances, same ranking.
+// FP64 oracle: one query per wave and slice, f64 distances, same ranking.
 template <int S>
 __global__ void __launch_bounds__(256) brute_f64_kernel(const BruteArgs A) {
-  const int lane = lane_id(), j = lane & 31, half = lane >> 5;
+  const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
   const uint64_t work = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (work >= A.nq * A.nslices) return;
   const uint64_t qi = work / A.nslices, slice = work % A.nslices;
-  if (A.status[qi] != 0) return;
+  if (G(A.status)[qi] != 0) return;
   const uint64_t r0 = slice * A.rows_per_slice;
   const uint64_t r1 = r0 + A.rows_per_slice < A.n ? r0 + A.rows_per_slice : A.n;
-  const float *qp = A.qprep + qi * A.stride;
+  gptr<float> qp = G(A.qprep) + qi * A.stride;
   KList<double, S> L;
 #pragma unroll
   for (int t = 0; t < S; t++) {
@@ -234,21 +240,15 @@ __global__ void __launch_bounds__(256) brute_f64_kernel(const BruteArgs A) {
   double kth_d = 0.0;
   uint64_t kth_k = 0;
   const int kk = (int)A.k;
-  for (uint64_t base = r0; base < r1; base += 2) {
-    const uint64_t row = base + half;
+  for (uint64_t base = r0; base < r1; base += 4) {
+    const uint64_t row = base + quarter;
     const bool rv = row < r1;
     double acc = 0.0;
     uint64_t key = 0;
     if (rv) {
-      const float *xp = A.rows + row * A.stride;
-      for (uint32_t e = j; e < A.dim; e += 32) {
-        // rows and queries share the layout; position of element e
-        uint32_t pos = e;
-        if (e < 32u * (uint32_t)A.T) {
-          uint32_t t = e >> 5, jj = e & 31, b = t >> 2;
-          uint32_t tw = (uint32_t)A.T - 4 * b < 4 ? (uint32_t)A.T - 4 * b : 4;
-          pos = b * 128 + jj * tw + (t & 3);
-        }
+      gptr<float> xp = G(A.rows) + row * A.stride;
+      // rows and queries share the layout, so a position is a position
+      for (uint32_t pos = i; pos < A.dim; pos += 16) {
         const double a = (double)qp[pos], c = (double)xp[pos];
         if (A.metric == 0) {
           const double d = a - c;
@@ -257,27 +257,26 @@ __global__ void __launch_bounds__(256) brute_f64_kernel(const BruteArgs A) {
           acc += a * c;
         }
       }
-      key = A.keys[row];
+      key = G(A.keys)[row];
     }
 #pragma unroll
-    for (int m = 1; m < 32; m <<= 1) acc += __shfl_xor(acc, m, 64);
-    double dist = A.metric == 0 ? acc : (A.metric == 1 ? -acc : 1.0 - acc);
-    const bool v0 = rl((uint32_t)rv, 0) != 0, v1 = rl((uint32_t)rv, 32) != 0;
-    const double d0 = rld(dist, 0), d1 = rld(dist, 32);
-    const uint64_t k0 = rl64(key, 0), k1 = rl64(key, 32);
-    if (v0) klist_offer<double, S>(L, size, kk, kth_d, kth_k, d0, k0);
-    if (v1) klist_offer<double, S>(L, size, kk, kth_d, kth_k, d1, k1);
+    for (int m = 1; m < 16; m <<= 1) acc += __shfl_xor(acc, m, 64);
+    const double dist = A.metric == 0 ? acc : (A.metric == 1 ? -acc : 1.0 - acc);
+#pragma unroll
+    for (int w = 0; w < 4; w++)
+      if (rl((uint32_t)rv, 16 * w))
+        klist_offer<double, S>(L, size, kk, kth_d, kth_k, rld(dist, 16 * w), rl64(key, 16 * w));
   }
   const uint64_t o = (qi * A.nslices + slice) * (uint64_t)A.k;
 #pragma unroll
   for (int t = 0; t < S; t++) {
     const int idx = lane * S + t;
     if (idx < size) {
-      A.part_d64[o + idx] = L.d[t];
-      A.part_k[o + idx] = L.key[t];
+      GM(A.part_d64)[o + idx] = L.d[t];
+      GM(A.part_k)[o + idx] = L.key[t];
     }
   }
-  if (lane == 0) A.part_n[qi * A.nslices + slice] = (uint32_t)size;
+  if (lane == 0) GM(A.part_n)[qi * A.nslices + slice] = (uint32_t)size;
 }
 
 template <int S>
@@ -301,10 +300,10 @@ __global__ void __launch_bounds__(256) brute_merge_f64_kernel(const BruteArgs A)
   const int kk = (int)A.k;
   for (uint32_t sl = 0; sl < A.nslices; sl++) {
     const uint64_t o = (qi * A.nslices + sl) * (uint64_t)A.k;
-    const int cnt = (int)A.part_n[qi * A.nslices + sl];
+    const int cnt = (int)G(A.part_n)[qi * A.nslices + sl];
     for (int i = 0; i < cnt; i++) {
-      const double d = A.part_d64[o + i];
-      const uint64_t key = A.part_k[o + i];
+      const double d = G(A.part_d64)[o + i];
+      const uint64_t key = G(A.part_k)[o + i];
       if (!(size < kk || kless<double>(d, key, kth_d, kth_k))) break;
       klist_offer<double, S>(L, size, kk, kth_d, kth_k, d, key);
     }
@@ -321,14 +320,17 @@ __global__ void __launch_bounds__(256) brute_merge_f64_kernel(const BruteArgs A)
   if (lane == 0) A.out_count[qi] = (uint32_t)size;
 }
 
-#define HH_BRUTE_INST(TM, QB)                                                     \
-  template __global__ void brute_partial_kernel<TM, QB, 1>(const BruteArgs);     \
-  template __global__ void brute_partial_kernel<TM, QB, 2>(const BruteArgs);     \
-  template __global__ void brute_partial_kernel<TM, QB, 4>(const BruteArgs);     \
-  template __global__ void brute_partial_kernel<TM, QB, 8>(const BruteArgs);
-HH_BRUTE_INST(4, 4)
-HH_BRUTE_INST(8, 2)
-HH_BRUTE_INST(32, 1)
+#define HH_BRUTE_INST(TM, EX, QB)                                                     \
+  template __global__ void brute_partial_kernel<TM, EX, QB, 1>(const BruteArgs);     \
+  template __global__ void brute_partial_kernel<TM, EX, QB, 2>(const BruteArgs);     \
+  template __global__ void brute_partial_kernel<TM, EX, QB, 4>(const BruteArgs);     \
+  template __global__ void brute_partial_kernel<TM, EX, QB, 8>(const BruteArgs);
+HH_BRUTE_INST(4, false, 4)
+HH_BRUTE_INST(4, true, 4)
+HH_BRUTE_INST(8, false, 2)
+HH_BRUTE_INST(8, true, 2)
+HH_BRUTE_INST(32, false, 1)
+HH_BRUTE_INST(32, true, 1)
 template __global__ void brute_merge_kernel<1>(const BruteArgs);
 template __global__ void brute_merge_kernel<2>(const BruteArgs);
 template __global__ void brute_merge_kernel<4>(const BruteArgs);

@@ -21,7 +21,7 @@ struct BruteArgs {
   uint32_t k;               // clamped to n
   uint32_t k_stride;
   uint32_t nslices;
-  uint64_t rows_per_slice;  // even
+  uint64_t rows_per_slice;  // multiple of 4
   float *part_d;            // nq x nslices x k
   double *part_d64;
   uint64_t *part_k;
@@ -32,7 +32,7 @@ struct BruteArgs {
   uint32_t *out_count;
 };
 
-template <int TMAX, int QB, int S>
+template <int TMAX, bool EXTRA, int QB, int S>
 __global__ void brute_partial_kernel(const BruteArgs A);
 template <int S>
 __global__ void brute_merge_kernel(const BruteArgs A);

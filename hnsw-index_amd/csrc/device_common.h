@@ -1,8 +1,12 @@
 // device_common.h — wave-level building blocks shared by the gfx950 kernels:
-// the exact-order f32 row distance, sorted register lists, LDS visited set.
+// the exact-order row distance, sorted register lists, the LDS visited set.
 //
-// Every helper assumes a 64-lane wavefront.  A "half" is lanes 0-31 or
-// 32-63; the row-distance helpers give each half its own row.
+// Every helper assumes a 64-lane wavefront.  Row distances are computed by
+// "quarters": the four 16-lane DPP rows of the wave each take one vector
+// row, and lane i of a quarter owns the crate's AVX2 accumulator lanes
+// j = i and j = i + 16 (src/kernels/x86.rs: accumulator c, lane l is
+// j = 8c + l).  The crate's fixed reduction tree then maps onto two DPP
+// row rotations, one in-lane add and two DPP quad permutations.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,6 +15,29 @@
 namespace hhd {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+
+// Global-address-space views of device pointers: loads through them compile
+// to global_load (vector memory counter only), not flat_load.  The host
+// pass of the compiler sees plain pointers (it never runs this code).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = __attribute__((address_space(1))) const T *;
+template <class T>
+using gmut = __attribute__((address_space(1))) T *;
+#else
+template <class T>
+using gptr = const T *;
+template <class T>
+using gmut = T *;
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> G(const T *p) {
+  return (gptr<T>)p;
+}
+template <class T>
+__device__ __forceinline__ gmut<T> GM(T *p) {
+  return (gmut<T>)p;
+}
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -37,7 +64,17 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
-__device__ __forceinline__ float shx(float v, int m) { return __shfl_xor(v, m, 64); }
+
+// DPP lane moves within a 16-lane row.
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dppi(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+constexpr int kRor8 = 0x128, kRor4 = 0x124, kXor1 = 0xB1, kXor2 = 0x4E;
 
 // f32::total_cmp key (the order of src/hnsw/search.rs:36-44 and
 // src/neighbor.rs:17-21).
@@ -57,142 +94,164 @@ __device__ __forceinline__ bool nless(float da, uint64_t ka, float db, uint64_t 
 }
 
 // ------------------------------------------------------------------ rows
-// A vector as one lane of a half-wavefront holds it: lane j (0..31) keeps
-// values 32t + j for t < T (the AVX2 accumulator-j stream), lanes j < 8 keep
-// the 8-wide remainder values 32T + 8u + j, lanes j < R the scalar tail.
-template <int TMAX>
-struct Lanes {
-  float m[TMAX];
-  float e[3];
+// dim = 32T + 8U + R.  Rows are stored accumulator-major: value 32t + j at
+//   (t/4)*128 + j*tw + t%4,  tw = min(4, T - 4*(t/4)),
+// then the 8U + R remainder values in natural order.
+struct Shape {
+  int T, U, R;
+  uint32_t stride;  // elements per row
+};
+
+// One quarter-lane's share of a vector: values 32t + i (lo) and
+// 32t + i + 16 (hi) for t < T, the remainder values 32T + 8u + i for i < 8,
+// the tail value 32T + 8U + i for i < R.
+template <int TMAX, bool EXTRA>
+struct QLanes {
+  float lo[TMAX], hi[TMAX];
+  float e[EXTRA ? 3 : 1];
   float r;
 };
 
-struct Shape {
-  int T, U, R;       // dim = 32T + 8U + R
-  uint32_t stride;   // floats per row
-};
+template <class T>
+__device__ __forceinline__ float to_f(T v);
+template <>
+__device__ __forceinline__ float to_f<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float to_f<uint16_t>(uint16_t v) {
+  return __uint_as_float((uint32_t)v << 16);
+}
 
-// Loads the lane's share of the row at p (accumulator-major layout).
-template <int TMAX>
-__device__ __forceinline__ void load_lanes(const float *__restrict__ p, const Shape &s, int j,
-                                           Lanes<TMAX> &x) {
+// Loads lane i's share of the row at p (f32 or bf16 elements).
+template <int TMAX, bool EXTRA, class E>
+__device__ __forceinline__ void load_q(const E *p_, const Shape &s, int i,
+                                       QLanes<TMAX, EXTRA> &x) {
+  gptr<E> p = G(p_);
 #pragma unroll
   for (int b = 0; b < (TMAX + 3) / 4; b++) {
     const int t0 = 4 * b;
     if (t0 < s.T) {
       const int tw = s.T - t0 < 4 ? s.T - t0 : 4;
-      const float *blk = p + b * 128;
+      gptr<E> blk = p + b * 128;
       if (tw == 4) {
-        float4 v = *reinterpret_cast<const float4 *>(blk + 4 * j);
-        x.m[t0] = v.x;
-        if (t0 + 1 < TMAX) x.m[t0 + 1] = v.y;
-        if (t0 + 2 < TMAX) x.m[t0 + 2] = v.z;
-        if (t0 + 3 < TMAX) x.m[t0 + 3] = v.w;
-      } else if (tw == 2) {
-        float2 v = *reinterpret_cast<const float2 *>(blk + 2 * j);
-        x.m[t0] = v.x;
-        if (t0 + 1 < TMAX) x.m[t0 + 1] = v.y;
-      } else if (tw == 3) {
-        x.m[t0] = blk[3 * j];
-        if (t0 + 1 < TMAX) x.m[t0 + 1] = blk[3 * j + 1];
-        if (t0 + 2 < TMAX) x.m[t0 + 2] = blk[3 * j + 2];
+        if constexpr (sizeof(E) == 4) {
+          const float4 a = *(gptr<float4>)(blk + 4 * i);
+          const float4 c = *(gptr<float4>)(blk + 64 + 4 * i);
+          x.lo[t0] = a.x;
+          x.hi[t0] = c.x;
+          if (t0 + 1 < TMAX) { x.lo[t0 + 1] = a.y; x.hi[t0 + 1] = c.y; }
+          if (t0 + 2 < TMAX) { x.lo[t0 + 2] = a.z; x.hi[t0 + 2] = c.z; }
+          if (t0 + 3 < TMAX) { x.lo[t0 + 3] = a.w; x.hi[t0 + 3] = c.w; }
+        } else {
+          const uint2 a = *(gptr<uint2>)(blk + 4 * i);
+          const uint2 c = *(gptr<uint2>)(blk + 64 + 4 * i);
+          x.lo[t0] = __uint_as_float(a.x << 16);
+          x.hi[t0] = __uint_as_float(c.x << 16);
+          if (t0 + 1 < TMAX) {
+            x.lo[t0 + 1] = __uint_as_float(a.x & 0xFFFF0000u);
+            x.hi[t0 + 1] = __uint_as_float(c.x & 0xFFFF0000u);
+          }
+          if (t0 + 2 < TMAX) {
+            x.lo[t0 + 2] = __uint_as_float(a.y << 16);
+            x.hi[t0 + 2] = __uint_as_float(c.y << 16);
+          }
+          if (t0 + 3 < TMAX) {
+            x.lo[t0 + 3] = __uint_as_float(a.y & 0xFFFF0000u);
+            x.hi[t0 + 3] = __uint_as_float(c.y & 0xFFFF0000u);
+          }
+        }
       } else {
-        x.m[t0] = blk[j];
+#pragma unroll
+        for (int tt = 0; tt < 3; tt++)
+          if (tt < tw && t0 + tt < TMAX) {
+            x.lo[t0 + tt] = to_f<E>(blk[tw * i + tt]);
+            x.hi[t0 + tt] = to_f<E>(blk[tw * (i + 16) + tt]);
+          }
       }
     }
   }
-  const float *rest = p + 32 * s.T;
+  if constexpr (EXTRA) {
+    gptr<E> rest = p + 32 * s.T;
 #pragma unroll
-  for (int u = 0; u < 3; u++)
-    if (u < s.U && j < 8) x.e[u] = rest[8 * u + j];
-  if (j < s.R) x.r = rest[8 * s.U + j];
-}
-
-// bf16 form of load_lanes: same positions, 2-byte values widened to f32.
-__device__ __forceinline__ float bf16f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-template <int TMAX>
-__device__ __forceinline__ void load_lanes_bf16(const uint16_t *__restrict__ p, const Shape &s,
-                                                int j, Lanes<TMAX> &x) {
-#pragma unroll
-  for (int b = 0; b < (TMAX + 3) / 4; b++) {
-    const int t0 = 4 * b;
-    if (t0 < s.T) {
-      const int tw = s.T - t0 < 4 ? s.T - t0 : 4;
-      const uint16_t *blk = p + b * 128;
-      if (tw == 4) {
-        uint2 v = *reinterpret_cast<const uint2 *>(blk + 4 * j);
-        x.m[t0] = __uint_as_float(v.x << 16);
-        if (t0 + 1 < TMAX) x.m[t0 + 1] = __uint_as_float(v.x & 0xFFFF0000u);
-        if (t0 + 2 < TMAX) x.m[t0 + 2] = __uint_as_float(v.y << 16);
-        if (t0 + 3 < TMAX) x.m[t0 + 3] = __uint_as_float(v.y & 0xFFFF0000u);
-      } else {
-#pragma unroll
-        for (int tt = 0; tt < 4; tt++)
-          if (tt < tw && t0 + tt < TMAX) x.m[t0 + tt] = bf16f(blk[tw * j + tt]);
-      }
-    }
+    for (int u = 0; u < 3; u++)
+      if (u < s.U && i < 8) x.e[u] = to_f<E>(rest[8 * u + i]);
+    if (i < s.R) x.r = to_f<E>(rest[8 * s.U + i]);
   }
-  const uint16_t *rest = p + 32 * s.T;
-#pragma unroll
-  for (int u = 0; u < 3; u++)
-    if (u < s.U && j < 8) x.e[u] = bf16f(rest[8 * u + j]);
-  if (j < s.R) x.r = bf16f(rest[8 * s.U + j]);
 }
 
-// Exact-order partial for one half: returns the accumulator of lane j
-// (src/kernels/x86.rs:47-69 / :96-120) and the lane's tail product.
-template <int TMAX, bool L2>
-__device__ __forceinline__ float lane_acc(const Lanes<TMAX> &q, const Lanes<TMAX> &x,
-                                          const Shape &s, int j, float &tail) {
-  float acc = 0.0f;
+// Lane i's accumulators j = i (lo) and j = i + 16 (hi) of the AVX2 kernel
+// (src/kernels/x86.rs:47-69 dot, :96-120 l2): one fused multiply-add per
+// element in t order, the 8-wide remainder into accumulator lanes 0..7,
+// and the unfused tail product (x86.rs:75-77, 126-129).
+template <int TMAX, bool EXTRA, bool L2>
+__device__ __forceinline__ void quarter_acc(const QLanes<TMAX, EXTRA> &q,
+                                            const QLanes<TMAX, EXTRA> &x, const Shape &s, int i,
+                                            float &lo, float &hi, float &tail) {
+  lo = 0.0f;
+  hi = 0.0f;
 #pragma unroll
   for (int t = 0; t < TMAX; t++) {
     if (t < s.T) {
       if (L2) {
-        float d = q.m[t] - x.m[t];
-        acc = __builtin_fmaf(d, d, acc);
+        const float dl = q.lo[t] - x.lo[t], dh = q.hi[t] - x.hi[t];
+        lo = __builtin_fmaf(dl, dl, lo);
+        hi = __builtin_fmaf(dh, dh, hi);
       } else {
-        acc = __builtin_fmaf(q.m[t], x.m[t], acc);
-      }
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < 3; u++) {
-    if (u < s.U && j < 8) {
-      if (L2) {
-        float d = q.e[u] - x.e[u];
-        acc = __builtin_fmaf(d, d, acc);
-      } else {
-        acc = __builtin_fmaf(q.e[u], x.e[u], acc);
+        lo = __builtin_fmaf(q.lo[t], x.lo[t], lo);
+        hi = __builtin_fmaf(q.hi[t], x.hi[t], hi);
       }
     }
   }
   tail = 0.0f;
-  if (j < s.R) {
-    if (L2) {
-      float d = q.r - x.r;
-      tail = __fmul_rn(d, d);
-    } else {
-      tail = __fmul_rn(q.r, x.r);
+  if constexpr (EXTRA) {
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+      if (u < s.U && i < 8) {
+        if (L2) {
+          const float d = q.e[u] - x.e[u];
+          lo = __builtin_fmaf(d, d, lo);
+        } else {
+          lo = __builtin_fmaf(q.e[u], x.e[u], lo);
+        }
+      }
+    }
+    if (i < s.R) {
+      if (L2) {
+        const float d = q.r - x.r;
+        tail = __fmul_rn(d, d);
+      } else {
+        tail = __fmul_rn(q.r, x.r);
+      }
     }
   }
-  return acc;
 }
 
-// The AVX2 lane combination s[l] = (a0+a1)+(a2+a3), then
+// The crate's lane combination: s[l] = (acc0+acc1)+(acc2+acc3) — lane l
+// holds acc_l (= acc0[l]) and acc_{l+16} (= acc2[l]), and acc_{l+8} /
+// acc_{l+24} sit 8 lanes away — then
 // h = ((s0+s4)+(s1+s5))+((s2+s6)+(s3+s7)) (src/kernels/x86.rs:21-27,70-73),
-// then the unfused scalar tail in order (x86.rs:75-77).  Every lane of the
-// half ends with the same bits (IEEE addition is commutative).  Must be
-// called by all 64 lanes.
-__device__ __forceinline__ float half_reduce(float acc, float tail, int R, int half) {
-  acc = __fadd_rn(acc, shx(acc, 8));
-  acc = __fadd_rn(acc, shx(acc, 16));
-  acc = __fadd_rn(acc, shx(acc, 4));
-  acc = __fadd_rn(acc, shx(acc, 1));
-  acc = __fadd_rn(acc, shx(acc, 2));
-  for (int r = 0; r < R; r++) acc = __fadd_rn(acc, __shfl(tail, (half << 5) | r, 64));
-  return acc;
+// then the tail values added one by one in order.  Every lane of a quarter
+// ends with identical bits (IEEE addition commutes).  All lanes must call.
+template <bool EXTRA>
+__device__ __forceinline__ float quarter_reduce(float lo, float hi, float tail, int R, int quarter) {
+  const float a = __fadd_rn(lo, dpp<kRor8>(lo));
+  const float b = __fadd_rn(hi, dpp<kRor8>(hi));
+  float s = __fadd_rn(a, b);
+  s = __fadd_rn(s, dpp<kRor4>(s));
+  s = __fadd_rn(s, dpp<kXor1>(s));
+  s = __fadd_rn(s, dpp<kXor2>(s));
+  if constexpr (EXTRA) {
+    for (int r = 0; r < R; r++) s = __fadd_rn(s, __shfl(tail, (quarter << 4) | r, 64));
+  }
+  return s;
+}
+
+// Integer and approximate sums: any order (the same tree is used).
+__device__ __forceinline__ int quarter_sum_i(int v) {
+  v += dppi<kRor8>(v);
+  v += dppi<kRor4>(v);
+  v += dppi<kXor1>(v);
+  v += dppi<kXor2>(v);
+  return v;
 }
 
 __device__ __forceinline__ float finish_metric(float sum, uint32_t metric) {
@@ -210,7 +269,6 @@ struct SList {
   uint32_t id[S];
 };
 
-// Position of (nd, nid) in the first `size` elements.
 template <int S>
 __device__ __forceinline__ int slist_rank(const SList<S> &L, int size, float nd, uint32_t nid) {
   const int lane = lane_id();
@@ -223,18 +281,18 @@ __device__ __forceinline__ int slist_rank(const SList<S> &L, int size, float nd,
   return p;
 }
 
-// Inserts at position p (< cap), shifting the tail right; the element at
-// index cap-1 falls off when the list was full.
+// Inserts at position p, shifting the tail right; the element at index
+// 64*S - 1 falls off.
 template <int S>
 __device__ __forceinline__ void slist_insert_at(SList<S> &L, int p, float nd, uint32_t nid) {
   const int lane = lane_id();
-  float upd = __shfl_up(L.d[S - 1], 1, 64);
-  uint32_t upi = (uint32_t)__shfl_up((int)L.id[S - 1], 1, 64);
+  const float upd = __shfl_up(L.d[S - 1], 1, 64);
+  const uint32_t upi = (uint32_t)__shfl_up((int)L.id[S - 1], 1, 64);
 #pragma unroll
   for (int s = S - 1; s >= 0; s--) {
     const int idx = lane * S + s;
-    float pd = s > 0 ? L.d[s > 0 ? s - 1 : 0] : upd;
-    uint32_t pi = s > 0 ? L.id[s > 0 ? s - 1 : 0] : upi;
+    const float pd = s > 0 ? L.d[s > 0 ? s - 1 : 0] : upd;
+    const uint32_t pi = s > 0 ? L.id[s > 0 ? s - 1 : 0] : upi;
     if (idx > p) {
       L.d[s] = pd;
       L.id[s] = pi;
@@ -245,11 +303,10 @@ __device__ __forceinline__ void slist_insert_at(SList<S> &L, int p, float nd, ui
   }
 }
 
-// Removes element 0.
 template <int S>
 __device__ __forceinline__ void slist_pop_front(SList<S> &L) {
-  float dn = __shfl_down(L.d[0], 1, 64);
-  uint32_t in = (uint32_t)__shfl_down((int)L.id[0], 1, 64);
+  const float dn = __shfl_down(L.d[0], 1, 64);
+  const uint32_t in = (uint32_t)__shfl_down((int)L.id[0], 1, 64);
 #pragma unroll
   for (int s = 0; s < S; s++) {
     L.d[s] = s < S - 1 ? L.d[s < S - 1 ? s + 1 : 0] : dn;
@@ -257,7 +314,6 @@ __device__ __forceinline__ void slist_pop_front(SList<S> &L) {
   }
 }
 
-// Element i (wave-uniform i).
 template <int S>
 __device__ __forceinline__ void slist_get(const SList<S> &L, int i, float &d, uint32_t &id) {
   const int ln = i / S, sl = i % S;
@@ -287,26 +343,28 @@ __device__ __forceinline__ int slist_count_not_above(const SList<S> &L, int size
 }
 
 // ------------------------------------------------------- visited set
-// Open-addressing hash of node ids in LDS (exact: no false positives or
-// negatives).  Returns true when id was not present and is now inserted.
-__device__ __forceinline__ bool lds_visit(uint32_t *table, uint32_t mask, int shift, uint32_t id) {
-  uint32_t h = (id * 0x9E3779B1u) >> shift;
+// Open-addressing hash of node ids in LDS, exact (no false positives or
+// negatives), any capacity (Lemire range reduction), linear probing.
+// Returns true when id was absent and is now inserted.
+__device__ __forceinline__ bool lds_visit(uint32_t *table, uint32_t cap, uint32_t id) {
+  uint32_t h = __umulhi(id * 0x9E3779B1u, cap);
   for (;;) {
-    uint32_t prev = atomicCAS(&table[h], kInvalid, id);
+    const uint32_t prev = atomicCAS(&table[h], kInvalid, id);
     if (prev == kInvalid) return true;
     if (prev == id) return false;
-    h = (h + 1) & mask;
+    h = h + 1 == cap ? 0 : h + 1;
   }
 }
 
 __device__ __forceinline__ bool bit_visit(uint32_t *bits, uint32_t id) {
-  uint32_t b = 1u << (id & 31);
-  uint32_t old = atomicOr(&bits[id >> 5], b);
+  const uint32_t b = 1u << (id & 31);
+  const uint32_t old = __hip_atomic_fetch_or(bits + (id >> 5), b, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
   return (old & b) == 0;
 }
 
 __device__ __forceinline__ bool bit_test(const uint32_t *bits, uint32_t id) {
-  return (bits[id >> 5] >> (id & 31)) & 1u;
+  return (G(bits)[id >> 5] >> (id & 31)) & 1u;
 }
 
 }  // namespace hhd

@@ -30,6 +30,13 @@ int synthetic(uint32_t kind, uint64_t n, uint64_t nq, uint32_t dim, uint32_t clu
               float *queries, uint32_t *categories);
 int graph_view(const hh_graph *g, hh_graph_desc *d);
 
+// Facts about an uploaded index used by the serving loop.
+int index_device(const hh_index *idx);
+uint32_t index_dim(const hh_index *idx);
+uint64_t index_nodes(const hh_index *idx);
+// Wavefronts the search kernel keeps resident on the whole GPU at ef <= 64.
+uint32_t index_resident_waves(hh_index *idx);
+
 // Row layout in HBM.  A dimension splits as D = 32*T + 8*U + R (the block
 // structure of the crate's AVX2 kernel, src/kernels/x86.rs:47-77).  The
 // first 32*T values are stored "accumulator-major": value 32*t + j (the one

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_set>
@@ -243,7 +245,14 @@ struct hh_index {
   uint32_t *deleted_bits = nullptr;
   float *sq_offset = nullptr;
   float sq_scale = 1.0f;
-  Workspace ws;
+  // Workspaces: one search runs in a workspace at a time; calls on
+  // different streams take different workspaces, so their kernels overlap.
+  std::mutex pool_mu;
+  std::condition_variable pool_cv;
+  std::vector<std::unique_ptr<Workspace>> pool;
+  std::vector<bool> pool_busy;
+  Workspace stats_ws;  // holds the stats counters shared by every launch
+  uint32_t tuned_hash_cap = 0;  // set by hh_index_autotune (0 = rule of thumb)
   hipStream_t stream = nullptr;  // used by the synchronous host-memory API
   // optional timing of the search kernel (hh_index_set_timing)
   bool timing = false;
@@ -251,7 +260,8 @@ struct hh_index {
 
   ~hh_index() {
     (void)hipSetDevice(device);
-    ws.release();
+    for (auto &w : pool) w->release();
+    stats_ws.release();
     dfree(rows);
     dfree(bf16);
     dfree(codes);
@@ -295,22 +305,25 @@ int pack_deleted(const uint8_t *deleted, uint64_t n, std::vector<uint32_t> &bits
   return HH_OK;
 }
 
-template <int TMAX, int SW>
+template <int TMAX, bool EX, int SW>
 void *search_kernel_ptr() {
-  return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, SW>);
+  return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, SW>);
 }
 
-void *pick_search_kernel(int tmax, int sw) {
-#define HH_PICK(TM)                                     \
-  if (tmax == TM) {                                     \
-    if (sw == 1) return search_kernel_ptr<TM, 1>();     \
-    if (sw == 2) return search_kernel_ptr<TM, 2>();     \
-    if (sw == 4) return search_kernel_ptr<TM, 4>();     \
-    if (sw == 8) return search_kernel_ptr<TM, 8>();     \
+void *pick_search_kernel(int tmax, bool extra, int sw) {
+#define HH_PICK(TM, EX)                                     \
+  if (tmax == TM && extra == EX) {                          \
+    if (sw == 1) return search_kernel_ptr<TM, EX, 1>();     \
+    if (sw == 2) return search_kernel_ptr<TM, EX, 2>();     \
+    if (sw == 4) return search_kernel_ptr<TM, EX, 4>();     \
+    if (sw == 8) return search_kernel_ptr<TM, EX, 8>();     \
   }
-  HH_PICK(4)
-  HH_PICK(8)
-  HH_PICK(32)
+  HH_PICK(4, false)
+  HH_PICK(4, true)
+  HH_PICK(8, false)
+  HH_PICK(8, true)
+  HH_PICK(32, false)
+  HH_PICK(32, true)
 #undef HH_PICK
   return nullptr;
 }
@@ -331,46 +344,81 @@ int first_failure(const std::vector<int32_t> &status, uint64_t *first_bad) {
   return HH_OK;
 }
 
-// Shared launch path of the uniform and the per-query-spec searches.
-int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
-                  uint32_t precision, const uint32_t *allow, const hh_query_spec *specs,
-                  uint32_t k_stride, const uint32_t *filters, uint32_t nfilters,
-                  uint64_t *out_keys, float *out_dist, uint32_t *out_count, int32_t *status,
-                  hipStream_t stream) {
-  if (nq == 0) return HH_OK;
-  if (!queries || !out_count || (k_stride > 0 && (!out_keys || !out_dist)))
-    return fail(HH_INVALID_ARGUMENT, "null device pointer");
-  const int tmax = tmax_for(idx->lay);
-  if (!tmax) return fail(HH_UNSUPPORTED, "search kernels support dim <= 1024");
-  uint32_t efmax = specs ? 512u : std::max(ef, k);
-  if (!specs && precision > 2) return fail(HH_INVALID_ARGUMENT, "unknown precision");
-  const int sw = slots_for(std::max<uint32_t>(efmax, 1));
-  if (!sw) return fail(HH_UNSUPPORTED, "ef (raised to k) above 512 is not supported");
-  if (!specs && precision == 2 && !idx->bf16)
-    return fail(HH_UNSUPPORTED, "bf16 rows not resident: call hh_index_enable_bf16");
-  HH_HIP(hipSetDevice(idx->device));
-  Workspace &ws = idx->ws;
-  std::lock_guard<std::mutex> lock(ws.mu);
-  int st = ws.ensure_queries(nq, idx->lay.stride, idx->cstride);
-  if (st) return st;
-  const uint32_t hash_cap = std::max<uint32_t>(efmax, 1) <= 128 ? 4096u : 8192u;
-  const uint32_t lds_words = hash_cap + 128;
-  const int wpb = std::max(1, std::min(4, (int)(65536 / (lds_words * 4))));
-  void *kern = pick_search_kernel(tmax, sw);
-  int blocks_per_cu = 0;
-  HH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kern, wpb * 64,
-                                                      lds_words * 4 * wpb));
-  blocks_per_cu = std::max(1, blocks_per_cu);
-  const uint64_t want = (nq + wpb - 1) / wpb;
-  const uint64_t blocks = std::min<uint64_t>(want, (uint64_t)blocks_per_cu * num_cus(idx->device));
-  const uint64_t spill_words = std::max<uint64_t>(1, (idx->n + 31) / 32);
-  st = ws.ensure_spill(blocks * wpb, spill_words);
-  if (st) return st;
-  HH_HIP(hipStreamWaitEvent(stream, ws.done, 0));
+// Launch shape of one search: kernel instance, visited-set capacity,
+// waves per block and grid size.
+struct SearchConfig {
+  void *kern = nullptr;
+  uint32_t hash_cap = 0, lds_words = 0;
+  int wpb = 1;
+  uint64_t blocks = 0, spill_words = 0;
+};
 
+struct SearchCall {
+  const float *queries;
+  uint64_t nq;
+  uint32_t k, ef, precision;
+  const uint32_t *allow;
+  const hh_query_spec *specs;
+  uint32_t k_stride;
+  const uint32_t *filters;
+  uint32_t nfilters;
+  uint64_t *out_keys;
+  float *out_dist;
+  uint32_t *out_count;
+  int32_t *status;
+};
+
+int check_call(hh_index *idx, const SearchCall &c) {
+  if (!c.queries || !c.out_count || (c.k_stride > 0 && (!c.out_keys || !c.out_dist)))
+    return fail(HH_INVALID_ARGUMENT, "null device pointer");
+  if (!tmax_for(idx->lay)) return fail(HH_UNSUPPORTED, "search kernels support dim <= 1024");
+  if (!c.specs && c.precision > 2) return fail(HH_INVALID_ARGUMENT, "unknown precision");
+  const uint32_t efmax = c.specs ? 512u : std::max(c.ef, c.k);
+  if (!slots_for(std::max<uint32_t>(efmax, 1)))
+    return fail(HH_UNSUPPORTED, "ef (raised to k) above 512 is not supported");
+  if (!c.specs && c.precision == 2 && !idx->bf16)
+    return fail(HH_UNSUPPORTED, "bf16 rows not resident: call hh_index_enable_bf16");
+  return HH_OK;
+}
+
+int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
+  const uint32_t efmax = std::max<uint32_t>(c.specs ? 512u : std::max(c.ef, c.k), 1);
+  const int sw = slots_for(efmax);
+  // visited-set capacity: 32 entries per unit of ef (a query at ef
+  // evaluates roughly 10-15 ef distances on the benchmark data), at least
+  // 2048 so the result arrays fit in it afterwards; past 3/4 full the query
+  // continues on its HBM bitset.
+  cfg.hash_cap = idx->tuned_hash_cap ? idx->tuned_hash_cap
+                                     : std::min<uint32_t>(16384, std::max<uint32_t>(2048, efmax * 32));
+  cfg.hash_cap = std::max<uint32_t>(cfg.hash_cap, 256u * (uint32_t)sw);
+  cfg.lds_words = cfg.hash_cap + 128;
+  cfg.kern = pick_search_kernel(tmax_for(idx->lay), idx->lay.U > 0 || idx->lay.R > 0, sw);
+  // waves per block: whichever of 1, 2, 4 keeps the most waves resident
+  int blocks_per_cu = 1, best_waves = 0;
+  for (int w : {4, 2, 1}) {
+    if (cfg.lds_words * 4 * w > 65536) continue;
+    int b = 0;
+    HH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, cfg.kern, w * 64, cfg.lds_words * 4 * w));
+    if (b * w > best_waves) {
+      best_waves = b * w;
+      cfg.wpb = w;
+      blocks_per_cu = b;
+    }
+  }
+  blocks_per_cu = std::max(1, blocks_per_cu);
+  const uint64_t want = (c.nq + cfg.wpb - 1) / cfg.wpb;
+  cfg.blocks = std::min<uint64_t>(want, (uint64_t)blocks_per_cu * num_cus(idx->device));
+  cfg.spill_words = std::max<uint64_t>(1, (idx->n + 31) / 32);
+  return HH_OK;
+}
+
+// Enqueues the prepare and search kernels of one call on `stream`, using
+// workspace `ws` (whose buffers the caller has sized).  No synchronisation.
+int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const SearchConfig &cfg,
+                   hipStream_t stream) {
   hhk::PrepArgs pa{};
-  pa.queries = queries;
-  pa.nq = nq;
+  pa.queries = c.queries;
+  pa.nq = c.nq;
   pa.dim = idx->dim;
   pa.metric = idx->metric;
   pa.T = (int)idx->lay.T;
@@ -380,16 +428,16 @@ int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, 
   pa.qcode = idx->codes ? ws.qcode : nullptr;
   pa.sq_offset = idx->sq_offset;
   pa.sq_scale = idx->sq_scale;
-  pa.precision = precision;
-  pa.specs = specs;
-  pa.k_stride = k_stride;
-  pa.nfilters = nfilters;
+  pa.precision = c.precision;
+  pa.specs = c.specs;
+  pa.k_stride = c.k_stride;
+  pa.nfilters = c.nfilters;
   pa.ef_cap = 512;
   pa.has_codes = idx->codes != nullptr;
   pa.has_bf16 = idx->bf16 != nullptr;
-  pa.status = status ? status : ws.status;
+  pa.status = c.status ? c.status : ws.status;
   pa.counter = ws.counter;
-  hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(grid_for(nq)), dim3(256), 0, stream, pa);
+  hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(grid_for(c.nq)), dim3(256), 0, stream, pa);
   HH_HIP(hipGetLastError());
 
   hhk::SearchArgs a{};
@@ -400,7 +448,8 @@ int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, 
   a.upper_off = idx->upper_off;
   a.upper = idx->upper;
   a.keys = idx->keys;
-  a.deleted_bits = idx->deleted_bits;
+  // the tombstone bitmap is only read when something is deleted
+  a.deleted_bits = idx->live < idx->n ? idx->deleted_bits : nullptr;
   a.n = (uint32_t)idx->n;
   a.m = idx->m;
   a.m0 = idx->m0;
@@ -417,31 +466,123 @@ int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, 
   a.qprep = ws.qprep;
   a.qcode = ws.qcode;
   a.status = pa.status;
-  a.nq = nq;
-  a.specs = specs;
-  a.k = k;
-  a.ef = ef;
-  a.precision = precision;
-  a.allow = allow;
-  a.filters = filters;
+  a.nq = c.nq;
+  a.specs = c.specs;
+  a.k = c.k;
+  a.ef = c.ef;
+  a.precision = c.precision;
+  a.allow = c.allow;
+  a.filters = c.filters;
   a.filter_words = (uint32_t)((idx->n + 31) / 32);
-  a.k_stride = k_stride;
-  a.out_keys = out_keys;
-  a.out_dist = out_dist;
-  a.out_count = out_count;
+  a.k_stride = c.k_stride;
+  a.out_keys = c.out_keys;
+  a.out_dist = c.out_dist;
+  a.out_count = c.out_count;
   a.counter = ws.counter;
   a.spill = ws.spill;
-  a.spill_words = spill_words;
-  a.stats = ws.stats;
-  a.hash_cap = hash_cap;
-  a.hash_limit = hash_cap / 2 + hash_cap / 4;
-  a.lds_words = lds_words;
+  a.spill_words = cfg.spill_words;
+  a.stats = idx->stats_ws.stats;
+  a.hash_cap = cfg.hash_cap;
+  a.hash_limit = cfg.hash_cap / 2 + cfg.hash_cap / 4;
+  a.lds_words = cfg.lds_words;
   void *args[] = {&a};
+  HH_HIP(hipLaunchKernel(cfg.kern, dim3((unsigned)cfg.blocks), dim3(cfg.wpb * 64), args,
+                         cfg.lds_words * 4 * cfg.wpb, stream));
+  return HH_OK;
+}
+
+// Takes a workspace that no other host thread is enqueuing into (creating
+// one, up to a small pool, when all are taken).
+Workspace *acquire_ws(hh_index *idx, int &st) {
+  std::unique_lock<std::mutex> lock(idx->pool_mu);
+  constexpr size_t kMaxPool = 4;
+  for (;;) {
+    for (size_t i = 0; i < idx->pool.size(); i++)
+      if (!idx->pool_busy[i] && hipEventQuery(idx->pool[i]->done) == hipSuccess) {
+        idx->pool_busy[i] = true;
+        return idx->pool[i].get();
+      }
+    if (idx->pool.size() < kMaxPool) {
+      std::unique_ptr<Workspace> w(new Workspace());
+      st = w->init();
+      if (st) return nullptr;
+      idx->pool.push_back(std::move(w));
+      idx->pool_busy.push_back(true);
+      return idx->pool.back().get();
+    }
+    for (size_t i = 0; i < idx->pool.size(); i++)
+      if (!idx->pool_busy[i]) {
+        idx->pool_busy[i] = true;  // still running on the GPU: the launch waits for its event
+        return idx->pool[i].get();
+      }
+    idx->pool_cv.wait(lock);
+  }
+}
+
+void release_ws(hh_index *idx, Workspace *w) {
+  std::lock_guard<std::mutex> lock(idx->pool_mu);
+  for (size_t i = 0; i < idx->pool.size(); i++)
+    if (idx->pool[i].get() == w) idx->pool_busy[i] = false;
+  idx->pool_cv.notify_one();
+}
+
+// Shared launch path of the uniform and the per-query-spec searches.
+int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
+                  uint32_t precision, const uint32_t *allow, const hh_query_spec *specs,
+                  uint32_t k_stride, const uint32_t *filters, uint32_t nfilters,
+                  uint64_t *out_keys, float *out_dist, uint32_t *out_count, int32_t *status,
+                  hipStream_t stream) {
+  if (nq == 0) return HH_OK;
+  const SearchCall c{queries, nq, k, ef, precision, allow, specs, k_stride, filters, nfilters,
+                     out_keys, out_dist, out_count, status};
+  int st = check_call(idx, c);
+  if (st) return st;
+  HH_HIP(hipSetDevice(idx->device));
+  SearchConfig cfg;
+  st = search_config(idx, c, cfg);
+  if (st) return st;
+  Workspace *ws = acquire_ws(idx, st);
+  if (!ws) return st;
+  struct Release {
+    hh_index *i;
+    Workspace *w;
+    ~Release() { release_ws(i, w); }
+  } rel{idx, ws};
+  st = ws->ensure_queries(nq, idx->lay.stride, idx->cstride);
+  if (!st) st = ws->ensure_spill(cfg.blocks * cfg.wpb, cfg.spill_words);
+  if (st) return st;
+  HH_HIP(hipStreamWaitEvent(stream, ws->done, 0));
   if (idx->timing) HH_HIP(hipEventRecord(idx->t0, stream));
-  HH_HIP(hipLaunchKernel(kern, dim3((unsigned)blocks), dim3(wpb * 64), args, lds_words * 4 * wpb,
-                         stream));
+  st = enqueue_search(idx, *ws, c, cfg, stream);
+  if (st) return st;
   if (idx->timing) HH_HIP(hipEventRecord(idx->t1, stream));
-  HH_HIP(hipEventRecord(ws.done, stream));
+  HH_HIP(hipEventRecord(ws->done, stream));
+  return HH_OK;
+}
+
+}  // namespace
+
+namespace hh {
+int index_device(const hh_index *idx) { return idx->device; }
+uint32_t index_dim(const hh_index *idx) { return idx->dim; }
+uint64_t index_nodes(const hh_index *idx) { return idx->n; }
+uint32_t index_resident_waves(hh_index *idx) {
+  SearchCall c{};
+  c.nq = 1u << 30;
+  c.k = 10;
+  c.ef = 64;
+  SearchConfig cfg;
+  if (!tmax_for(idx->lay) || search_config(idx, c, cfg)) return 1024;
+  return (uint32_t)std::max<uint64_t>(1, cfg.blocks * cfg.wpb);
+}
+}  // namespace hh
+
+namespace {
+
+// Waits for every launch on the index's workspaces.
+int drain(hh_index *idx) {
+  std::lock_guard<std::mutex> lock(idx->pool_mu);
+  for (auto &w : idx->pool) HH_HIP(hipEventSynchronize(w->done));
   return HH_OK;
 }
 
@@ -481,7 +622,7 @@ int hh_index_upload(const hh_graph_desc *d, int device, hh_index **out) {
   idx->entry = d->entry_id;
   idx->top = d->top_level;
   idx->lay = hh::RowLayout(d->dim);
-  int st = idx->ws.init();
+  int st = idx->stats_ws.init();
   if (st) return st;
   HH_HIP(hipStreamCreateWithFlags(&idx->stream, hipStreamNonBlocking));
   if (d->n == 0) {
@@ -552,8 +693,8 @@ int hh_index_update_deleted(hh_index *idx, const uint8_t *deleted, uint64_t live
   HH_HIP(hipSetDevice(idx->device));
   std::vector<uint32_t> bits;
   pack_deleted(deleted, idx->n, bits);
-  std::lock_guard<std::mutex> lock(idx->ws.mu);
-  HH_HIP(hipEventSynchronize(idx->ws.done));
+  int st = drain(idx);
+  if (st) return st;
   HH_HIP(hipMemcpy(idx->deleted_bits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice));
   idx->live = live;
   return HH_OK;
@@ -575,10 +716,10 @@ int hh_index_enable_bf16(hh_index *idx) {
 int hh_index_stats(hh_index *idx, hh_search_stats *out) {
   if (!idx || !out) return fail(HH_INVALID_ARGUMENT, "null argument");
   HH_HIP(hipSetDevice(idx->device));
-  std::lock_guard<std::mutex> lock(idx->ws.mu);
-  HH_HIP(hipEventSynchronize(idx->ws.done));
+  int st = drain(idx);
+  if (st) return st;
   unsigned long long v[8] = {0};
-  HH_HIP(hipMemcpy(v, idx->ws.stats, sizeof v, hipMemcpyDeviceToHost));
+  HH_HIP(hipMemcpy(v, idx->stats_ws.stats, sizeof v, hipMemcpyDeviceToHost));
   out->queries = v[0];
   out->n_dist = v[1];
   out->n_upper = v[2];
@@ -612,9 +753,9 @@ int hh_index_last_kernel_ms(hh_index *idx, float *ms) {
 int hh_index_reset_stats(hh_index *idx) {
   if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
   HH_HIP(hipSetDevice(idx->device));
-  std::lock_guard<std::mutex> lock(idx->ws.mu);
-  HH_HIP(hipEventSynchronize(idx->ws.done));
-  HH_HIP(hipMemset(idx->ws.stats, 0, 8 * sizeof(unsigned long long)));
+  int st = drain(idx);
+  if (st) return st;
+  HH_HIP(hipMemset(idx->stats_ws.stats, 0, 8 * sizeof(unsigned long long)));
   return HH_OK;
 }
 
@@ -694,6 +835,137 @@ int hh_search_batch(hh_index *idx, const float *queries, uint64_t nq, uint32_t k
   return first_failure(status, first_bad);
 }
 
+int hh_search_batch_spec(hh_index *idx, const float *queries, uint64_t nq,
+                         const hh_query_spec *specs, uint32_t k_stride, const uint32_t *filters,
+                         uint32_t nfilters, uint64_t *out_keys, float *out_dist,
+                         uint32_t *out_count, int32_t *status) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (nq == 0) return HH_OK;
+  if (!queries || !specs || !out_count || !status || (k_stride && (!out_keys || !out_dist)))
+    return fail(HH_INVALID_ARGUMENT, "null argument");
+  HH_HIP(hipSetDevice(idx->device));
+  float *dq = nullptr, *dd = nullptr;
+  uint64_t *dk = nullptr;
+  uint32_t *dc = nullptr, *df = nullptr;
+  int32_t *ds = nullptr;
+  hh_query_spec *dsp = nullptr;
+  auto cleanup = [&]() {
+    dfree(dq);
+    dfree(dd);
+    dfree(dk);
+    dfree(dc);
+    dfree(df);
+    dfree(ds);
+    dfree(dsp);
+  };
+  const uint64_t kk = std::max<uint32_t>(k_stride, 1);
+  const uint64_t words = (idx->n + 31) / 32;
+  int st = dalloc(&dq, nq * idx->dim);
+  if (!st) st = dalloc(&dd, nq * kk);
+  if (!st) st = dalloc(&dk, nq * kk);
+  if (!st) st = dalloc(&dc, nq);
+  if (!st) st = dalloc(&ds, nq);
+  if (!st) st = dalloc(&dsp, nq);
+  if (!st && filters && nfilters) st = dalloc(&df, std::max<uint64_t>(1, words * nfilters));
+  if (st) {
+    cleanup();
+    return st;
+  }
+  hipStream_t s = idx->stream;
+  hipError_t e = hipMemcpyAsync(dq, queries, nq * idx->dim * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dsp, specs, nq * sizeof(hh_query_spec), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && df && words)
+    e = hipMemcpyAsync(df, filters, words * nfilters * 4, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(HH_DEVICE_ERROR, hipGetErrorString(e));
+  }
+  st = launch_search(idx, dq, nq, 0, 0, 0, nullptr, dsp, k_stride, df, df ? nfilters : 0, dk, dd,
+                     dc, ds, s);
+  if (st) {
+    (void)hipStreamSynchronize(s);
+    cleanup();
+    return st;
+  }
+  e = hipMemcpyAsync(status, ds, nq * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(out_count, dc, nq * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && k_stride)
+    e = hipMemcpyAsync(out_keys, dk, nq * k_stride * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && k_stride)
+    e = hipMemcpyAsync(out_dist, dd, nq * k_stride * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  cleanup();
+  if (e != hipSuccess) return fail(HH_DEVICE_ERROR, hipGetErrorString(e));
+  return HH_OK;
+}
+
+}  // extern "C"
+
+struct hh_search_plan {
+  hh_index *idx = nullptr;
+  Workspace ws;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+  ~hh_search_plan() {
+    if (idx) (void)hipSetDevice(idx->device);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+    ws.release();
+  }
+};
+
+extern "C" {
+
+int hh_search_plan_create(hh_index *idx, const float *queries, uint64_t nq, uint32_t k,
+                          uint32_t ef, uint32_t precision, const uint32_t *allow_bits,
+                          const hh_query_spec *specs, uint32_t k_stride, const uint32_t *filters,
+                          uint32_t nfilters, uint64_t *out_keys, float *out_dist,
+                          uint32_t *out_count, int32_t *status, hh_search_plan **out) {
+  if (!idx || !out) return fail(HH_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (nq == 0) return fail(HH_INVALID_ARGUMENT, "empty batch");
+  const SearchCall c{queries, nq, specs ? 0u : k, specs ? 0u : ef, specs ? 0u : precision,
+                     specs ? nullptr : allow_bits, specs, specs ? k_stride : k,
+                     specs ? filters : nullptr, specs ? nfilters : 0u, out_keys, out_dist,
+                     out_count, status};
+  int st = check_call(idx, c);
+  if (st) return st;
+  HH_HIP(hipSetDevice(idx->device));
+  std::unique_ptr<hh_search_plan> p(new hh_search_plan());
+  p->idx = idx;
+  SearchConfig cfg;
+  st = search_config(idx, c, cfg);
+  if (!st) st = p->ws.init();
+  if (!st) st = p->ws.ensure_queries(nq, idx->lay.stride, idx->cstride);
+  if (!st) st = p->ws.ensure_spill(cfg.blocks * cfg.wpb, cfg.spill_words);
+  if (st) return st;
+  HH_HIP(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+  HH_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+  st = enqueue_search(idx, p->ws, c, cfg, p->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(p->cap, &g);
+  p->graph = g;
+  if (st) return st;
+  if (e != hipSuccess) return fail(HH_DEVICE_ERROR, std::string("graph capture: ") + hipGetErrorString(e));
+  HH_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  *out = p.release();
+  return HH_OK;
+}
+
+int hh_search_plan_launch(hh_search_plan *p, void *stream) {
+  if (!p) return fail(HH_INVALID_ARGUMENT, "null plan");
+  HH_HIP(hipSetDevice(p->idx->device));
+  hipStream_t s = (hipStream_t)stream;
+  HH_HIP(hipStreamWaitEvent(s, p->ws.done, 0));
+  HH_HIP(hipGraphLaunch(p->exec, s));
+  HH_HIP(hipEventRecord(p->ws.done, s));
+  return HH_OK;
+}
+
+void hh_search_plan_free(hh_search_plan *p) { delete p; }
+
 // ---------------------------------------------------------------- brute
 int hh_brute_upload(uint32_t dim, uint32_t metric, const uint64_t *keys, const float *vectors,
                     uint64_t n, int device, hh_brute **out, uint64_t *bad_key) {
@@ -741,12 +1013,12 @@ void hh_brute_free(hh_brute *b) { delete b; }
 
 namespace {
 
-template <int TM, int QB>
+template <int TM, bool EX, int QB>
 void *brute_partial_ptr(int s) {
-  if (s == 1) return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, QB, 1>);
-  if (s == 2) return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, QB, 2>);
-  if (s == 4) return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, QB, 4>);
-  return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, QB, 8>);
+  if (s == 1) return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, EX, QB, 1>);
+  if (s == 2) return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, EX, QB, 2>);
+  if (s == 4) return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, EX, QB, 4>);
+  return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, EX, QB, 8>);
 }
 
 int brute_launch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
@@ -774,8 +1046,8 @@ int brute_launch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
   nslices = std::min<uint64_t>(nslices, std::max<uint64_t>(1, (b->n + 1) / 2));
   nslices = std::min<uint64_t>(nslices, 1024);
   uint64_t rps = (b->n + nslices - 1) / nslices;
-  rps = (rps + 1) / 2 * 2;
-  if (rps == 0) rps = 2;
+  rps = (rps + 3) / 4 * 4;
+  if (rps == 0) rps = 4;
   nslices = std::max<uint64_t>(1, (b->n + rps - 1) / rps);
   st = ws.ensure_part(nq * nslices * std::max<uint32_t>(kc, 1), nq * nslices, f64);
   if (st) return st;
@@ -841,7 +1113,10 @@ int brute_launch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
                          : s == 4 ? (void *)&hhk::brute_merge_f64_kernel<4>
                                   : (void *)&hhk::brute_merge_f64_kernel<8>;
   } else {
-    pk = tmax == 4 ? brute_partial_ptr<4, 4>(s) : tmax == 8 ? brute_partial_ptr<8, 2>(s) : brute_partial_ptr<32, 1>(s);
+    const bool ex = b->lay.U > 0 || b->lay.R > 0;
+    if (tmax == 4) pk = ex ? brute_partial_ptr<4, true, 4>(s) : brute_partial_ptr<4, false, 4>(s);
+    else if (tmax == 8) pk = ex ? brute_partial_ptr<8, true, 2>(s) : brute_partial_ptr<8, false, 2>(s);
+    else pk = ex ? brute_partial_ptr<32, true, 1>(s) : brute_partial_ptr<32, false, 1>(s);
     mk = s == 1 ? (void *)&hhk::brute_merge_kernel<1>
                 : s == 2 ? (void *)&hhk::brute_merge_kernel<2>
                          : s == 4 ? (void *)&hhk::brute_merge_kernel<4> : (void *)&hhk::brute_merge_kernel<8>;

@@ -11,8 +11,9 @@
 // Per wave: the candidate list C and the result list W are sorted register
 // lists (element i in lane i/S); the visited set is an LDS hash table that
 // moves to a per-wave HBM bitset if a query visits more nodes than the
-// table holds; neighbour rows are gathered two at a time, one per
-// half-wavefront, 512 contiguous bytes per row at D=128.
+// table holds; neighbour rows are gathered four at a time, one per 16-lane
+// quarter, 512 contiguous bytes per row at D=128.  The adjacency row of the
+// next candidate is fetched while the current one is expanded.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -27,36 +28,48 @@ struct Stats {
 };
 
 // Distances from the query to rows ids[0..cnt) (ids in LDS), written to
-// out[0..cnt) (LDS), with the traversal precision `prec`.  All 64 lanes
-// must call it.
+// out[0..cnt) (LDS), with traversal precision `prec` (0 exact f32,
+// 1 int8 codes, 2 bf16 rows).  All 64 lanes must call it.
 template <int TMAX>
-__device__ void gather_dists(const SearchArgs &P, const Lanes<TMAX> &q, const uint32_t qc[8],
-                             int prec, const uint32_t *ids, int cnt, float *out) {
-  const int lane = lane_id(), j = lane & 31, half = lane >> 5;
+constexpr int code_chunks() { return (TMAX * 32 + 127) / 128; }
+
+template <int TMAX, bool EXTRA>
+__device__ __forceinline__ void gather_dists(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
+                                             const uint2 (&qc)[code_chunks<TMAX>()], int prec,
+                                             const uint32_t *ids, int cnt, float *out) {
+  const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
   const Shape s{P.T, P.U, P.R, P.stride};
   if (prec == 1) {
     // int8 codes of the crate's quantizer: exact integer sums
     // (src/kernels/int8.rs:13-48), rescaled as src/quantization.rs:85-92.
     const int chunks = (int)(P.cstride >> 7);
     const float s2 = __fmul_rn(P.sq_scale, P.sq_scale);
-    for (int base = 0; base < cnt; base += 2) {
-      const int row = base + half;
+    for (int base = 0; base < cnt; base += 4) {
+      const int row = base + quarter;
       int acc = 0;
       if (row < cnt) {
-        const uint32_t *cr =
-            reinterpret_cast<const uint32_t *>(P.codes + (size_t)ids[row] * P.cstride);
-        for (int c = 0; c < chunks; c++) {
-          int a = (int)qc[c], b = (int)cr[c * 32 + j];
-          if (P.metric == 0)
-            acc += __builtin_amdgcn_sdot4(a, a, 0, false) - 2 * __builtin_amdgcn_sdot4(a, b, 0, false) +
-                   __builtin_amdgcn_sdot4(b, b, 0, false);
-          else
-            acc += __builtin_amdgcn_sdot4(a, b, 0, false);
+        gptr<uint2> cr = (gptr<uint2>)(P.codes + (size_t)ids[row] * P.cstride);
+#pragma unroll
+        for (int c = 0; c < code_chunks<TMAX>(); c++) {
+          if (c < chunks) {
+            const uint2 b = cr[c * 16 + i];
+            const uint2 a = qc[c];
+            if (P.metric == 0) {
+              acc += __builtin_amdgcn_sdot4((int)a.x, (int)a.x, 0, false) -
+                     2 * __builtin_amdgcn_sdot4((int)a.x, (int)b.x, 0, false) +
+                     __builtin_amdgcn_sdot4((int)b.x, (int)b.x, 0, false);
+              acc += __builtin_amdgcn_sdot4((int)a.y, (int)a.y, 0, false) -
+                     2 * __builtin_amdgcn_sdot4((int)a.y, (int)b.y, 0, false) +
+                     __builtin_amdgcn_sdot4((int)b.y, (int)b.y, 0, false);
+            } else {
+              acc += __builtin_amdgcn_sdot4((int)a.x, (int)b.x, 0, false);
+              acc += __builtin_amdgcn_sdot4((int)a.y, (int)b.y, 0, false);
+            }
+          }
         }
       }
-#pragma unroll
-      for (int m = 1; m < 32; m <<= 1) acc += __shfl_xor(acc, m, 64);
-      if (j == 0 && row < cnt) {
+      acc = quarter_sum_i(acc);
+      if (i == 0 && row < cnt) {
         float v;
         if (P.metric == 0) v = __fmul_rn(__uint2float_rn((uint32_t)acc), s2);
         else if (P.metric == 1) v = __fmul_rn(-__int2float_rn(acc), s2);
@@ -66,74 +79,52 @@ __device__ void gather_dists(const SearchArgs &P, const Lanes<TMAX> &q, const ui
     }
     return;
   }
-  if (prec == 2) {
-    // bf16 rows in the same accumulator-major layout, f32 accumulation.
-    for (int base = 0; base < cnt; base += 2) {
-      const int row = base + half;
-      Lanes<TMAX> x;
-      float acc = 0.0f, tail = 0.0f;
-      if (row < cnt) {
-        load_lanes_bf16<TMAX>(P.bf16 + (size_t)ids[row] * P.stride, s, j, x);
-        if (P.metric == 0) acc = lane_acc<TMAX, true>(q, x, s, j, tail);
-        else acc = lane_acc<TMAX, false>(q, x, s, j, tail);
-      }
-      float h = half_reduce(acc, tail, s.R, half);
-      if (j == 0 && row < cnt) out[row] = finish_metric(h, P.metric);
-    }
-    return;
-  }
-  // exact f32, AVX2 order
-  constexpr int B = TMAX <= 4 ? 8 : (TMAX <= 8 ? 4 : 1);
-  for (int base = 0; base < cnt; base += 2 * B) {
-    Lanes<TMAX> x[B];
+  constexpr int B = TMAX <= 4 ? 4 : (TMAX <= 8 ? 2 : 1);
+  for (int base = 0; base < cnt; base += 4 * B) {
+    QLanes<TMAX, EXTRA> x[B];
 #pragma unroll
     for (int b = 0; b < B; b++) {
-      const int row = base + 2 * b + half;
-      if (row < cnt) load_lanes<TMAX>(P.rows + (size_t)ids[row] * P.stride, s, j, x[b]);
-    }
-    float acc[B], tail[B];
-#pragma unroll
-    for (int b = 0; b < B; b++) {
-      const int row = base + 2 * b + half;
+      const int row = base + 4 * b + quarter;
       if (row < cnt) {
-        if (P.metric == 0) acc[b] = lane_acc<TMAX, true>(q, x[b], s, j, tail[b]);
-        else acc[b] = lane_acc<TMAX, false>(q, x[b], s, j, tail[b]);
-      } else {
-        acc[b] = 0.0f;
-        tail[b] = 0.0f;
+        if (prec == 2) load_q<TMAX, EXTRA, uint16_t>(P.bf16 + (size_t)ids[row] * P.stride, s, i, x[b]);
+        else load_q<TMAX, EXTRA, float>(P.rows + (size_t)ids[row] * P.stride, s, i, x[b]);
       }
     }
 #pragma unroll
     for (int b = 0; b < B; b++) {
-      float h = half_reduce(acc[b], tail[b], s.R, half);
-      const int row = base + 2 * b + half;
-      if (j == 0 && row < cnt) out[row] = finish_metric(h, P.metric);
+      const int row = base + 4 * b + quarter;
+      float lo = 0.0f, hi = 0.0f, tail = 0.0f;
+      if (row < cnt) {
+        if (P.metric == 0) quarter_acc<TMAX, EXTRA, true>(q, x[b], s, i, lo, hi, tail);
+        else quarter_acc<TMAX, EXTRA, false>(q, x[b], s, i, lo, hi, tail);
+      }
+      const float h = finish_metric(quarter_reduce<EXTRA>(lo, hi, tail, s.R, quarter), P.metric);
+      if (i == 0 && row < cnt) out[row] = h;
     }
-    if (base + 2 * B >= cnt) break;
   }
 }
 
-template <int TMAX>
 __device__ __forceinline__ bool admit(const SearchArgs &P, const uint32_t *allow, uint32_t id) {
   if (P.deleted_bits && bit_test(P.deleted_bits, id)) return false;
   if (allow && !bit_test(allow, id)) return false;
   return true;
 }
 
-template <int TMAX, int SW>
-__global__ void __launch_bounds__(256) hnsw_search_kernel(const SearchArgs P) {
+template <int TMAX, bool EXTRA, int SW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? 4 : 2)))
+hnsw_search_kernel(const SearchArgs P) {
   constexpr int SC = 2 * SW;
   extern __shared__ uint32_t lds[];
-  const int lane = lane_id(), j = lane & 31;
+  const int lane = lane_id(), i16 = lane & 15;
   const int wave = threadIdx.x >> 6;
   const uint32_t slot = blockIdx.x * (blockDim.x >> 6) + wave;
   uint32_t *table = lds + (size_t)wave * P.lds_words;
   uint32_t *st_id = table + P.hash_cap;
   float *st_d = reinterpret_cast<float *>(st_id + 64);
   uint32_t *spill = P.spill ? P.spill + (size_t)slot * P.spill_words : nullptr;
-  const uint32_t hmask = P.hash_cap - 1;
-  const int hshift = 32 - __ffs(P.hash_cap) + 1;
+  const uint32_t hcap = P.hash_cap;
   const Shape shp{P.T, P.U, P.R, P.stride};
+  gptr<uint32_t> adj0 = G(P.adj0);
   Stats st = {0, 0, 0, 0, 0, 0, 0};
 
   for (;;) {
@@ -143,16 +134,17 @@ __global__ void __launch_bounds__(256) hnsw_search_kernel(const SearchArgs P) {
     if (qi >= P.nq) break;
     uint32_t k = P.k, ef = P.ef, prec = P.precision, filt = 0xFFFFFFFFu;
     if (P.specs) {
-      k = P.specs[qi].k;
-      ef = P.specs[qi].ef;
-      prec = P.specs[qi].precision;
-      filt = P.specs[qi].filter;
+      const hh_query_spec sp = G(P.specs)[qi];
+      k = sp.k;
+      ef = sp.ef;
+      prec = sp.precision;
+      filt = sp.filter;
     }
     k = rl(k, 0); ef = rl(ef, 0); prec = rl(prec, 0); filt = rl(filt, 0);
     const uint64_t obase = (uint64_t)qi * P.k_stride;
-    const int status = (int)rl((uint32_t)P.status[qi], 0);
+    const int status = (int)rl((uint32_t)G(P.status)[qi], 0);
     if (status != 0 || k == 0 || P.live == 0 || P.top < 0) {
-      if (lane == 0) P.out_count[qi] = 0;
+      if (lane == 0) GM(P.out_count)[qi] = 0;
       continue;
     }
     if (ef < k) ef = k;
@@ -161,20 +153,24 @@ __global__ void __launch_bounds__(256) hnsw_search_kernel(const SearchArgs P) {
     st.q++;
 
     // query, in the row layout, plus its int8 code words
-    Lanes<TMAX> q;
+    QLanes<TMAX, EXTRA> q;
 #pragma unroll
-    for (int t = 0; t < TMAX; t++) q.m[t] = 0.0f;
-    q.e[0] = q.e[1] = q.e[2] = q.r = 0.0f;
-    load_lanes<TMAX>(P.qprep + (size_t)qi * P.stride, shp, j, q);
-    uint32_t qc[8];
+    for (int t = 0; t < TMAX; t++) q.lo[t] = q.hi[t] = 0.0f;
+    q.e[0] = q.r = 0.0f;
+    if constexpr (EXTRA) q.e[1] = q.e[2] = 0.0f;
+    load_q<TMAX, EXTRA, float>(P.qprep + (size_t)qi * P.stride, shp, i16, q);
+    uint2 qc[code_chunks<TMAX>()];
 #pragma unroll
-    for (int c = 0; c < 8; c++) qc[c] = 0;
+    for (int c = 0; c < code_chunks<TMAX>(); c++) qc[c] = make_uint2(0, 0);
     if (prec == 1) {
-      const uint32_t *cq = reinterpret_cast<const uint32_t *>(P.qcode + (size_t)qi * P.cstride);
-      for (int c = 0; c < (int)(P.cstride >> 7) && c < 8; c++) qc[c] = cq[c * 32 + j];
+      gptr<uint2> cq = (gptr<uint2>)(P.qcode + (size_t)qi * P.cstride);
+      const int chunks = (int)(P.cstride >> 7);
+#pragma unroll
+      for (int c = 0; c < code_chunks<TMAX>(); c++)
+        if (c < chunks) qc[c] = cq[c * 16 + i16];
     }
 
-    for (uint32_t i = lane; i < P.hash_cap; i += 64) table[i] = kInvalid;
+    for (uint32_t t = lane; t < hcap; t += 64) table[t] = kInvalid;
     wave_sync();
     int mode = 0;
     uint32_t vcount = 0;
@@ -183,34 +179,32 @@ __global__ void __launch_bounds__(256) hnsw_search_kernel(const SearchArgs P) {
     uint32_t cur = P.entry;
     if (lane == 0) st_id[0] = cur;
     wave_sync();
-    gather_dists<TMAX>(P, q, qc, (int)prec, st_id, 1, st_d);
+    gather_dists<TMAX, EXTRA>(P, q, qc, (int)prec, st_id, 1, st_d);
     wave_sync();
-    float best = st_d[0];
-    best = rlf(best, 0);
+    float best = rlf(st_d[0], 0);
     st.up++;
     for (int layer = P.top; layer >= 1; layer--) {
       for (;;) {
         const uint32_t from = cur;
-        const uint32_t *blk = P.upper + P.upper_off[from] + (size_t)(layer - 1) * (P.m + 1);
-        uint32_t cnt = blk[0];
-        cnt = rl(cnt, 0);
+        gptr<uint32_t> blk = G(P.upper) + G(P.upper_off)[from] + (size_t)(layer - 1) * (P.m + 1);
+        uint32_t cnt = rl(blk[0], 0);
         if (cnt > P.m) cnt = P.m;
         for (uint32_t base = 0; base < cnt; base += 64) {
           const int c = (int)(cnt - base < 64 ? cnt - base : 64);
-          uint32_t nid = lane < c ? blk[1 + base + lane] : kInvalid;
+          const uint32_t nid = lane < c ? blk[1 + base + lane] : kInvalid;
           if (lane < c) st_id[lane] = nid;
           wave_sync();
-          gather_dists<TMAX>(P, q, qc, (int)prec, st_id, c, st_d);
+          gather_dists<TMAX, EXTRA>(P, q, qc, (int)prec, st_id, c, st_d);
           wave_sync();
-          float dv = lane < c ? st_d[lane] : 0.0f;
+          const float dv = lane < c ? st_d[lane] : 0.0f;
           // first list position holding the IEEE-smallest distance
           float md = dv;
           int mi = lane < c ? lane : 64;
 #pragma unroll
           for (int m = 1; m < 64; m <<= 1) {
-            float od = __shfl_xor(md, m, 64);
-            int oi = __shfl_xor(mi, m, 64);
-            bool take = oi < 64 && (mi == 64 || od < md || (!(md < od) && oi < mi));
+            const float od = __shfl_xor(md, m, 64);
+            const int oi = __shfl_xor(mi, m, 64);
+            const bool take = oi < 64 && (mi == 64 || od < md || (!(md < od) && oi < mi));
             if (take) {
               md = od;
               mi = oi;
@@ -238,16 +232,18 @@ __global__ void __launch_bounds__(256) hnsw_search_kernel(const SearchArgs P) {
     for (int s = 0; s < SC; s++) { C.d[s] = 0.0f; C.id[s] = kInvalid; }
     int wsize = 0, csize = 0;
     float worst = 0.0f;
-    if (lane == 0) lds_visit(table, hmask, hshift, cur);
+    if (lane == 0) lds_visit(table, hcap, cur);
     wave_sync();
     vcount = 1;
     slist_insert_at<SC>(C, 0, best, cur);
     csize = 1;
-    if (admit<TMAX>(P, allow, cur)) {
+    if (admit(P, allow, cur)) {
       slist_insert_at<SW>(W, 0, best, cur);
       wsize = 1;
       worst = best;
     }
+    const bool pf_ok = P.m0 <= 64;
+    uint32_t pf_id = kInvalid, pf_adj = kInvalid;
 
     while (csize > 0) {
       float cd;
@@ -257,33 +253,45 @@ __global__ void __launch_bounds__(256) hnsw_search_kernel(const SearchArgs P) {
       csize--;
       if (wsize >= (int)ef && cd > worst) break;
       st.exp++;
-      const uint32_t *adj = P.adj0 + (size_t)cid * P.m0;
+      const size_t arow = (size_t)cid * P.m0;
       for (uint32_t base = 0; base < P.m0; base += 64) {
         const int c = (int)(P.m0 - base < 64 ? P.m0 - base : 64);
-        uint32_t nid = lane < c ? adj[base + lane] : kInvalid;
+        uint32_t nid;
+        if (base == 0 && cid == pf_id) nid = pf_adj;
+        else nid = lane < c ? adj0[arow + base + lane] : kInvalid;
+        if (base == 0 && pf_ok) {
+          // prefetch the adjacency row of the next candidate in line
+          if (csize > 0) {
+            float nd;
+            slist_get<SC>(C, 0, nd, pf_id);
+            pf_adj = lane < (int)P.m0 ? adj0[(size_t)pf_id * P.m0 + lane] : kInvalid;
+          } else {
+            pf_id = kInvalid;
+          }
+        }
         bool isnew = false;
-        if (nid != kInvalid) isnew = mode == 0 ? lds_visit(table, hmask, hshift, nid) : bit_visit(spill, nid);
+        if (nid != kInvalid) isnew = mode == 0 ? lds_visit(table, hcap, nid) : bit_visit(spill, nid);
         const uint64_t nm = __ballot(isnew);
         const int nn = popc64(nm);
         if (nn == 0) continue;
         if (isnew) st_id[mbcnt(nm)] = nid;
         wave_sync();
-        uint32_t myid = lane < nn ? st_id[lane] : kInvalid;
-        bool adm = lane < nn && admit<TMAX>(P, allow, myid);
-        gather_dists<TMAX>(P, q, qc, (int)prec, st_id, nn, st_d);
+        const uint32_t myid = lane < nn ? st_id[lane] : kInvalid;
+        const bool adm = lane < nn && admit(P, allow, myid);
+        gather_dists<TMAX, EXTRA>(P, q, qc, (int)prec, st_id, nn, st_d);
         wave_sync();
-        float myd = lane < nn ? st_d[lane] : 0.0f;
+        const float myd = lane < nn ? st_d[lane] : 0.0f;
         const uint64_t am = __ballot(adm);
         vcount += nn;
         st.dist += nn;
         uint64_t cm = wsize >= (int)ef ? __ballot(lane < nn && myd < worst) : __ballot(lane < nn);
         while (cm) {
-          const int i = __ffsll((unsigned long long)cm) - 1;
+          const int b = __ffsll((unsigned long long)cm) - 1;
           cm &= cm - 1;
-          const float di = rlf(myd, i);
-          const uint32_t ii = rl(myid, i);
+          const float di = rlf(myd, b);
+          const uint32_t ii = rl(myid, b);
           if (!(wsize < (int)ef || di < worst)) continue;
-          if ((am >> i) & 1) {
+          if ((am >> b) & 1) {
             const int p = slist_rank<SW>(W, wsize, di, ii);
             slist_insert_at<SW>(W, p, di, ii);
             if (wsize < (int)ef) wsize++;
@@ -304,9 +312,11 @@ __global__ void __launch_bounds__(256) hnsw_search_kernel(const SearchArgs P) {
         }
         wave_sync();
         if (mode == 0 && vcount > P.hash_limit && spill) {
-          for (uint32_t i = lane; i < P.hash_cap; i += 64) {
-            uint32_t v = table[i];
-            if (v != kInvalid) atomicOr(&spill[v >> 5], 1u << (v & 31));
+          for (uint32_t t = lane; t < hcap; t += 64) {
+            const uint32_t v = table[t];
+            if (v != kInvalid)
+              __hip_atomic_fetch_or(spill + (v >> 5), 1u << (v & 31), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
           }
           __threadfence();
           mode = 1;
@@ -317,8 +327,8 @@ __global__ void __launch_bounds__(256) hnsw_search_kernel(const SearchArgs P) {
 
     // ---- results: optional exact re-rank, keys, (distance, key) order
     float *res_d = reinterpret_cast<float *>(table);
-    uint32_t *res_id = table + 512;
-    uint32_t *res_key = table + 1024;  // 2 words per entry
+    uint32_t *res_id = table + 64 * SW;
+    uint32_t *res_key = table + 128 * SW;  // 2 words per entry
 #pragma unroll
     for (int s = 0; s < SW; s++) {
       const int idx = lane * SW + s;
@@ -331,33 +341,41 @@ __global__ void __launch_bounds__(256) hnsw_search_kernel(const SearchArgs P) {
     if (prec != 0) {
       for (int base = 0; base < wsize; base += 64) {
         const int c = wsize - base < 64 ? wsize - base : 64;
-        gather_dists<TMAX>(P, q, qc, 0, res_id + base, c, res_d + base);
+        gather_dists<TMAX, EXTRA>(P, q, qc, 0, res_id + base, c, res_d + base);
         wave_sync();
       }
       st.rr += wsize;
     }
     for (int idx = lane; idx < wsize; idx += 64) {
-      uint64_t key = P.keys[res_id[idx]];
+      const uint64_t key = G(P.keys)[res_id[idx]];
       res_key[2 * idx] = (uint32_t)key;
       res_key[2 * idx + 1] = (uint32_t)(key >> 32);
     }
     wave_sync();
+    // W is sorted by (distance, id); the crate orders by (distance, key).
+    // Without equal distances the orders agree; otherwise rank by counting.
+    bool ties = false;
+    for (int idx = lane; idx + 1 < wsize; idx += 64) ties |= tkey(res_d[idx]) == tkey(res_d[idx + 1]);
+    const bool direct = prec == 0 && __ballot(ties) == 0;
     for (int idx = lane; idx < wsize; idx += 64) {
       const float d = res_d[idx];
       const uint64_t key = ((uint64_t)res_key[2 * idx + 1] << 32) | res_key[2 * idx];
-      int rank = 0;
-      for (int o = 0; o < wsize; o++) {
-        const uint64_t ko = ((uint64_t)res_key[2 * o + 1] << 32) | res_key[2 * o];
-        rank += nless(res_d[o], ko, d, key);
+      int rank = idx;
+      if (!direct) {
+        rank = 0;
+        for (int o = 0; o < wsize; o++) {
+          const uint64_t ko = ((uint64_t)res_key[2 * o + 1] << 32) | res_key[2 * o];
+          rank += nless(res_d[o], ko, d, key);
+        }
       }
       if (rank < (int)k) {
-        P.out_keys[obase + rank] = key;
-        P.out_dist[obase + rank] = d;
+        GM(P.out_keys)[obase + rank] = key;
+        GM(P.out_dist)[obase + rank] = d;
       }
     }
-    if (lane == 0) P.out_count[qi] = (uint32_t)(wsize < (int)k ? wsize : (int)k);
+    if (lane == 0) GM(P.out_count)[qi] = (uint32_t)(wsize < (int)k ? wsize : (int)k);
     if (mode == 1) {
-      for (uint32_t w = lane; w < P.spill_words; w += 64) spill[w] = 0u;
+      for (uint32_t w = lane; w < P.spill_words; w += 64) GM(spill)[w] = 0u;
       __threadfence();
     }
     wave_sync();
@@ -381,25 +399,25 @@ __global__ void prepare_queries_kernel(const PrepArgs A) {
   const uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (qi == 0 && A.counter) *A.counter = 0u;
   if (qi >= A.nq) return;
-  const float *v = A.queries + qi * A.dim;
+  gptr<float> v = G(A.queries) + qi * A.dim;
   int status = 0;
   double norm_sq = 0.0;
-  for (uint32_t i = 0; i < A.dim; i++) {
-    float x = v[i];
+  for (uint32_t e = 0; e < A.dim; e++) {
+    const float x = v[e];
     if (!isfinite(x)) {
       status = 4;
       break;
     }
-    double xd = (double)x;
+    const double xd = (double)x;
     norm_sq = __dadd_rn(norm_sq, __dmul_rn(xd, xd));
   }
   if (status == 0 && norm_sq > 3.4028234663852886e38 / 8.0) status = 4;
   if (status == 0 && A.metric == 2 && norm_sq == 0.0) status = 5;
   uint32_t prec = A.precision;
   if (A.specs) {
-    prec = A.specs[qi].precision;
+    const hh_query_spec sp = G(A.specs)[qi];
+    prec = sp.precision;
     if (status == 0) {
-      const hh_query_spec sp = A.specs[qi];
       if (sp.precision > 2) status = 103;
       else if (sp.k > A.k_stride) status = 103;
       else if (sp.filter != 0xFFFFFFFFu && sp.filter >= A.nfilters) status = 103;
@@ -408,19 +426,19 @@ __global__ void prepare_queries_kernel(const PrepArgs A) {
   }
   if (status == 0 && prec == 1 && !A.has_codes) status = 8;
   if (status == 0 && prec == 2 && !A.has_bf16) status = 102;
-  A.status[qi] = status;
+  GM(A.status)[qi] = status;
   if (status != 0) return;
   const double inv = A.metric == 2 ? 1.0 / sqrt(norm_sq) : 1.0;
-  float *out = A.qprep + qi * A.stride;
-  int8_t *code = A.qcode ? A.qcode + qi * A.cstride : nullptr;
+  gmut<float> out = GM(A.qprep) + qi * A.stride;
+  gmut<int8_t> code = A.qcode ? GM(A.qcode) + qi * A.cstride : nullptr;
   const float sinv = 1.0f / A.sq_scale;
   for (uint32_t e = 0; e < A.dim; e++) {
     float x = v[e];
     if (A.metric == 2) x = (float)__dmul_rn((double)x, inv);
     uint32_t pos = e;
     if (e < 32u * A.T) {
-      uint32_t t = e >> 5, jj = e & 31, b = t >> 2;
-      uint32_t tw = A.T - 4 * b < 4 ? A.T - 4 * b : 4;
+      const uint32_t t = e >> 5, jj = e & 31, b = t >> 2;
+      const uint32_t tw = A.T - 4 * b < 4 ? A.T - 4 * b : 4;
       pos = b * 128 + jj * tw + (t & 3);
     }
     out[pos] = x;
@@ -434,17 +452,16 @@ __global__ void prepare_queries_kernel(const PrepArgs A) {
     for (uint32_t e = A.dim; e < A.cstride; e++) code[e] = 0;
 }
 
-template __global__ void hnsw_search_kernel<4, 1>(const SearchArgs);
-template __global__ void hnsw_search_kernel<4, 2>(const SearchArgs);
-template __global__ void hnsw_search_kernel<4, 4>(const SearchArgs);
-template __global__ void hnsw_search_kernel<4, 8>(const SearchArgs);
-template __global__ void hnsw_search_kernel<8, 1>(const SearchArgs);
-template __global__ void hnsw_search_kernel<8, 2>(const SearchArgs);
-template __global__ void hnsw_search_kernel<8, 4>(const SearchArgs);
-template __global__ void hnsw_search_kernel<8, 8>(const SearchArgs);
-template __global__ void hnsw_search_kernel<32, 1>(const SearchArgs);
-template __global__ void hnsw_search_kernel<32, 2>(const SearchArgs);
-template __global__ void hnsw_search_kernel<32, 4>(const SearchArgs);
-template __global__ void hnsw_search_kernel<32, 8>(const SearchArgs);
+#define HH_SEARCH_INST(TM, EX)                                                      \
+  template __global__ void hnsw_search_kernel<TM, EX, 1>(const SearchArgs);         \
+  template __global__ void hnsw_search_kernel<TM, EX, 2>(const SearchArgs);         \
+  template __global__ void hnsw_search_kernel<TM, EX, 4>(const SearchArgs);         \
+  template __global__ void hnsw_search_kernel<TM, EX, 8>(const SearchArgs);
+HH_SEARCH_INST(4, false)
+HH_SEARCH_INST(4, true)
+HH_SEARCH_INST(8, false)
+HH_SEARCH_INST(8, true)
+HH_SEARCH_INST(32, false)
+HH_SEARCH_INST(32, true)
 
 }  // namespace hhk

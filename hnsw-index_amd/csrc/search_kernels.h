@@ -67,7 +67,7 @@ struct PrepArgs {
   uint32_t *counter;
 };
 
-template <int TMAX, int SW>
+template <int TMAX, bool EXTRA, int SW>
 __global__ void hnsw_search_kernel(const SearchArgs P);
 __global__ void prepare_queries_kernel(const PrepArgs A);
 

@@ -22,7 +22,7 @@ from ._lib import lib
 
 __all__ = [
     "Backend", "BruteForceIndex", "Error", "HnswIndex", "HnswParams", "Metric", "Neighbor",
-    "NeighborSelection", "SearchOptions", "Precision", "eval", "synthetic", "MAX_DIM",
+    "NeighborSelection", "SearchOptions", "Precision", "Server", "eval", "synthetic", "MAX_DIM",
     "MAX_NODES",
 ]
 
@@ -379,6 +379,37 @@ class HnswIndex:
         _check(st, query=bad.value)
         return ok[:, :k], od[:, :k], oc
 
+    def search_mixed(self, queries, ks, efs, precisions=None, filter_ids=None, filters=None):
+        """One launch for queries with their own k, ef, precision and filter
+        (filter_ids[i] indexes `filters`, a list of bool masks over node ids,
+        or -1 for none).  Returns (keys, dists, counts, status) padded to
+        max(ks); an invalid query gets its status and no results."""
+        queries = _f32(queries).reshape(-1, self._dim)
+        nq = queries.shape[0]
+        specs = np.zeros((nq, 4), np.uint32)
+        specs[:, 0] = ks
+        specs[:, 1] = efs
+        specs[:, 2] = 0 if precisions is None else np.asarray(precisions)
+        specs[:, 3] = 0xFFFFFFFF if filter_ids is None else \
+            np.where(np.asarray(filter_ids) < 0, 0xFFFFFFFF, np.asarray(filter_ids))
+        if precisions is not None and (np.asarray(precisions) == Precision.BF16).any() and not self._bf16:
+            self.device_index()
+            self.enable_bf16()
+        idx = self.device_index()
+        kst = max(int(np.max(ks)) if nq else 1, 1)
+        fl = None
+        nf = 0
+        if filters:
+            fl = np.ascontiguousarray(np.stack([_bits_from_mask(m) for m in filters]))
+            nf = len(filters)
+        ok = np.zeros((nq, kst), np.uint64)
+        od = np.zeros((nq, kst), np.float32)
+        oc = np.zeros(nq, np.uint32)
+        ost = np.zeros(nq, np.int32)
+        _check(lib.hh_search_batch_spec(idx, _ptr(queries), nq, _ptr(specs), kst, _ptr(fl), nf,
+                                        _ptr(ok), _ptr(od), _ptr(oc), _ptr(ost)))
+        return ok, od, oc, ost
+
     def search_batch(self, queries, k, ef):
         queries = _f32(queries)
         if queries.size % self._dim:
@@ -402,6 +433,76 @@ class HnswIndex:
 
     def search_with(self, query, k, options):
         return self._single(query, k, options._ef, options._precision, options._filter)
+
+
+class Server:
+    """Continuous-batching request loop over an index's device copy
+    (hh_serve_*): submit single queries with their own k, ef, precision and
+    registered filter; results come back per ticket with their latency."""
+
+    def __init__(self, index, k_max=100, max_batch=0, max_inflight=2, max_filters=8,
+                 max_queue=0):
+        self._index = index
+        self._dim = index.dim()
+        cfg = L.ServeConfig(max_batch, max_inflight, k_max, max_filters, max_queue)
+        h = ctypes.c_void_p()
+        _check(lib.hh_serve_open(index.device_index(), ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._k_max = k_max
+
+    def register_filter(self, mask):
+        """mask: bool array over node ids. Returns the filter id."""
+        fid = ctypes.c_uint32()
+        _check(lib.hh_serve_register_filter(self._h, _ptr(_bits_from_mask(mask)), ctypes.byref(fid)))
+        return fid.value
+
+    def submit(self, query, k, ef, precision=Precision.F32, filter_id=-1):
+        q = _f32(query).ravel()
+        if q.size != self._dim:
+            raise DimensionMismatch(f"dimension mismatch: expected {self._dim}, got {q.size}",
+                                    expected=self._dim, actual=q.size)
+        spec = L.QuerySpec(int(k), int(ef), int(precision),
+                           0xFFFFFFFF if filter_id < 0 else int(filter_id))
+        t = ctypes.c_uint64()
+        _check(lib.hh_serve_submit(self._h, _ptr(q), ctypes.byref(spec), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket, wait=True):
+        """(keys, dists, status, latency_ms) or None while pending."""
+        keys = np.zeros(self._k_max, np.uint64)
+        dist = np.zeros(self._k_max, np.float32)
+        cnt = ctypes.c_uint32()
+        st = ctypes.c_int32()
+        lat = ctypes.c_double()
+        done = ctypes.c_int()
+        _check(lib.hh_serve_poll(self._h, ticket, int(wait), _ptr(keys), _ptr(dist),
+                                 ctypes.byref(cnt), ctypes.byref(st), ctypes.byref(lat),
+                                 ctypes.byref(done)))
+        if not done.value:
+            return None
+        return keys[:cnt.value], dist[:cnt.value], st.value, lat.value
+
+    def stats(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        st = lib.hh_serve_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return {"served": a.value, "batches": b.value, "queued": c.value, "error": st}
+
+    def close(self):
+        if self._h:
+            h, self._h = self._h, None
+            _check(lib.hh_serve_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class BruteForceIndex:

@@ -45,6 +45,11 @@ class QuerySpec(ctypes.Structure):
     _fields_ = [("k", U32), ("ef", U32), ("precision", U32), ("filter", U32)]
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("max_batch", U32), ("max_inflight", U32), ("k_max", U32), ("max_filters", U32),
+                ("max_queue", U64)]
+
+
 def _sig(name, restype, *args):
     f = getattr(lib, name)
     f.restype = restype
@@ -74,6 +79,18 @@ _sig("hh_index_set_timing", INT, P, INT)
 _sig("hh_index_last_kernel_ms", INT, P, ctypes.POINTER(ctypes.c_float))
 _sig("hh_search_batch_device", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, P)
 _sig("hh_search_batch_spec_device", INT, P, P, U64, P, U32, P, U32, P, P, P, P, P)
+_sig("hh_search_batch_spec", INT, P, P, U64, P, U32, P, U32, P, P, P, P)
+_sig("hh_search_plan_create", INT, P, P, U64, U32, U32, U32, P, P, U32, P, U32, P, P, P, P,
+     ctypes.POINTER(P))
+_sig("hh_search_plan_launch", INT, P, P)
+_sig("hh_search_plan_free", None, P)
+_sig("hh_serve_open", INT, P, ctypes.POINTER(ServeConfig), ctypes.POINTER(P))
+_sig("hh_serve_register_filter", INT, P, P, ctypes.POINTER(U32))
+_sig("hh_serve_submit", INT, P, P, ctypes.POINTER(QuerySpec), ctypes.POINTER(U64))
+_sig("hh_serve_poll", INT, P, U64, INT, P, P, ctypes.POINTER(U32), ctypes.POINTER(I32),
+     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(INT))
+_sig("hh_serve_stats", INT, P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64))
+_sig("hh_serve_close", INT, P)
 _sig("hh_search_batch", INT, P, P, U64, U32, U32, U32, P, P, P, P, ctypes.POINTER(U64))
 _sig("hh_brute_upload", INT, U32, U32, P, P, U64, INT, ctypes.POINTER(P), ctypes.POINTER(U64))
 _sig("hh_brute_free", None, P)

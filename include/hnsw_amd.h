@@ -196,6 +196,32 @@ int hh_search_batch_spec_device(hh_index *idx, const float *queries, uint64_t nq
                                 float *out_dist, uint32_t *out_count, int32_t *status,
                                 void *stream);
 
+/* Host-memory form of the mixed batch.  Unlike hh_search_batch, an invalid
+ * query does not fail the call: its hh_status goes to status[i] (host, nq
+ * entries) and it gets no results, as a serving loop needs. */
+int hh_search_batch_spec(hh_index *idx, const float *queries, uint64_t nq,
+                         const hh_query_spec *specs, uint32_t k_stride, const uint32_t *filters,
+                         uint32_t nfilters, uint64_t *out_keys, float *out_dist,
+                         uint32_t *out_count, int32_t *status);
+
+/* A search whose launch (query preparation + the persistent search kernel)
+ * is captured once in a HIP graph and replayed: for callers that issue the
+ * same batch shape repeatedly on fixed device buffers.  Arguments as in
+ * hh_search_batch_device (specs != NULL selects the mixed form, as
+ * hh_search_batch_spec_device; then k, ef, precision and allow_bits are
+ * ignored).  The plan owns its workspace; its launches are ordered with an
+ * event, so launches of one plan never overlap.  A plan captures the index
+ * as it is at creation: re-create it after hh_index_update_deleted or
+ * hh_index_enable_bf16. */
+typedef struct hh_search_plan hh_search_plan;
+int hh_search_plan_create(hh_index *idx, const float *queries, uint64_t nq, uint32_t k,
+                          uint32_t ef, uint32_t precision, const uint32_t *allow_bits,
+                          const hh_query_spec *specs, uint32_t k_stride, const uint32_t *filters,
+                          uint32_t nfilters, uint64_t *out_keys, float *out_dist,
+                          uint32_t *out_count, int32_t *status, hh_search_plan **out);
+int hh_search_plan_launch(hh_search_plan *plan, void *stream);
+void hh_search_plan_free(hh_search_plan *plan);
+
 /* Synchronous host-memory form of hh_search_batch_device with the crate's
  * error behaviour: on an invalid query returns its status and writes its
  * index to *first_bad (src/hnsw/mod.rs:513-536); allow_bits is a host
@@ -203,6 +229,37 @@ int hh_search_batch_spec_device(hh_index *idx, const float *queries, uint64_t nq
 int hh_search_batch(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
                     uint32_t precision, const uint32_t *allow_bits, uint64_t *out_keys,
                     float *out_dist, uint32_t *out_count, uint64_t *first_bad);
+
+/* ------------------------------------------------------------------ */
+/* Serving loop (no crate counterpart: continuous batching of single     */
+/* requests over a resident index).                                      */
+/* ------------------------------------------------------------------ */
+typedef struct hh_server hh_server;
+typedef struct hh_serve_config {
+  uint32_t max_batch;     /* queries admitted per launch (0: resident waves) */
+  uint32_t max_inflight;  /* launches in flight at once, 1..4 (0: 2) */
+  uint32_t k_max;         /* largest k a request may ask for (1..512) */
+  uint32_t max_filters;   /* size of the registered-filter table */
+  uint64_t max_queue;     /* waiting requests before submit refuses (0: 64 batches) */
+} hh_serve_config;
+
+/* Starts the request loop on its own host thread and streams. */
+int hh_serve_open(hh_index *idx, const hh_serve_config *cfg, hh_server **out);
+/* Uploads a filter bitmap (over node ids) for requests to reference by id
+ * in hh_query_spec.filter. */
+int hh_serve_register_filter(hh_server *s, const uint32_t *allow_bits, uint32_t *filter_id);
+/* Queues one query (host memory, dim floats) with its spec; returns a
+ * ticket.  HH_CAPACITY_EXCEEDED when the queue is full. */
+int hh_serve_submit(hh_server *s, const float *query, const hh_query_spec *spec, uint64_t *ticket);
+/* Result of a ticket: *done = 0 while pending (or blocks when wait != 0).
+ * keys/dist receive up to spec.k entries; latency is submit-to-completion. */
+int hh_serve_poll(hh_server *s, uint64_t ticket, int wait, uint64_t *keys, float *dist,
+                  uint32_t *count, int32_t *status, double *latency_ms, int *done);
+/* Requests served, launches made and requests waiting; returns the first
+ * launch error seen, if any. */
+int hh_serve_stats(hh_server *s, uint64_t *served, uint64_t *batches, uint64_t *queued);
+/* Drains every queued request, then stops the loop and frees the server. */
+int hh_serve_close(hh_server *s);
 
 /* ------------------------------------------------------------------ */
 /* Exact brute force (BruteForceIndex, src/brute.rs:28-180).            */

@@ -7,7 +7,10 @@ import oracle as O
 
 
 def oracle_graph(idx):
-    v = idx.graph_arrays()
+    """The oracle's view of the index's current graph (arrays copied, so the
+    view outlives later mutation of the index)."""
+    v = {k: (np.array(x, copy=True) if isinstance(x, np.ndarray) else x)
+         for k, x in idx.graph_arrays().items()}
     return O.OracleGraph(v["dim"], v["metric"], v["m"], v["m0"], v["vectors"], v["keys"],
                          v["levels"], v["layer0"], v["upper_off"], v["upper"], v["entry_id"],
                          v["top"], deleted=v["deleted"], codes=v["codes"],

@@ -180,3 +180,70 @@ def test_large_ef_spills_visited_set_and_stays_exact():
     (gk, gd, gc), (ok, od, oc), st, cnt = _compare(idx, q, 50, 512)
     assert np.array_equal(gk, ok) and same_bits(gd, od)
     assert st["n_overflow"] == 0
+
+
+def test_mixed_batch_equals_individual_searches():
+    """One launch with per-query k, ef, precision and filters: each result
+    equals the same query searched alone (north-star batching)."""
+    b, q = O.dataset_gaussian(4000, 96, 32, 17)
+    idx = build(b, threads=4)
+    idx.enable_quantization()
+    rng = np.random.default_rng(1)
+    ks = rng.choice([1, 10, 100], 96)
+    efs = rng.integers(16, 300, 96)
+    precs = rng.choice([0, 1, 2], 96)
+    keys = np.arange(4000)
+    filters = [keys % 7 == 0, keys % 11 < 2, keys % 3 == 1]
+    fids = rng.choice([-1, 0, 1, 2], 96)
+    gk, gd, gc, gs = idx.search_mixed(q, ks, efs, precs, fids, filters)
+    assert (gs == 0).all()
+    for i in range(96):
+        flt = None if fids[i] < 0 else filters[fids[i]]
+        ak, ad, ac = idx.search_arrays(q[i:i + 1], int(ks[i]), int(efs[i]), H.Precision(int(precs[i])), flt)
+        assert gc[i] == ac[0]
+        assert np.array_equal(gk[i, :gc[i]], ak[0, :ac[0]]), i
+        assert same_bits(gd[i, :gc[i]], ad[0, :ac[0]]), i
+
+
+def test_mixed_batch_reports_per_query_errors():
+    b, q = O.dataset_gaussian(500, 4, 8, 3)
+    idx = build(b)
+    qq = q.copy()
+    qq[1, 0] = np.nan
+    gk, gd, gc, gs = idx.search_mixed(qq, [5, 5, 5, 5], [20, 20, 20, 20], [0, 0, 1, 0])
+    assert gs.tolist() == [0, 4, 8, 0]  # NonFiniteVector, QuantizationDisabled
+    assert gc.tolist() == [5, 0, 0, 5]
+
+
+def test_graph_captured_plan_replays_match_eager():
+    import ctypes
+    import torch
+    b, q = O.dataset_gaussian(3000, 200, 64, 23)
+    idx = build(b, threads=4)
+    h = idx.device_index()
+    dev = torch.device("cuda:0")
+    qd = torch.from_numpy(q).to(dev)
+    nq, k = 200, 10
+    outs = [torch.zeros((nq, k), dtype=torch.int64, device=dev),
+            torch.zeros((nq, k), dtype=torch.float32, device=dev),
+            torch.zeros(nq, dtype=torch.int32, device=dev),
+            torch.zeros(nq, dtype=torch.int32, device=dev)]
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    plan = ctypes.c_void_p()
+    assert H.lib.hh_search_plan_create(h, P(qd), nq, k, 48, 0, None, None, 0, None, 0,
+                                       *[P(t) for t in outs], ctypes.byref(plan)) == 0
+    ek, ed, ec = idx.search_arrays(q, k, 48)
+    stream = torch.cuda.current_stream()
+    try:
+        for rep in range(4):
+            for t in outs:
+                t.zero_()
+            assert H.lib.hh_search_plan_launch(plan, ctypes.c_void_p(stream.cuda_stream)) == 0
+            torch.cuda.synchronize()
+            gk = outs[0].cpu().numpy().view(np.uint64)
+            gd = outs[1].cpu().numpy()
+            gc = outs[2].cpu().numpy().view(np.uint32)
+            assert np.array_equal(gc, ec) and np.array_equal(gk, ek) and same_bits(gd, ed), rep
+            _ = idx.search_arrays(q[:5], 3, 16)  # eager work between replays
+    finally:
+        H.lib.hh_search_plan_free(plan)

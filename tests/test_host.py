@@ -1,0 +1,272 @@
+"""Host side of the engine (CPU only): the C ABI exports, the CPU graph
+construction (the crate's insert / insert_batch restated), the host
+quantizer and generators, each checked against the oracle or the crate's
+own published figures."""
+import ctypes
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import hnsw_amd as H
+import oracle as O
+from helpers import build, oracle_graph, same_bits
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KAT = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_kats.json")))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "hnsw_amd.h")).read()
+    declared = set(re.findall(r"\b(hh_[a-z0-9_]+)\s*\(", header))
+    assert len(declared) > 30
+    lib = ctypes.CDLL(os.path.join(ROOT, "hnsw-index_amd", "libhnsw_amd.so"))
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert lib.hh_abi_version() == 1
+
+
+def test_device_count_is_not_an_error_without_a_gpu():
+    assert H.lib.hh_device_count() >= 0
+
+
+def test_host_kernels_are_the_avx2_sequence_bit_for_bit():
+    """hh_cpu_distance runs the crate's AVX2 intrinsic sequence
+    (src/kernels/x86.rs); the oracle restates it lane by lane in C."""
+    rng = np.random.default_rng(5)
+    for n in list(range(0, 70)) + [96, 100, 128, 255, 256, 960]:
+        for _ in range(3):
+            a = rng.standard_normal(n).astype(np.float32) * 3
+            b = rng.standard_normal(n).astype(np.float32) * 3
+            for metric in (0, 1, 2):
+                host = H.lib.hh_cpu_distance(metric, a.ctypes.data, b.ctypes.data, n)
+                want = O.distance(metric, a, b, O.AVX2)
+                assert same_bits([host], [want]), (n, metric)
+
+
+def test_generators_match_the_oracle():
+    b, q = O.dataset_gaussian(300, 20, 17, 42)
+    d = H.synthetic.Dataset.gaussian(300, 20, 17, 42)
+    assert np.array_equal(d.base(), b) and np.array_equal(d.queries(), q)
+    b, q = O.dataset_clustered(300, 20, 9, 7, 3)
+    d = H.synthetic.Dataset.clustered(300, 20, 9, 7, 3)
+    assert np.array_equal(d.base(), b) and np.array_equal(d.queries(), q)
+
+
+def test_benchmark_generators_have_the_stated_distributions():
+    d = H.synthetic.Dataset.uniform_clusters(20000, 2000, 32, 50, 0.1, 0.12, 42, threads=4)
+    again = H.synthetic.Dataset.uniform_clusters(20000, 2000, 32, 50, 0.1, 0.12, 42, threads=1)
+    assert np.array_equal(d.base(), again.base()), "independent of the thread count"
+    b = d.base().astype(np.float64)
+    assert -1.5 < b.min() and b.max() < 1.5
+    # nearest of 50 centres is within ~0.1*sqrt(32) of every row
+    u = H.synthetic.Dataset.unit_uniform(1000, 10, 960, 1)
+    assert np.allclose(np.linalg.norm(u.base().astype(np.float64), axis=1), 1.0, atol=1e-5)
+    assert (u.base() >= 0).all()
+    ln = H.synthetic.Dataset.lognormal(5000, 10, 64, 0.5, 20, 9)
+    norms = np.linalg.norm(ln.base().astype(np.float64), axis=1) / 8.0
+    assert abs(np.log(norms).std() - 0.5) < 0.05
+    assert set(np.unique(ln.categories)) == set(range(20))
+
+
+def test_host_quantizer_matches_the_oracle_bit_for_bit():
+    for metric in (H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine):
+        b, _ = O.dataset_gaussian(500, 1, 24, 4)
+        idx = build(b, metric, threads=2)
+        idx.enable_quantization()
+        v = idx.graph_arrays()
+        off, scale = O.sq_train(int(metric), v["vectors"])
+        assert np.array_equal(v["sq_offset"], off) and v["sq_scale"] == scale
+        assert np.array_equal(v["codes"], O.sq_encode_rows(off, scale, v["vectors"]))
+
+
+def _assert_invariants(v):
+    """src/hnsw/tests.rs:28-53"""
+    n, m, m0 = v["n"], v["m"], v["m0"]
+    lay0 = v["layer0"].reshape(n, m0 + 1)
+    for node in range(n):
+        level = int(v["levels"][node])
+        for layer in range(level + 1):
+            if layer == 0:
+                cnt = min(int(lay0[node, 0]), m0)
+                nb = lay0[node, 1:1 + cnt]
+                cap = m0
+            else:
+                o = int(v["upper_off"][node]) + (layer - 1) * (m + 1)
+                cnt = min(int(v["upper"][o]), m)
+                nb = v["upper"][o + 1:o + 1 + cnt]
+                cap = m
+            assert cnt <= cap
+            assert (nb < n).all(), "dangling id"
+            assert (nb != node).all(), "self loop"
+            assert (v["levels"][nb] >= layer).all()
+            assert len(set(nb.tolist())) == len(nb), "duplicate neighbour"
+    if n:
+        assert int(v["levels"][v["entry_id"]]) == v["top"] == int(v["levels"].max())
+
+
+def _reachable(v):
+    """src/hnsw/tests.rs:57-79"""
+    n, m0 = v["n"], v["m0"]
+    lay0 = v["layer0"].reshape(n, m0 + 1)
+    seen = np.zeros(n, bool)
+    queue = [i for i in range(n) if i == v["entry_id"] or v["levels"][i] > 0]
+    seen[queue] = True
+    while queue:
+        nxt = []
+        for node in queue:
+            for nb in lay0[node, 1:1 + min(int(lay0[node, 0]), m0)]:
+                if not seen[nb]:
+                    seen[nb] = True
+                    nxt.append(int(nb))
+        queue = nxt
+    return int(seen.sum())
+
+
+@pytest.mark.parametrize("selection", [H.NeighborSelection.simple(), H.NeighborSelection(),
+                                       H.NeighborSelection.heuristic_with(True, True)])
+@pytest.mark.parametrize("threads", [1, 4])
+def test_graphs_satisfy_invariants_and_stay_connected(selection, threads):
+    """src/hnsw/tests.rs:137-166"""
+    b, _ = O.dataset_clustered(3000, 1, 16, 20, 5)
+    idx = build(b, params=H.HnswParams(8).with_selection(selection), threads=threads)
+    v = idx.graph_arrays()
+    _assert_invariants(v)
+    if selection.heuristic:
+        assert _reachable(v) >= 2970
+    deg = np.minimum(v["layer0"].reshape(3000, 17)[:, 0], 16).mean()
+    assert 4.0 < deg <= 16.0
+
+
+def test_sequential_build_is_deterministic():
+    """src/hnsw/tests.rs:129-135"""
+    b, _ = O.dataset_gaussian(500, 10, 8, 1)
+    ia, ic = build(b), build(b)  # graph_arrays() borrows from the live index
+    a, c = ia.graph_arrays(), ic.graph_arrays()
+    for key in ("layer0", "upper", "upper_off", "levels", "keys"):
+        assert np.array_equal(a[key], c[key])
+    assert a["entry_id"] == c["entry_id"]
+
+
+def test_levels_follow_the_crate_hash_and_distribution():
+    """sample_level (src/hnsw/build.rs:19-28) and its tests (:229-275)."""
+    b = np.zeros((20000, 1), np.float32) + np.arange(20000, dtype=np.float32)[:, None]
+    idx = build(b, params=H.HnswParams(16).with_seed(42), threads=8)
+    levels = idx.graph_arrays()["levels"]
+    ml = 1.0 / math.log(16)
+    for i in (0, 1, 2, 12345, 19999):
+        h = O.mix64(42 ^ O.mix64((i + 0x9E3779B97F4A7C15) % 2**64))
+        u = ((h >> 11) + 1) * 2.0**-53
+        assert levels[i] == min(31, math.floor(-math.log(u) * ml))
+    for lv in (1, 2):
+        expected = 20000 * 16.0**-lv
+        assert abs((levels >= lv).sum() - expected) < 5 * math.sqrt(expected) + 2
+
+
+def test_parameter_validation():
+    """src/params.rs:204-223, src/hnsw/tests.rs:97-111"""
+    for p in [H.HnswParams(1), H.HnswParams(513), H.HnswParams(16).with_m0(15),
+              H.HnswParams(16).with_m0(4096), H.HnswParams(16).with_ef_construction(0),
+              H.HnswParams(16).with_level_multiplier(0.0),
+              H.HnswParams(16).with_level_multiplier(float("nan")),
+              H.HnswParams(16).with_level_multiplier(11.0)]:
+        with pytest.raises(H.InvalidParameter):
+            H.HnswIndex(4, H.Metric.L2, p)
+    with pytest.raises(H.InvalidDimension):
+        H.HnswIndex(0)
+    with pytest.raises(H.InvalidDimension):
+        H.HnswIndex(65537)
+
+
+def test_insert_rejects_invalid_input_without_side_effects():
+    """src/hnsw/tests.rs:168-206"""
+    idx = H.HnswIndex(2, H.Metric.Cosine)
+    idx.insert(1, [1.0, 0.0])
+    with pytest.raises(H.DuplicateKey):
+        idx.insert(1, [0.0, 1.0])
+    with pytest.raises(H.DimensionMismatch):
+        idx.insert(2, [1.0])
+    with pytest.raises(H.NonFiniteVector):
+        idx.insert(2, [np.nan, 1.0])
+    with pytest.raises(H.ZeroVector):
+        idx.insert(2, [0.0, 0.0])
+    with pytest.raises(H.Error):
+        idx.insert_batch([2, 3], np.array([1.0, 1.0, 0.0, 0.0]))
+    with pytest.raises(H.DuplicateKey):
+        idx.insert_batch([2, 2], np.array([1.0, 1.0, 1.0, 2.0]))
+    assert len(idx) == 1 and idx.node_count() == 1
+
+
+def test_recall_table_is_reproduced_exactly():
+    """docs/benchmarks.md:162-169: sequential builds on gaussian 10k x 128,
+    200 queries, k=10 — the oracle's search on the host-built graph gives
+    the published figure at every ef."""
+    k = KAT["recall_table_gaussian_10k"]
+    base, queries = O.dataset_gaussian(*k["data"])
+    tk, _, tc = O.brute_search_batch(O.L2, np.arange(10000), base, queries, 10)
+    for m, want in (("m16", k["m16"]), ("m32", k["m32"])):
+        g = oracle_graph(build(base, m=int(m[1:]), threads=1))
+        got = []
+        for ef in k["efs"]:
+            fk, _, fc, _ = g.search_batch(queries, 10, ef)
+            got.append(O.recall_at_k(tk, tc, fk, fc, 10))
+        assert np.allclose(got, want, atol=k["noise"]), (m, got)
+
+
+def test_recall_floors_of_the_crate():
+    """tests/recall.rs:74-128 on host-built graphs, searched by the oracle."""
+    f = KAT["recall_floors"]
+    base, queries = O.dataset_clustered(10000, 200, 128, 50, 42)
+    tk, _, tc = O.brute_search_batch(O.L2, np.arange(10000), base, queries, 10)
+    g = oracle_graph(build(base, threads=1))
+    fk, _, fc, _ = g.search_batch(queries, 10, f["ef"])
+    assert O.recall_at_k(tk, tc, fk, fc, 10) >= f["clustered50_m16"]
+    base, queries = O.dataset_gaussian(5000, 100, 64, 7)
+    for metric in (O.COSINE, O.IP):
+        prepared = np.stack([O.prepare(metric, 64, r)[1] for r in base])
+        tk, _, tc = O.brute_search_batch(metric, np.arange(5000), prepared, queries, 10)
+        g = oracle_graph(build(base, H.Metric(metric), threads=1))
+        fk, _, fc, _ = g.search_batch(queries, 10, f["ef"])
+        assert O.recall_at_k(tk, tc, fk, fc, 10) >= f["cosine_ip_5k_64"]
+
+
+def test_parallel_build_and_int8_recall_floors():
+    """tests/recall.rs:98-128"""
+    f = KAT["recall_floors"]
+    base, queries = O.dataset_gaussian(10000, 200, 128, 42)
+    tk, _, tc = O.brute_search_batch(O.L2, np.arange(10000), base, queries, 10)
+    idx = build(base, m=32, threads=4)
+    g = oracle_graph(idx)
+    fk, _, fc, _ = g.search_batch(queries, 10, f["ef"])
+    assert O.recall_at_k(tk, tc, fk, fc, 10) >= f["parallel4_m32"]
+    idx.enable_quantization()
+    g = oracle_graph(idx)
+    fk, _, fc, _ = g.search_batch(queries, 10, f["ef"], quantized=True)
+    assert O.recall_at_k(tk, tc, fk, fc, 10) >= f["int8_rerank_m32"]
+
+
+def test_oracle_search_edge_cases_on_host_graphs():
+    """src/hnsw/tests.rs:113-127 and :208-218 with the oracle as searcher."""
+    b, q = O.dataset_gaussian(16, 20, 8, 3)
+    g = oracle_graph(build(b, params=H.HnswParams(16).with_selection(H.NeighborSelection.simple())))
+    for i in range(20):
+        ok, od, _, _ = g.search(q[i], 5, 5)
+        bk, bd = O.brute_search(O.L2, np.arange(16), b, q[i], 5)
+        assert np.array_equal(ok, bk) and same_bits(od, bd)
+    b, q = O.dataset_gaussian(100, 1, 4, 2)
+    g = oracle_graph(build(b))
+    assert len(g.search(q[0], 0, 10)[0]) == 0
+    assert len(g.search(q[0], 10, 1)[0]) == 10
+    assert len(g.search(q[0], 500, 500)[0]) == 100
+
+
+def test_search_needs_a_gpu_and_never_falls_back_to_the_cpu():
+    if H.lib.hh_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    b, q = O.dataset_gaussian(50, 1, 4, 2)
+    idx = build(b)
+    with pytest.raises(H.DeviceError):
+        idx.search(q[0], 5, 10)
