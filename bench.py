@@ -58,9 +58,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import hnsw_amd as H
     from hnsw_amd import _lib as L
+    from hnsw_amd.device import Buffer, Stream, synchronize
 
     if H.lib.hh_device_count() < 1:
         print("no GPU visible", file=sys.stderr)
@@ -81,14 +81,12 @@ def main():
         log(f"graph build: {e}/{n} linked ({time.time() - t0:.1f} s, {threads} threads)")
     build_s = time.time() - t0
 
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
-    sptr = ctypes.c_void_p(stream.cuda_stream)
-    q_dev = torch.from_numpy(data.queries()).to(dev)
-    out_k = torch.zeros((nq, k), dtype=torch.int64, device=dev)
-    out_d = torch.zeros((nq, k), dtype=torch.float32, device=dev)
-    out_c = torch.zeros(nq, dtype=torch.int32, device=dev)
-    status = torch.zeros(nq, dtype=torch.int32, device=dev)
+    # device buffers and the launch stream, on the library's own HIP runtime
+    stream = Stream()
+    sptr = stream.handle
+    q_dev = Buffer.from_array(data.queries())
+    out_k, out_d = Buffer(nq * k * 8), Buffer(nq * k * 4)
+    out_c, status = Buffer(nq * 4), Buffer(nq * 4)
     log("uploading the index")
     h = idx.device_index()
 
@@ -96,9 +94,23 @@ def main():
     log("ground truth: GPU brute force over all rows")
     bf = H.BruteForceIndex(dim, H.Metric.L2)
     bf.insert_batch(keys, data.base())
+    bf.search_arrays(data.queries()[:256], k)  # warm-up (workspace, code objects)
     t = time.time()
     tk, td, tc = bf.search_arrays(data.queries(), k)
     bf_s = time.time() - t
+    mfma_used, mfma_redo = bf.mfma_stats()
+    bf.set_mfma(0)
+    t = time.time()
+    ek, ed, ec = bf.search_arrays(data.queries(), k)
+    stream_s = time.time() - t
+    bf.set_mfma(1)
+    same = bool(np.array_equal(ek, tk) and np.array_equal(ed.view(np.uint32), td.view(np.uint32)))
+    if not same:
+        raise SystemExit("MFMA brute force differs from the streaming kernel")
+    gemm_tflops = 2.0 * nq * n * dim / bf_s / 1e12
+    log(f"brute force: MFMA shortlist {bf_s:.3f} s ({mfma_used} queries, {mfma_redo} re-run "
+        f"exactly, {gemm_tflops:.1f} TFLOP/s of 2*Q*N*D), streaming kernel {stream_s:.3f} s, "
+        f"identical results")
     f64n = min(args.f64_check, nq)
     fk, fd, fc = bf.search_arrays(data.queries()[:f64n], k, f64=True)
     f64_agree = float(np.mean([len(set(tk[i].tolist()) & set(fk[i].tolist())) / k
@@ -109,24 +121,24 @@ def main():
 
     def run(ef, precision):
         st = H.lib.hh_search_batch_device(
-            h, ctypes.c_void_p(q_dev.data_ptr()), nq, k, ef, int(precision), None,
-            ctypes.c_void_p(out_k.data_ptr()), ctypes.c_void_p(out_d.data_ptr()),
-            ctypes.c_void_p(out_c.data_ptr()), ctypes.c_void_p(status.data_ptr()), sptr)
+            h, q_dev.ptr, nq, k, ef, int(precision), None,
+            out_k.ptr, out_d.ptr, out_c.ptr, status.ptr, sptr)
         if st:
             raise RuntimeError(L.last_error())
 
     def recall():
-        gk = out_k.cpu().numpy().view(np.uint64)
-        gc = out_c.cpu().numpy().view(np.uint32)
+        stream.synchronize()
+        gk = out_k.download((nq, k), np.uint64)
+        gc = out_c.download(nq, np.uint32)
         return H.lib.hh_recall_at_k(nq, tk.ctypes.data, tc.ctypes.data, k, gk.ctypes.data,
                                     gc.ctypes.data, k, k)
 
     def timed(ef, precision, steps):
-        torch.cuda.synchronize()
+        synchronize()
         t = time.perf_counter()
         for _ in range(steps):
             run(ef, precision)
-        torch.cuda.synchronize()
+        synchronize()
         return (time.perf_counter() - t) / steps
 
     sweep = []
@@ -139,16 +151,25 @@ def main():
     chosen = next((s for s in sweep if s["recall"] >= args.target), sweep[-1])
     ef = chosen["ef"]
 
+    # autotune the launch shape for this ef (untimed)
+    cap, wpb = ctypes.c_uint32(), ctypes.c_int()
+    st = H.lib.hh_index_autotune(h, q_dev.ptr, nq, k, ef, 0,
+                                 out_k.ptr, out_d.ptr, out_c.ptr, status.ptr,
+                                 sptr, ctypes.byref(cap), ctypes.byref(wpb))
+    if st:
+        raise RuntimeError(L.last_error())
+    log(f"autotune ef={ef}: visited-set capacity {cap.value}, {wpb.value} waves per block")
+
     # timed region
     for _ in range(args.warmup):
         run(ef, H.Precision.F32)
-    torch.cuda.synchronize()
+    synchronize()
     step_ms = []
     t_all = time.perf_counter()
     for _ in range(args.steps):
         t = time.perf_counter()
         run(ef, H.Precision.F32)
-        torch.cuda.synchronize()
+        synchronize()
         step_ms.append((time.perf_counter() - t) * 1e3)
     total = time.perf_counter() - t_all
     qps = nq * args.steps / total
@@ -165,7 +186,7 @@ def main():
     H.lib.hh_index_set_timing(h, 0)
     idx.search_stats(reset=True)
     run(ef, H.Precision.F32)
-    torch.cuda.synchronize()
+    synchronize()
     st = idx.search_stats()
     n_dist = st["n_dist"] + st["n_upper"]
     alg_bytes = n_dist * dim * 4 + st["n_rerank"] * dim * 4
@@ -193,6 +214,31 @@ def main():
     variants["int8"] = {"recall": round(r, 4), "qps": round(nq / sec)}
     log(f"int8 ef={ef}: recall={r:.4f} qps={nq / sec:,.0f}")
 
+    # serving loop: Poisson arrivals replayed by the native load generator
+    serve = None
+    try:
+        rate = min(0.5 * qps, 400_000.0)
+        nreq = 50_000
+        rng = np.random.default_rng(7)
+        arrivals = np.cumsum(rng.exponential(1e6 / rate, nreq))
+        qs = data.queries()[np.arange(nreq) % nq]
+        with H.Server(idx, k_max=k, max_inflight=2) as srv:
+            t = time.perf_counter()
+            lat, sk, sd, sc, sst = srv.replay(qs, np.full(nreq, k), np.full(nreq, ef), arrivals)
+            wall = time.perf_counter() - t
+            sstats = srv.stats()
+        served_recall = H.lib.hh_recall_at_k(nq, tk.ctypes.data, tc.ctypes.data, k,
+                                             np.ascontiguousarray(sk[:nq]).ctypes.data,
+                                             np.ascontiguousarray(sc[:nq]).ctypes.data, k, k)
+        serve = {"offered_qps": round(rate), "achieved_qps": round(nreq / wall),
+                 "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                 "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                 "requests": nreq, "launches": sstats["batches"],
+                 "recall_at_10": round(served_recall, 4)}
+        log(f"serving: {serve}")
+    except Exception as e:
+        log(f"serving measurement failed: {e}")
+
     # CPU baseline: the oracle's restatement of search_batch on host cores
     cpu = None
     try:
@@ -204,15 +250,31 @@ def main():
                           v["top"])
         ns = min(args.cpu_sample, nq)
         sample = data.queries()[:ns]
-        g.search_batch(sample[:64], k, ef, threads=threads)
-        t = time.perf_counter()
+        load0 = os.getloadavg()[0]
+        # warm-up pass (also the recall of the CPU results), then the median
+        # of 3 timed passes on all threads, then one single-thread pass
         ok, od, oc, _ = g.search_batch(sample, k, ef, threads=threads)
-        cpu_s = time.perf_counter() - t
+        cpu_recall = O.recall_at_k(tk[:ns], tc[:ns], ok, oc, k)
+        passes = []
+        for _ in range(3):
+            t = time.perf_counter()
+            g.search_batch(sample, k, ef, threads=threads)
+            passes.append(time.perf_counter() - t)
+        cpu_s = sorted(passes)[1]
+        n1 = min(ns, 2000)
+        t = time.perf_counter()
+        g.search_batch(sample[:n1], k, ef, threads=1)
+        one_s = time.perf_counter() - t
+        load1 = os.getloadavg()[0]
         cpu = {"value": round(ns / cpu_s), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"{ns} of the {nq} queries, same graph, k={k}, ef={ef}, "
-                         f"{cpu_s:.1f} s on {threads} threads (oracle C restatement of "
-                         "search_batch with AVX2-order kernels)"}
-        log(f"cpu baseline {ns / cpu_s:,.0f} qps on {threads} threads")
+               "sample": f"{ns} of the {nq} queries, same graph, k={k}, ef={ef}: warm-up pass, "
+                         f"then median of 3 passes ({', '.join(f'{p:.2f}' for p in passes)} s) "
+                         f"on {threads} threads; oracle C restatement of search_batch with "
+                         "AVX2-order kernels",
+               "single_thread_qps": round(n1 / one_s), "recall_at_10": round(cpu_recall, 4),
+               "loadavg_start_end": [round(load0, 2), round(load1, 2)]}
+        log(f"cpu baseline {ns / cpu_s:,.0f} qps on {threads} threads, "
+            f"{n1 / one_s:,.0f} qps on 1 thread, recall {cpu_recall:.4f}")
     except Exception as e:  # the baseline is reported, never required
         log(f"cpu baseline unavailable: {e}")
 
@@ -245,8 +307,13 @@ def main():
         "adjacency_bytes_per_launch": adj_bytes,
         "sweep_f32": sweep,
         "variants": variants,
-        "ground_truth": {"brute_force_s": round(bf_s, 3), "f64_agreement": round(f64_agree, 5)},
+        "serving": serve,
+        "ground_truth": {"brute_force_s": round(bf_s, 3), "f64_agreement": round(f64_agree, 5),
+                         "mfma_queries": mfma_used, "mfma_rerun_exact": mfma_redo,
+                         "streaming_kernel_s": round(stream_s, 3),
+                         "gemm_tflops_2QND": round(gemm_tflops, 2)},
         "graph_build_s": round(build_s, 1),
+        "launch_shape": {"visited_set_capacity": cap.value, "waves_per_block": wpb.value},
     }
     print(json.dumps(line), flush=True)
 

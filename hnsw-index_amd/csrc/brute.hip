@@ -104,6 +104,12 @@ __global__ void __launch_bounds__(256) brute_partial_kernel(const BruteArgs A) {
   const uint64_t r0 = slice * A.rows_per_slice;
   const uint64_t r1 = r0 + A.rows_per_slice < A.n ? r0 + A.rows_per_slice : A.n;
   const Shape s{A.T, A.U, A.R, A.stride};
+  if (A.only) {
+    bool any = false;
+    for (int b = 0; b < QB; b++)
+      if (g * QB + b < A.nq && G(A.only)[g * QB + b]) any = true;
+    if (!any) return;
+  }
   QLanes<TMAX, EXTRA> q[QB];
   int qvalid[QB];
 #pragma unroll
@@ -113,7 +119,7 @@ __global__ void __launch_bounds__(256) brute_partial_kernel(const BruteArgs A) {
     q[b].e[0] = q[b].r = 0.0f;
     if constexpr (EXTRA) q[b].e[1] = q[b].e[2] = 0.0f;
     const uint64_t qi = g * QB + b;
-    qvalid[b] = qi < A.nq && G(A.status)[qi] == 0;
+    qvalid[b] = qi < A.nq && G(A.status)[qi] == 0 && (!A.only || G(A.only)[qi]);
     if (qi < A.nq) load_q<TMAX, EXTRA, float>(A.qprep + qi * A.stride, s, i, q[b]);
   }
   KList<float, S> L[QB];
@@ -183,8 +189,9 @@ __global__ void __launch_bounds__(256) brute_merge_kernel(const BruteArgs A) {
   const int lane = lane_id();
   const uint64_t qi = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (qi >= A.nq) return;
-  if (A.status[qi] != 0) {
-    if (lane == 0) A.out_count[qi] = 0;
+  if (A.only && !G(A.only)[qi]) return;
+  if (G(A.status)[qi] != 0) {
+    if (lane == 0) GM(A.out_count)[qi] = 0;
     return;
   }
   KList<float, S> L;
@@ -284,8 +291,8 @@ __global__ void __launch_bounds__(256) brute_merge_f64_kernel(const BruteArgs A)
   const int lane = lane_id();
   const uint64_t qi = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (qi >= A.nq) return;
-  if (A.status[qi] != 0) {
-    if (lane == 0) A.out_count[qi] = 0;
+  if (G(A.status)[qi] != 0) {
+    if (lane == 0) GM(A.out_count)[qi] = 0;
     return;
   }
   KList<double, S> L;

@@ -30,6 +30,28 @@ struct BruteArgs {
   float *out_dist;
   double *out_dist64;
   uint32_t *out_count;
+  const uint32_t *only;     // if set, queries with only[q] == 0 are skipped entirely
+};
+
+// MFMA shortlist passes (brute_mfma.hip).
+struct MfmaArgs {
+  const float *rows;
+  uint64_t n;
+  uint32_t stride, dim, metric;
+  const float *qprep;
+  const int32_t *status;
+  uint64_t nq;
+  const float *xx;          // per-row squared norms (L2)
+  const float *qq;          // per-query squared norms (L2)
+  uint32_t mode;            // 0: dense sample distances, 1: candidate filter
+  uint64_t nrows, row_step; // rows of this pass: r * row_step, r < nrows
+  float *dense;             // mode 0: nq x nrows
+  float *tau;               // per-query threshold
+  uint32_t *cand;           // mode 1: nq x cand_cap row ids
+  uint32_t *cand_count;
+  uint32_t cand_cap;
+  double max_norm;          // largest row norm (error bound)
+  uint32_t *redo;           // per query: 1 = re-run exactly
 };
 
 template <int TMAX, bool EXTRA, int QB, int S>
@@ -40,5 +62,13 @@ template <int S>
 __global__ void brute_f64_kernel(const BruteArgs A);
 template <int S>
 __global__ void brute_merge_f64_kernel(const BruteArgs A);
+template <int KS>
+__global__ void brute_mfma_kernel(const MfmaArgs A);
+__global__ void brute_tau_kernel(const MfmaArgs A);
+template <int TMAX, bool EXTRA, int S>
+__global__ void brute_rerank_kernel(const MfmaArgs A, const BruteArgs B);
+__global__ void row_norms_kernel(const float *rows, uint64_t n, uint32_t stride, float *xx);
+__global__ void query_norms_kernel(const float *qprep, uint64_t nq, uint32_t stride, float *qq);
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v);
 
 }  // namespace hhk

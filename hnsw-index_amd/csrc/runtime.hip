@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <condition_variable>
 #include <memory>
@@ -75,6 +76,11 @@ __global__ void to_bf16_kernel(const float *src, uint16_t *dst, uint64_t count) 
   }
 }
 
+// Candidates a query may hold beyond its register list (8 bytes each, per
+// resident wavefront).  Only a query that visits more than this many
+// not-yet-expanded nodes drops candidates (counted in n_overflow).
+constexpr uint32_t kCandPool = 8192;
+
 int grid_for(uint64_t work) {
   uint64_t g = (work + 255) / 256;
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, 65536));
@@ -133,6 +139,7 @@ struct Workspace {
   uint32_t *counter = nullptr;
   unsigned long long *stats = nullptr;
   uint32_t *spill = nullptr;
+  uint64_t *cpool = nullptr;  // per-wave candidate pools (kCandPool entries)
   uint64_t spill_slots = 0;
   // brute-force partial lists
   float *part_d = nullptr;
@@ -140,6 +147,32 @@ struct Workspace {
   uint64_t *part_k = nullptr;
   uint32_t *part_n = nullptr;
   uint64_t cap_part = 0, cap_partn = 0;
+  // MFMA shortlist buffers
+  float *m_dense = nullptr, *m_tau = nullptr, *m_qq = nullptr;
+  uint32_t *m_cand = nullptr, *m_cnt = nullptr, *m_redo = nullptr;
+  uint64_t cap_mq = 0, cap_dense = 0, cap_cand = 0;
+
+  int ensure_mfma(uint64_t nq, uint64_t dense, uint64_t cand) {
+    if (nq <= cap_mq && dense <= cap_dense && cand <= cap_cand) return HH_OK;
+    HH_HIP(hipEventSynchronize(done));
+    dfree(m_dense);
+    dfree(m_tau);
+    dfree(m_qq);
+    dfree(m_cand);
+    dfree(m_cnt);
+    dfree(m_redo);
+    cap_mq = std::max(nq, cap_mq);
+    cap_dense = std::max(dense, cap_dense);
+    cap_cand = std::max(cand, cap_cand);
+    int st = dalloc(&m_dense, cap_dense);
+    if (!st) st = dalloc(&m_tau, cap_mq);
+    if (!st) st = dalloc(&m_qq, cap_mq);
+    if (!st) st = dalloc(&m_cand, cap_cand);
+    if (!st) st = dalloc(&m_cnt, cap_mq);
+    if (!st) st = dalloc(&m_redo, cap_mq);
+    if (st) cap_mq = cap_dense = cap_cand = 0;
+    return st;
+  }
 
   int init() {
     HH_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
@@ -158,10 +191,17 @@ struct Workspace {
     dfree(counter);
     dfree(stats);
     dfree(spill);
+    dfree(cpool);
     dfree(part_d);
     dfree(part_d64);
     dfree(part_k);
     dfree(part_n);
+    dfree(m_dense);
+    dfree(m_tau);
+    dfree(m_qq);
+    dfree(m_cand);
+    dfree(m_cnt);
+    dfree(m_redo);
     if (done) (void)hipEventDestroy(done);
     done = nullptr;
   }
@@ -186,7 +226,9 @@ struct Workspace {
     if (slots <= spill_slots) return HH_OK;
     HH_HIP(hipEventSynchronize(done));
     dfree(spill);
+    dfree(cpool);
     int st = dalloc(&spill, slots * words);
+    if (!st) st = dalloc(&cpool, slots * kCandPool);
     if (st) {
       spill_slots = 0;
       return st;
@@ -252,7 +294,10 @@ struct hh_index {
   std::vector<std::unique_ptr<Workspace>> pool;
   std::vector<bool> pool_busy;
   Workspace stats_ws;  // holds the stats counters shared by every launch
-  uint32_t tuned_hash_cap = 0;  // set by hh_index_autotune (0 = rule of thumb)
+  // launch shape chosen by hh_index_autotune per ef bucket (index log2 of
+  // the list slots: ef <= 64, 128, 256, 512); 0 = rule of thumb
+  uint32_t tuned_hash_cap[4] = {0, 0, 0, 0};
+  int tuned_wpb[4] = {0, 0, 0, 0};
   hipStream_t stream = nullptr;  // used by the synchronous host-memory API
   // optional timing of the search kernel (hh_index_set_timing)
   bool timing = false;
@@ -285,12 +330,17 @@ struct hh_brute {
   hh::RowLayout lay;
   float *rows = nullptr;
   uint64_t *keys = nullptr;
+  float *xx = nullptr;       // per-row squared norms (f32), MFMA L2 form
+  double max_norm = 0.0;     // largest prepared row norm
+  int mfma_mode = 1;          // 0 exact streaming only, 1 automatic, 2 MFMA when supported
+  uint64_t last_mfma = 0;    // queries of the last batch that took the MFMA path
   Workspace ws;
   ~hh_brute() {
     (void)hipSetDevice(device);
     ws.release();
     dfree(rows);
     dfree(keys);
+    dfree(xx);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -388,14 +438,18 @@ int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
   // evaluates roughly 10-15 ef distances on the benchmark data), at least
   // 2048 so the result arrays fit in it afterwards; past 3/4 full the query
   // continues on its HBM bitset.
-  cfg.hash_cap = idx->tuned_hash_cap ? idx->tuned_hash_cap
-                                     : std::min<uint32_t>(16384, std::max<uint32_t>(2048, efmax * 32));
+  const int bucket = sw == 1 ? 0 : sw == 2 ? 1 : sw == 4 ? 2 : 3;
+  cfg.hash_cap = idx->tuned_hash_cap[bucket]
+                     ? idx->tuned_hash_cap[bucket]
+                     : std::min<uint32_t>(16384, std::max<uint32_t>(2048, efmax * 32));
   cfg.hash_cap = std::max<uint32_t>(cfg.hash_cap, 256u * (uint32_t)sw);
   cfg.lds_words = cfg.hash_cap + 128;
   cfg.kern = pick_search_kernel(tmax_for(idx->lay), idx->lay.U > 0 || idx->lay.R > 0, sw);
-  // waves per block: whichever of 1, 2, 4 keeps the most waves resident
+  // waves per block: the tuned value, else whichever of 1, 2, 4 keeps the
+  // most waves resident
   int blocks_per_cu = 1, best_waves = 0;
   for (int w : {4, 2, 1}) {
+    if (idx->tuned_wpb[bucket] && w != idx->tuned_wpb[bucket]) continue;
     if (cfg.lds_words * 4 * w > 65536) continue;
     int b = 0;
     HH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, cfg.kern, w * 64, cfg.lds_words * 4 * w));
@@ -481,6 +535,8 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.counter = ws.counter;
   a.spill = ws.spill;
   a.spill_words = cfg.spill_words;
+  a.cpool = ws.cpool;
+  a.cpool_cap = kCandPool;
   a.stats = idx->stats_ws.stats;
   a.hash_cap = cfg.hash_cap;
   a.hash_limit = cfg.hash_cap / 2 + cfg.hash_cap / 4;
@@ -597,6 +653,71 @@ int hh_device_count(void) {
     return 0;
   }
   return n;
+}
+
+int hh_device_alloc(int device, uint64_t bytes, void **out) {
+  if (!out) return fail(HH_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  HH_HIP(hipSetDevice(device));
+  if (bytes == 0) return HH_OK;
+  if (hipMalloc(out, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return fail(HH_OUT_OF_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  return HH_OK;
+}
+
+int hh_device_free(void *p) {
+  if (p) HH_HIP(hipFree(p));
+  return HH_OK;
+}
+
+int hh_memcpy(void *dst, const void *src, uint64_t bytes, int kind, void *stream) {
+  if (bytes == 0) return HH_OK;
+  if (!dst || !src) return fail(HH_INVALID_ARGUMENT, "null pointer");
+  hipMemcpyKind k;
+  switch (kind) {
+    case 0: k = hipMemcpyHostToDevice; break;
+    case 1: k = hipMemcpyDeviceToHost; break;
+    case 2: k = hipMemcpyDeviceToDevice; break;
+    default: return fail(HH_INVALID_ARGUMENT, "unknown copy kind");
+  }
+  if (stream) HH_HIP(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream));
+  else HH_HIP(hipMemcpy(dst, src, bytes, k));
+  return HH_OK;
+}
+
+int hh_memset(void *dst, int value, uint64_t bytes) {
+  if (bytes == 0) return HH_OK;
+  if (!dst) return fail(HH_INVALID_ARGUMENT, "null pointer");
+  HH_HIP(hipMemset(dst, value, bytes));
+  return HH_OK;
+}
+
+int hh_stream_create(int device, void **out) {
+  if (!out) return fail(HH_INVALID_ARGUMENT, "null argument");
+  HH_HIP(hipSetDevice(device));
+  hipStream_t s = nullptr;
+  HH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *out = s;
+  return HH_OK;
+}
+
+int hh_stream_destroy(void *stream) {
+  if (stream) HH_HIP(hipStreamDestroy((hipStream_t)stream));
+  return HH_OK;
+}
+
+int hh_stream_synchronize(void *stream) {
+  HH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return HH_OK;
+}
+
+int hh_device_synchronize(int device) {
+  HH_HIP(hipSetDevice(device));
+  HH_HIP(hipDeviceSynchronize());
+  return HH_OK;
 }
 
 int hh_index_upload(const hh_graph_desc *d, int device, hh_index **out) {
@@ -727,6 +848,7 @@ int hh_index_stats(hh_index *idx, hh_search_stats *out) {
   out->n_rerank = v[4];
   out->n_overflow = v[5];
   out->n_visited_spill = v[6];
+  out->n_cand_spill = v[7];
   return HH_OK;
 }
 
@@ -747,6 +869,62 @@ int hh_index_last_kernel_ms(hh_index *idx, float *ms) {
   HH_HIP(hipSetDevice(idx->device));
   HH_HIP(hipEventSynchronize(idx->t1));
   HH_HIP(hipEventElapsedTime(ms, idx->t0, idx->t1));
+  return HH_OK;
+}
+
+int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
+                      uint32_t precision, uint64_t *out_keys, float *out_dist,
+                      uint32_t *out_count, int32_t *status, void *stream,
+                      uint32_t *chosen_hash_cap, int *chosen_wpb) {
+  if (!idx || !queries || nq == 0) return fail(HH_INVALID_ARGUMENT, "null or empty argument");
+  const uint32_t efmax = std::max<uint32_t>(std::max(ef, k), 1);
+  const int sw = slots_for(efmax);
+  if (!sw) return fail(HH_UNSUPPORTED, "ef (raised to k) above 512 is not supported");
+  const int bucket = sw == 1 ? 0 : sw == 2 ? 1 : sw == 4 ? 2 : 3;
+  HH_HIP(hipSetDevice(idx->device));
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  HH_HIP(hipEventCreate(&e0));
+  HH_HIP(hipEventCreate(&e1));
+  float best_ms = 1e30f;
+  uint32_t best_cap = 0;
+  int best_wpb = 0;
+  const uint32_t caps[] = {1024, 1536, 2048, 3072, 4096, 6144, 8192, 12288, 16384};
+  int st = HH_OK;
+  for (uint32_t cap : caps) {
+    if (cap < 256u * (uint32_t)sw || cap * 4 < efmax * 16) continue;
+    for (int w : {1, 2, 4}) {
+      if ((cap + 128) * 4 * (uint32_t)w > 65536) continue;
+      idx->tuned_hash_cap[bucket] = cap;
+      idx->tuned_wpb[bucket] = w;
+      float ms_best = 1e30f;
+      for (int rep = 0; rep < 3 && !st; rep++) {
+        if (hipEventRecord(e0, s) != hipSuccess) st = HH_DEVICE_ERROR;
+        if (!st)
+          st = launch_search(idx, queries, nq, k, ef, precision, nullptr, nullptr, k, nullptr, 0,
+                             out_keys, out_dist, out_count, status, s);
+        if (!st && hipEventRecord(e1, s) != hipSuccess) st = HH_DEVICE_ERROR;
+        float ms = 0;
+        if (!st && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          st = HH_DEVICE_ERROR;
+        if (rep > 0) ms_best = std::min(ms_best, ms);  // first run warms up
+      }
+      if (st) break;
+      if (ms_best < best_ms) {
+        best_ms = ms_best;
+        best_cap = cap;
+        best_wpb = w;
+      }
+    }
+    if (st) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  idx->tuned_hash_cap[bucket] = st ? 0 : best_cap;
+  idx->tuned_wpb[bucket] = st ? 0 : best_wpb;
+  if (st) return st == HH_DEVICE_ERROR ? fail(st, "autotune launch failed") : st;
+  if (chosen_hash_cap) *chosen_hash_cap = best_cap;
+  if (chosen_wpb) *chosen_wpb = best_wpb;
   return HH_OK;
 }
 
@@ -1001,8 +1179,25 @@ int hh_brute_upload(uint32_t dim, uint32_t metric, const uint64_t *keys, const f
   HH_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   st = upload_rows(prepared.data(), n, dim, b->lay, &b->rows);
   if (!st) st = dalloc(&b->keys, n);
+  if (!st) st = dalloc(&b->xx, n);
   if (st) return st;
-  if (n) HH_HIP(hipMemcpy(b->keys, keys, n * 8, hipMemcpyHostToDevice));
+  if (n) {
+    HH_HIP(hipMemcpy(b->keys, keys, n * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(hhk::row_norms_kernel, dim3(grid_for(n)), dim3(256), 0, nullptr, b->rows, n,
+                       b->lay.stride, b->xx);
+    HH_HIP(hipGetLastError());
+    HH_HIP(hipDeviceSynchronize());
+    double mx = 0.0;
+    for (uint64_t i = 0; i < n; i++) {
+      double ss = 0.0;
+      for (uint32_t e = 0; e < dim; e++) {
+        const double v = prepared[i * dim + e];
+        ss += v * v;
+      }
+      mx = std::max(mx, ss);
+    }
+    b->max_norm = std::sqrt(mx);
+  }
   *out = b.release();
   return HH_OK;
 }
@@ -1021,6 +1216,106 @@ void *brute_partial_ptr(int s) {
   return reinterpret_cast<void *>(&hhk::brute_partial_kernel<TM, EX, QB, 8>);
 }
 
+void *mfma_ptr(uint32_t ks) {
+  switch (ks) {
+    case 8: return reinterpret_cast<void *>(&hhk::brute_mfma_kernel<8>);
+    case 16: return reinterpret_cast<void *>(&hhk::brute_mfma_kernel<16>);
+    case 24: return reinterpret_cast<void *>(&hhk::brute_mfma_kernel<24>);
+    case 32: return reinterpret_cast<void *>(&hhk::brute_mfma_kernel<32>);
+    case 40: return reinterpret_cast<void *>(&hhk::brute_mfma_kernel<40>);
+    case 48: return reinterpret_cast<void *>(&hhk::brute_mfma_kernel<48>);
+    case 56: return reinterpret_cast<void *>(&hhk::brute_mfma_kernel<56>);
+    case 64: return reinterpret_cast<void *>(&hhk::brute_mfma_kernel<64>);
+    default: return nullptr;
+  }
+}
+
+// Whether a batch takes the MFMA shortlist path: f32, unfiltered, D <= 128,
+// k <= 64, and (in automatic mode) enough rows for the GEMM to pay.
+bool mfma_eligible(const hh_brute *b, uint64_t nq, uint32_t kc, const uint32_t *allow, bool f64) {
+  if (b->mfma_mode == 0 || f64 || allow || kc == 0 || kc > 64 || b->n == 0) return false;
+  if (b->lay.stride > 128 || !mfma_ptr(b->lay.stride / 2)) return false;
+  if (b->mfma_mode == 2) return b->n >= 64;
+  return b->n >= 32768 && nq >= 32;
+}
+
+// MFMA shortlist for the queries [c0, c0 + cn) of a batch (queries already
+// prepared into ws.qprep).  Writes results for queries proven exact and
+// sets ws.m_redo for the rest.
+int mfma_chunk(hh_brute *b, Workspace &ws, const hhk::BruteArgs &ba, uint64_t c0, uint64_t cn,
+               uint32_t kc, uint64_t row_step, uint32_t cap, hipStream_t stream) {
+  const uint32_t stride = b->lay.stride;
+  hhk::MfmaArgs m{};
+  m.rows = b->rows;
+  m.n = b->n;
+  m.stride = stride;
+  m.dim = b->dim;
+  m.metric = b->metric;
+  m.qprep = ws.qprep + c0 * stride;
+  m.status = ba.status + c0;
+  m.nq = cn;
+  m.xx = b->xx;
+  m.qq = ws.m_qq;
+  m.dense = ws.m_dense;
+  m.tau = ws.m_tau;
+  m.cand = ws.m_cand;
+  m.cand_count = ws.m_cnt;
+  m.cand_cap = cap;
+  m.max_norm = b->max_norm;
+  m.redo = ws.m_redo + c0;
+  void *mk = mfma_ptr(stride / 2);
+  const unsigned gx = (unsigned)((cn + 63) / 64);
+  hipLaunchKernelGGL(hhk::query_norms_kernel, dim3(grid_for(cn)), dim3(256), 0, stream, m.qprep,
+                     cn, stride, ws.m_qq);
+  HH_HIP(hipGetLastError());
+  // 1. sample pass (dense) and per-query thresholds
+  m.mode = 0;
+  m.row_step = row_step;
+  m.nrows = (b->n + row_step - 1) / row_step;
+  {
+    const uint64_t tiles = (m.nrows + 31) / 32;
+    const uint64_t gy = std::max<uint64_t>(
+        1, std::min<uint64_t>((tiles + 3) / 4, (2048 + gx - 1) / gx));
+    void *args[] = {&m};
+    HH_HIP(hipLaunchKernel(mk, dim3(gx, (unsigned)gy), dim3(256), args, 0, stream));
+  }
+  hipLaunchKernelGGL(hhk::brute_tau_kernel, dim3((unsigned)((cn + 3) / 4)), dim3(256), 0, stream,
+                     m);
+  HH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(cn)), dim3(256), 0, stream, ws.m_cnt, cn,
+                     0u);
+  HH_HIP(hipGetLastError());
+  // 2. candidate filter over every row
+  m.mode = 1;
+  m.row_step = 1;
+  m.nrows = b->n;
+  {
+    const uint64_t tiles = (b->n + 31) / 32;
+    const uint64_t gy = std::max<uint64_t>(
+        1, std::min<uint64_t>((tiles + 3) / 4, (2048 + gx - 1) / gx));
+    void *args[] = {&m};
+    HH_HIP(hipLaunchKernel(mk, dim3(gx, (unsigned)gy), dim3(256), args, 0, stream));
+  }
+  // 3. exact re-rank of the candidates
+  hhk::BruteArgs rb = ba;
+  rb.qprep = ws.qprep + c0 * stride;
+  rb.out_keys = ba.out_keys + c0 * ba.k_stride;
+  rb.out_dist = ba.out_dist + c0 * ba.k_stride;
+  rb.out_count = ba.out_count + c0;
+  const bool ex = b->lay.U > 0 || b->lay.R > 0;
+  const int s = slots_for(kc);
+  void *rk;
+  if (ex)
+    rk = s == 1 ? (void *)&hhk::brute_rerank_kernel<4, true, 1>
+                : (void *)&hhk::brute_rerank_kernel<4, true, 2>;
+  else
+    rk = s == 1 ? (void *)&hhk::brute_rerank_kernel<4, false, 1>
+                : (void *)&hhk::brute_rerank_kernel<4, false, 2>;
+  void *args[] = {&m, &rb};
+  HH_HIP(hipLaunchKernel(rk, dim3((unsigned)((cn + 3) / 4)), dim3(256), args, 0, stream));
+  return HH_OK;
+}
+
 int brute_launch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
                  const uint32_t *allow, uint64_t *out_keys, float *out_dist, double *out_dist64,
                  uint32_t *out_count, int32_t *status, hipStream_t stream, bool f64) {
@@ -1037,6 +1332,7 @@ int brute_launch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
   HH_HIP(hipSetDevice(b->device));
   Workspace &ws = b->ws;
   std::lock_guard<std::mutex> lock(ws.mu);
+  b->last_mfma = 0;
   int st = ws.ensure_queries(nq, b->lay.stride, 0);
   if (st) return st;
   const int qb = f64 ? 1 : (tmax == 4 ? 4 : (tmax == 8 ? 2 : 1));
@@ -1101,6 +1397,24 @@ int brute_launch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
     HH_HIP(hipEventRecord(ws.done, stream));
     return HH_OK;
   }
+  if (mfma_eligible(b, nq, kc, allow, f64)) {
+    // sample stride: about 8 * row_step candidates per query, >= 4k
+    const uint64_t row_step = std::min<uint64_t>(256, std::max<uint64_t>(16, 4 * (uint64_t)kc));
+    const uint32_t cap = (uint32_t)(32 * row_step);
+    const uint64_t nsample = (b->n + row_step - 1) / row_step;
+    // query chunk: dense sample block <= 1 GiB
+    uint64_t chunk = std::max<uint64_t>(64, ((1ull << 28) / std::max<uint64_t>(1, nsample)) / 64 * 64);
+    chunk = std::min<uint64_t>(chunk, nq);
+    st = ws.ensure_mfma(nq, chunk * nsample, chunk * (uint64_t)cap);
+    if (st) return st;
+    for (uint64_t c0 = 0; c0 < nq; c0 += chunk) {
+      st = mfma_chunk(b, ws, a, c0, std::min<uint64_t>(chunk, nq - c0), kc, row_step, cap, stream);
+      if (st) return st;
+    }
+    // exact streaming pass for the flagged queries only
+    a.only = ws.m_redo;
+    b->last_mfma = nq;
+  }
   const uint64_t work = f64 ? nq * nslices : groups * nslices;
   void *pk;
   void *mk;
@@ -1150,6 +1464,30 @@ int hh_brute_search_batch_f64_device(hh_brute *b, const float *queries, uint64_t
 extern "C" {
 
 uint64_t hh_brute_len(const hh_brute *b) { return b ? b->n : 0; }
+
+int hh_brute_set_mfma(hh_brute *b, int mode) {
+  if (!b) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (mode < 0 || mode > 2) return fail(HH_INVALID_ARGUMENT, "mode must be 0, 1 or 2");
+  std::lock_guard<std::mutex> lock(b->ws.mu);
+  b->mfma_mode = mode;
+  return HH_OK;
+}
+
+int hh_brute_mfma_stats(hh_brute *b, uint64_t *mfma_queries, uint64_t *redo_queries) {
+  if (!b) return fail(HH_INVALID_ARGUMENT, "null index");
+  HH_HIP(hipSetDevice(b->device));
+  std::lock_guard<std::mutex> lock(b->ws.mu);
+  HH_HIP(hipEventSynchronize(b->ws.done));
+  uint64_t redo = 0;
+  if (b->last_mfma) {
+    std::vector<uint32_t> f(b->last_mfma);
+    HH_HIP(hipMemcpy(f.data(), b->ws.m_redo, f.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t v : f) redo += v != 0;
+  }
+  if (mfma_queries) *mfma_queries = b->last_mfma;
+  if (redo_queries) *redo_queries = redo;
+  return HH_OK;
+}
 
 static int brute_host(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
                       const uint32_t *allow_bits, uint64_t *out_keys, float *out_dist,

@@ -24,8 +24,32 @@ using namespace hhd;
 namespace hhk {
 
 struct Stats {
-  unsigned long long q, dist, up, exp, rr, ovf, spill;
+  unsigned long long q, dist, up, exp, rr, ovf, spill, cspill;
 };
+
+// A candidate as one 64-bit word ordered like (distance total order, id).
+__device__ __forceinline__ uint64_t ckey(float d, uint32_t id) {
+  return ((uint64_t)((uint32_t)tkey(d) ^ 0x80000000u) << 32) | id;
+}
+__device__ __forceinline__ float ckey_d(uint64_t k) {
+  const int32_t t = (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
+  return __int_as_float(t ^ (int32_t)(((uint32_t)(t >> 31)) >> 1));
+}
+
+// Appends candidate x to the wave's pool (all lanes store the same word).
+// `dead`: the candidate can no longer be expanded (results full and its
+// distance above the worst result, src/hnsw/search.rs:98-100): dropped.
+__device__ __forceinline__ void pool_push(gmut<uint64_t> pool, bool has_pool, uint32_t cap,
+                                          bool dead, uint64_t x, uint32_t &pn, uint64_t &pmin,
+                                          unsigned long long &ovf, unsigned long long &moved) {
+  if (dead) return;
+  const bool room = has_pool && pn < cap;
+  if (room) pool[pn] = x;
+  ovf += room ? 0 : 1;
+  moved += room ? 1 : 0;
+  pmin = room && x < pmin ? x : pmin;
+  pn += room ? 1 : 0;
+}
 
 // Distances from the query to rows ids[0..cnt) (ids in LDS), written to
 // out[0..cnt) (LDS), with traversal precision `prec` (0 exact f32,
@@ -125,12 +149,20 @@ hnsw_search_kernel(const SearchArgs P) {
   const uint32_t hcap = P.hash_cap;
   const Shape shp{P.T, P.U, P.R, P.stride};
   gptr<uint32_t> adj0 = G(P.adj0);
-  Stats st = {0, 0, 0, 0, 0, 0, 0};
+  Stats st = {0, 0, 0, 0, 0, 0, 0, 0};
+  // Candidates beyond the register list go to a per-wave pool in HBM, so
+  // the candidate set is as unbounded as the crate's heap up to the pool's
+  // capacity.  Invariant: every pooled entry orders after every entry of C,
+  // so the front of C is always the global minimum.
+  gmut<uint64_t> pool = GM(P.cpool) + (size_t)slot * P.cpool_cap;
+  const bool has_pool = P.cpool != nullptr;
 
   for (;;) {
-    uint32_t qi = 0;
-    if (lane == 0) qi = atomicAdd(P.counter, 1u);
-    qi = rl(qi, 0);
+    // Side effects in this loop run on every lane with wave-uniform values
+    // (never under `lane == 0` branches): correlated lane-0 branches around
+    // the back edge let the compiler run lanes a loop iteration apart, which
+    // breaks the readlane broadcasts below.  Only lane 0 adds 1 here.
+    const uint32_t qi = rl(atomicAdd(P.counter, lane == 0 ? 1u : 0u), 0);
     if (qi >= P.nq) break;
     uint32_t k = P.k, ef = P.ef, prec = P.precision, filt = 0xFFFFFFFFu;
     if (P.specs) {
@@ -144,10 +176,14 @@ hnsw_search_kernel(const SearchArgs P) {
     const uint64_t obase = (uint64_t)qi * P.k_stride;
     const int status = (int)rl((uint32_t)G(P.status)[qi], 0);
     if (status != 0 || k == 0 || P.live == 0 || P.top < 0) {
-      if (lane == 0) GM(P.out_count)[qi] = 0;
+      GM(P.out_count)[qi] = 0;
       continue;
     }
     if (ef < k) ef = k;
+    // Candidate-list capacity depends on the query's own ef only (twice
+    // its 64-multiple bucket), never on the kernel instance, so a query
+    // returns the same results alone, in a mixed batch or when served.
+    const int ccap = ef <= 64 ? 128 : ef <= 128 ? 256 : ef <= 256 ? 512 : 1024;
     const uint32_t *allow = P.allow;
     if (P.specs) allow = filt == 0xFFFFFFFFu ? nullptr : P.filters + (size_t)filt * P.filter_words;
     st.q++;
@@ -177,7 +213,7 @@ hnsw_search_kernel(const SearchArgs P) {
 
     // ---- entry point and greedy descent (search_graph, search.rs:146-161)
     uint32_t cur = P.entry;
-    if (lane == 0) st_id[0] = cur;
+    st_id[0] = cur;
     wave_sync();
     gather_dists<TMAX, EXTRA>(P, q, qc, (int)prec, st_id, 1, st_d);
     wave_sync();
@@ -232,7 +268,9 @@ hnsw_search_kernel(const SearchArgs P) {
     for (int s = 0; s < SC; s++) { C.d[s] = 0.0f; C.id[s] = kInvalid; }
     int wsize = 0, csize = 0;
     float worst = 0.0f;
-    if (lane == 0) lds_visit(table, hcap, cur);
+    uint32_t pn = 0;            // pooled candidates
+    uint64_t pmin = ~0ull;      // smallest pooled candidate
+    lds_visit(table, hcap, cur);
     wave_sync();
     vcount = 1;
     slist_insert_at<SC>(C, 0, best, cur);
@@ -245,7 +283,44 @@ hnsw_search_kernel(const SearchArgs P) {
     const bool pf_ok = P.m0 <= 64;
     uint32_t pf_id = kInvalid, pf_adj = kInvalid;
 
-    while (csize > 0) {
+    for (;;) {
+      if (csize == 0) {
+        if (pn == 0) break;
+        // refill C with the smallest pooled entries; the rest stay pooled
+        // (compacted in place: each processed entry writes at most one)
+        const uint32_t total = pn;
+        uint32_t w = 0;
+        uint64_t nmin = ~0ull;
+        for (uint32_t base = 0; base < total; base += 64) {
+          const uint32_t c = total - base < 64 ? total - base : 64;
+          const uint64_t mine = lane < (int)c ? pool[base + lane] : ~0ull;
+          for (uint32_t j = 0; j < c; j++) {
+            const uint64_t x = rl64(mine, (int)j);
+            const float xd = ckey_d(x);
+            if (wsize >= (int)ef && xd > worst) continue;
+            uint64_t out = x;
+            if (csize == ccap) {
+              float md;
+              uint32_t mid;
+              slist_get<SC>(C, csize - 1, md, mid);
+              const uint64_t cmax = ckey(md, mid);
+              if (x < cmax) {
+                slist_insert_at<SC>(C, slist_rank<SC>(C, csize, xd, (uint32_t)x), xd, (uint32_t)x);
+                out = cmax;
+              }
+              pool[w] = out;
+              w++;
+              nmin = out < nmin ? out : nmin;
+            } else {
+              slist_insert_at<SC>(C, slist_rank<SC>(C, csize, xd, (uint32_t)x), xd, (uint32_t)x);
+              csize++;
+            }
+          }
+        }
+        pn = w;
+        pmin = nmin;
+        if (csize == 0) break;
+      }
       float cd;
       uint32_t cid;
       slist_get<SC>(C, 0, cd, cid);
@@ -300,14 +375,31 @@ hnsw_search_kernel(const SearchArgs P) {
             slist_get<SW>(W, wsize - 1, wd, wi);
             worst = wd;
           }
-          if (csize == SC * 64) {
-            if (wsize >= (int)ef) csize = slist_count_not_above<SC>(C, csize, worst);
-            if (csize == SC * 64) st.ovf++;
+          if (csize == ccap && wsize >= (int)ef) csize = slist_count_not_above<SC>(C, csize, worst);
+          const uint64_t x = ckey(di, ii);
+          bool to_c;
+          if (csize == 0) {
+            to_c = pn == 0 || x < pmin;
+          } else {
+            float md;
+            uint32_t mid;
+            slist_get<SC>(C, csize - 1, md, mid);
+            const uint64_t cmax = ckey(md, mid);
+            if (x < cmax) {
+              to_c = true;
+              if (csize == ccap)  // C's largest moves out
+                pool_push(pool, has_pool, P.cpool_cap, wsize >= (int)ef && md > worst, cmax, pn,
+                          pmin, st.ovf, st.cspill);
+            } else {
+              to_c = csize < ccap && (pn == 0 || x < pmin);
+            }
           }
-          const int p = slist_rank<SC>(C, csize, di, ii);
-          if (p < SC * 64) {
-            slist_insert_at<SC>(C, p, di, ii);
-            if (csize < SC * 64) csize++;
+          if (to_c) {
+            slist_insert_at<SC>(C, slist_rank<SC>(C, csize, di, ii), di, ii);
+            if (csize < ccap) csize++;
+          } else {
+            pool_push(pool, has_pool, P.cpool_cap, wsize >= (int)ef && di > worst, x, pn, pmin,
+                      st.ovf, st.cspill);
           }
         }
         wave_sync();
@@ -373,7 +465,7 @@ hnsw_search_kernel(const SearchArgs P) {
         GM(P.out_dist)[obase + rank] = d;
       }
     }
-    if (lane == 0) GM(P.out_count)[qi] = (uint32_t)(wsize < (int)k ? wsize : (int)k);
+    GM(P.out_count)[qi] = (uint32_t)(wsize < (int)k ? wsize : (int)k);
     if (mode == 1) {
       for (uint32_t w = lane; w < P.spill_words; w += 64) GM(spill)[w] = 0u;
       __threadfence();
@@ -388,6 +480,7 @@ hnsw_search_kernel(const SearchArgs P) {
     atomicAdd(&P.stats[4], st.rr);
     atomicAdd(&P.stats[5], st.ovf);
     atomicAdd(&P.stats[6], st.spill);
+    atomicAdd(&P.stats[7], st.cspill);
   }
 }
 
@@ -448,6 +541,7 @@ __global__ void prepare_queries_kernel(const PrepArgs A) {
       code[e] = (int8_t)c;
     }
   }
+  for (uint32_t e = A.dim; e < A.stride; e++) out[e] = 0.0f;
   if (code && prec == 1)
     for (uint32_t e = A.dim; e < A.cstride; e++) code[e] = 0;
 }

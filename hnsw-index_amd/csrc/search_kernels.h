@@ -45,6 +45,8 @@ struct SearchArgs {
   uint32_t *counter;
   uint32_t *spill;              // per-wave visited bitsets (spill_words each)
   uint64_t spill_words;
+  uint64_t *cpool;              // per-wave candidate pools (cpool_cap each)
+  uint32_t cpool_cap;
   unsigned long long *stats;    // 7 counters
   uint32_t hash_cap, hash_limit, lds_words;
 };

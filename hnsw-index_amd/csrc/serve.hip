@@ -318,6 +318,62 @@ int hh_serve_stats(hh_server *s, uint64_t *served, uint64_t *batches, uint64_t *
   return s->error;
 }
 
+int hh_serve_replay(hh_server *s, const float *queries, const hh_query_spec *specs, uint64_t n,
+                    const double *arrival_us, double *latency_ms, uint64_t *out_keys,
+                    float *out_dist, uint32_t *out_count, int32_t *out_status) {
+  if (!s || !queries || !specs || !arrival_us || !latency_ms)
+    return fail(HH_INVALID_ARGUMENT, "null argument");
+  std::vector<uint64_t> tickets(n, 0);
+  std::atomic<uint64_t> submitted{0};
+  std::atomic<int> err{HH_OK};
+  std::string err_msg;
+  const auto t0 = Clock::now();
+  // submit on time from this thread; collect from a second one
+  std::thread collector([&] {
+    std::vector<uint64_t> keys(s->k_max);
+    std::vector<float> dist(s->k_max);
+    for (uint64_t i = 0; i < n; i++) {
+      while (submitted.load(std::memory_order_acquire) <= i) {
+        if (err.load()) return;
+        std::this_thread::yield();
+      }
+      uint32_t cnt = 0;
+      int32_t st = 0;
+      double lat = 0;
+      int done = 0;
+      if (hh_serve_poll(s, tickets[i], 1, keys.data(), dist.data(), &cnt, &st, &lat, &done) || !done) {
+        err = HH_DEVICE_ERROR;
+        return;
+      }
+      latency_ms[i] = lat;
+      if (out_count) out_count[i] = cnt;
+      if (out_status) out_status[i] = st;
+      if (out_keys) std::memcpy(out_keys + i * s->k_max, keys.data(), cnt * 8);
+      if (out_dist) std::memcpy(out_dist + i * s->k_max, dist.data(), cnt * 4);
+    }
+  });
+  for (uint64_t i = 0; i < n; i++) {
+    const auto due = t0 + std::chrono::nanoseconds((int64_t)(arrival_us[i] * 1000.0));
+    while (Clock::now() < due) {
+    }
+    int st;
+    for (;;) {
+      st = hh_serve_submit(s, queries + i * s->dim, specs + i, &tickets[i]);
+      if (st != HH_CAPACITY_EXCEEDED) break;
+      std::this_thread::yield();  // queue full: admission back-pressure
+    }
+    if (st) {
+      err = st;
+      err_msg = hh::last_error();
+      break;
+    }
+    submitted.store(i + 1, std::memory_order_release);
+  }
+  collector.join();
+  if (err.load()) return fail(err.load(), err_msg.empty() ? "replay failed" : err_msg);
+  return HH_OK;
+}
+
 int hh_serve_close(hh_server *s) {
   if (!s) return HH_OK;
   {

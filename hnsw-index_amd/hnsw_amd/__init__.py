@@ -482,6 +482,27 @@ class Server:
             return None
         return keys[:cnt.value], dist[:cnt.value], st.value, lat.value
 
+    def replay(self, queries, ks, efs, arrival_us, precisions=None, filter_ids=None):
+        """Native load generator: submits query i at arrival_us[i] and returns
+        (latency_ms, keys, dists, counts, status) per request."""
+        queries = _f32(queries).reshape(-1, self._dim)
+        n = queries.shape[0]
+        specs = np.zeros((n, 4), np.uint32)
+        specs[:, 0] = ks
+        specs[:, 1] = efs
+        specs[:, 2] = 0 if precisions is None else np.asarray(precisions)
+        specs[:, 3] = 0xFFFFFFFF if filter_ids is None else \
+            np.where(np.asarray(filter_ids) < 0, 0xFFFFFFFF, np.asarray(filter_ids))
+        arr = np.ascontiguousarray(arrival_us, np.float64)
+        lat = np.zeros(n, np.float64)
+        keys = np.zeros((n, self._k_max), np.uint64)
+        dist = np.zeros((n, self._k_max), np.float32)
+        cnt = np.zeros(n, np.uint32)
+        st = np.zeros(n, np.int32)
+        _check(lib.hh_serve_replay(self._h, _ptr(queries), _ptr(specs), n, _ptr(arr), _ptr(lat),
+                                   _ptr(keys), _ptr(dist), _ptr(cnt), _ptr(st)))
+        return lat, keys, dist, cnt, st
+
     def stats(self):
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         st = lib.hh_serve_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
@@ -517,6 +538,7 @@ class BruteForceIndex:
         self._keys = np.zeros(0, np.uint64)
         self._vecs = np.zeros((0, dim), np.float32)
         self._h = None
+        self._mfma = 1
 
     def __del__(self):
         try:
@@ -554,6 +576,23 @@ class BruteForceIndex:
             lib.hh_brute_free(self._h)
         self._h = h
         self._keys, self._vecs = new_keys, new_vecs
+        _check(lib.hh_brute_set_mfma(self._h, self._mfma))
+
+    def set_mfma(self, mode):
+        """Execution choice for f32 batches: 0 exact streaming kernel only,
+        1 automatic, 2 MFMA shortlist whenever D <= 128 and k <= 64.  All
+        modes return identical results."""
+        if mode not in (0, 1, 2):
+            raise ValueError("mode must be 0, 1 or 2")
+        self._mfma = mode
+        if self._h:
+            _check(lib.hh_brute_set_mfma(self._h, mode))
+
+    def mfma_stats(self):
+        """(queries of the last batch on the MFMA path, of which re-run exactly)."""
+        a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(lib.hh_brute_mfma_stats(self._handle(), ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
 
     def _handle(self):
         if self._h is None:
@@ -562,6 +601,7 @@ class BruteForceIndex:
             _check(lib.hh_brute_upload(self._dim, int(self._metric), None, None, 0,
                                        self._device, ctypes.byref(h), ctypes.byref(bad)))
             self._h = h
+            _check(lib.hh_brute_set_mfma(self._h, self._mfma))
         return self._h
 
     def search_arrays(self, queries, k, filter=None, f64=False):
@@ -605,4 +645,4 @@ class BruteForceIndex:
         return _results(*self.search_arrays(queries, k))
 
 
-from . import eval, synthetic  # noqa: E402  (public submodules, as in the crate)
+from . import device, eval, synthetic  # noqa: E402  (public submodules, as in the crate)

@@ -38,7 +38,8 @@ class GraphDesc(ctypes.Structure):
 
 class SearchStats(ctypes.Structure):
     _fields_ = [("queries", U64), ("n_dist", U64), ("n_upper", U64), ("n_expand", U64),
-                ("n_rerank", U64), ("n_overflow", U64), ("n_visited_spill", U64)]
+                ("n_rerank", U64), ("n_overflow", U64), ("n_visited_spill", U64),
+                ("n_cand_spill", U64)]
 
 
 class QuerySpec(ctypes.Structure):
@@ -76,6 +77,8 @@ _sig("hh_index_enable_bf16", INT, P)
 _sig("hh_index_stats", INT, P, ctypes.POINTER(SearchStats))
 _sig("hh_index_reset_stats", INT, P)
 _sig("hh_index_set_timing", INT, P, INT)
+_sig("hh_index_autotune", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, ctypes.POINTER(U32),
+     ctypes.POINTER(INT))
 _sig("hh_index_last_kernel_ms", INT, P, ctypes.POINTER(ctypes.c_float))
 _sig("hh_search_batch_device", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, P)
 _sig("hh_search_batch_spec_device", INT, P, P, U64, P, U32, P, U32, P, P, P, P, P)
@@ -91,10 +94,21 @@ _sig("hh_serve_poll", INT, P, U64, INT, P, P, ctypes.POINTER(U32), ctypes.POINTE
      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(INT))
 _sig("hh_serve_stats", INT, P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64))
 _sig("hh_serve_close", INT, P)
+_sig("hh_serve_replay", INT, P, P, P, U64, P, P, P, P, P, P)
 _sig("hh_search_batch", INT, P, P, U64, U32, U32, U32, P, P, P, P, ctypes.POINTER(U64))
 _sig("hh_brute_upload", INT, U32, U32, P, P, U64, INT, ctypes.POINTER(P), ctypes.POINTER(U64))
 _sig("hh_brute_free", None, P)
 _sig("hh_brute_len", U64, P)
+_sig("hh_device_alloc", I32, INT, U64, ctypes.POINTER(P))
+_sig("hh_device_free", I32, P)
+_sig("hh_memcpy", I32, P, P, U64, INT, P)
+_sig("hh_memset", I32, P, INT, U64)
+_sig("hh_stream_create", I32, INT, ctypes.POINTER(P))
+_sig("hh_stream_destroy", I32, P)
+_sig("hh_stream_synchronize", I32, P)
+_sig("hh_device_synchronize", I32, INT)
+_sig("hh_brute_set_mfma", I32, P, I32)
+_sig("hh_brute_mfma_stats", I32, P, ctypes.POINTER(U64), ctypes.POINTER(U64))
 _sig("hh_brute_search_batch_device", INT, P, P, U64, U32, P, P, P, P, P, P)
 _sig("hh_brute_search_batch_f64_device", INT, P, P, U64, U32, P, P, P, P, P)
 _sig("hh_brute_search_batch", INT, P, P, U64, U32, P, P, P, P, ctypes.POINTER(U64))

@@ -62,6 +62,23 @@ const char *hh_last_error(void);
 int hh_device_count(void);
 uint32_t hh_abi_version(void);
 
+/* Device memory and streams of the HIP runtime this library runs on (no
+ * counterpart in the reference, which has no device).  Callers without a
+ * GPU framework of their own use these to hand device buffers and streams
+ * to the *_device entry points; a process that also loads another HIP
+ * runtime (for example one bundled with a Python framework) must not mix
+ * that runtime's streams with these. */
+int hh_device_alloc(int device, uint64_t bytes, void **out);
+int hh_device_free(void *p);
+/* kind: 0 host->device, 1 device->host, 2 device->device; stream NULL =
+ * synchronous copy. */
+int hh_memcpy(void *dst, const void *src, uint64_t bytes, int kind, void *stream);
+int hh_memset(void *dst, int value, uint64_t bytes);
+int hh_stream_create(int device, void **out);
+int hh_stream_destroy(void *stream);
+int hh_stream_synchronize(void *stream);
+int hh_device_synchronize(int device);
+
 /* ------------------------------------------------------------------ */
 /* Host-side graph construction.                                        */
 /* The crate builds its graph on the CPU (HnswIndex::insert /          */
@@ -151,11 +168,22 @@ typedef struct hh_search_stats {
   uint64_t n_upper;         /* distance evaluations in the upper descent */
   uint64_t n_expand;        /* layer-0 expansions */
   uint64_t n_rerank;        /* exact re-scores after approximate traversal */
-  uint64_t n_overflow;      /* candidates dropped from a full candidate list */
+  uint64_t n_overflow;      /* candidates dropped: the per-query candidate pool was full */
   uint64_t n_visited_spill; /* queries whose visited set left LDS */
+  uint64_t n_cand_spill;    /* candidates moved from registers to the HBM pool */
 } hh_search_stats;
 int hh_index_stats(hh_index *idx, hh_search_stats *out);
 int hh_index_reset_stats(hh_index *idx);
+/* Autotuner (no crate counterpart): times the search of one device batch
+ * (arguments as hh_search_batch_device) under each candidate visited-set
+ * capacity (LDS per wavefront, hence wavefronts per CU) and wavefronts per
+ * block, and keeps the fastest for this ef bucket (ef <= 64, 128, 256,
+ * 512) on this index.  Synchronous.  Results of the timed runs are left in
+ * the output buffers. */
+int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
+                      uint32_t precision, uint64_t *out_keys, float *out_dist,
+                      uint32_t *out_count, int32_t *status, void *stream,
+                      uint32_t *chosen_hash_cap, int *chosen_wpb);
 /* Times the search kernel of every following launch with a pair of HIP
  * events on the launch stream (no reference counterpart: measurement). */
 int hh_index_set_timing(hh_index *idx, int on);
@@ -258,6 +286,13 @@ int hh_serve_poll(hh_server *s, uint64_t ticket, int wait, uint64_t *keys, float
 /* Requests served, launches made and requests waiting; returns the first
  * launch error seen, if any. */
 int hh_serve_stats(hh_server *s, uint64_t *served, uint64_t *batches, uint64_t *queued);
+/* Load generator for measurement: submits request i (queries + i*dim,
+ * specs[i]) at arrival_us[i] microseconds after the call starts, from a
+ * native thread, and writes each request's submit-to-result latency.
+ * Result arrays (optional) hold n*k_max keys/distances and n counts. */
+int hh_serve_replay(hh_server *s, const float *queries, const hh_query_spec *specs, uint64_t n,
+                    const double *arrival_us, double *latency_ms, uint64_t *out_keys,
+                    float *out_dist, uint32_t *out_count, int32_t *out_status);
 /* Drains every queued request, then stops the loop and frees the server. */
 int hh_serve_close(hh_server *s);
 
@@ -295,6 +330,16 @@ int hh_brute_search_batch_f64(hh_brute *b, const float *queries, uint64_t nq, ui
                               uint64_t *out_keys, double *out_dist, uint32_t *out_count,
                               uint64_t *first_bad);
 uint64_t hh_brute_len(const hh_brute *b);
+/* Execution choice for f32 batches (no counterpart in the reference, whose
+ * brute force is one loop): 0 = exact streaming kernel only, 1 = automatic
+ * (MFMA shortlist for unfiltered batches with n >= 32768, nq >= 32,
+ * D <= 128, k <= 64), 2 = MFMA shortlist whenever D <= 128 and k <= 64.
+ * Every mode returns the same results: the shortlist path proves each
+ * query exact or re-runs it through the streaming kernel. */
+int hh_brute_set_mfma(hh_brute *b, int mode);
+/* For the last batch: queries that took the shortlist path and how many of
+ * them were re-run exactly.  Waits for that batch. */
+int hh_brute_mfma_stats(hh_brute *b, uint64_t *mfma_queries, uint64_t *redo_queries);
 
 /* ------------------------------------------------------------------ */
 /* Host helpers that mirror public crate items.                         */

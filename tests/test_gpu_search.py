@@ -217,33 +217,71 @@ def test_mixed_batch_reports_per_query_errors():
 
 def test_graph_captured_plan_replays_match_eager():
     import ctypes
-    import torch
+    from hnsw_amd.device import Buffer, Stream
     b, q = O.dataset_gaussian(3000, 200, 64, 23)
     idx = build(b, threads=4)
     h = idx.device_index()
-    dev = torch.device("cuda:0")
-    qd = torch.from_numpy(q).to(dev)
     nq, k = 200, 10
-    outs = [torch.zeros((nq, k), dtype=torch.int64, device=dev),
-            torch.zeros((nq, k), dtype=torch.float32, device=dev),
-            torch.zeros(nq, dtype=torch.int32, device=dev),
-            torch.zeros(nq, dtype=torch.int32, device=dev)]
-    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    qd = Buffer.from_array(q)
+    outs = [Buffer(nq * k * 8), Buffer(nq * k * 4), Buffer(nq * 4), Buffer(nq * 4)]
     plan = ctypes.c_void_p()
-    assert H.lib.hh_search_plan_create(h, P(qd), nq, k, 48, 0, None, None, 0, None, 0,
-                                       *[P(t) for t in outs], ctypes.byref(plan)) == 0
+    assert H.lib.hh_search_plan_create(h, qd.ptr, nq, k, 48, 0, None, None, 0, None, 0,
+                                       *[t.ptr for t in outs], ctypes.byref(plan)) == 0
     ek, ed, ec = idx.search_arrays(q, k, 48)
-    stream = torch.cuda.current_stream()
+    stream = Stream()
     try:
         for rep in range(4):
             for t in outs:
-                t.zero_()
-            assert H.lib.hh_search_plan_launch(plan, ctypes.c_void_p(stream.cuda_stream)) == 0
-            torch.cuda.synchronize()
-            gk = outs[0].cpu().numpy().view(np.uint64)
-            gd = outs[1].cpu().numpy()
-            gc = outs[2].cpu().numpy().view(np.uint32)
+                t.zero()
+            assert H.lib.hh_search_plan_launch(plan, stream.handle) == 0
+            stream.synchronize()
+            gk = outs[0].download((nq, k), np.uint64)
+            gd = outs[1].download((nq, k), np.float32)
+            gc = outs[2].download(nq, np.uint32)
             assert np.array_equal(gc, ec) and np.array_equal(gk, ek) and same_bits(gd, ed), rep
             _ = idx.search_arrays(q[:5], 3, 16)  # eager work between replays
     finally:
         H.lib.hh_search_plan_free(plan)
+        stream.close()
+
+
+@pytest.mark.parametrize("ef", [32, 64, 128, 300])
+def test_selective_filters_match_the_oracle_exactly(ef):
+    """Category filters passing 5-15% of an inner-product index keep the
+    crate's unbounded candidate heap busy (results fill slowly, so almost
+    every visited node becomes a candidate); candidates beyond the register
+    list move to the per-query HBM pool, so results, distances and work
+    counters stay identical to the oracle's, and recall@10 against the exact
+    filtered truth is the oracle's."""
+    d = H.synthetic.Dataset.lognormal(20000, 60, 32, 0.5, 20, 5)
+    b, q, cat = d.base(), d.queries(), d.categories
+    idx = build(b, H.Metric.InnerProduct, m=24, threads=8)
+    g = oracle_graph(idx)
+    rng = np.random.default_rng(ef)
+    keys = np.arange(20000)
+    gpu_found, ora_found, truth = [], [], []
+    idx.search_stats(reset=True)
+    n_expand = 0
+    for i in range(60):
+        cats = rng.choice(20, int(rng.integers(1, 4)), replace=False)
+        allow = np.isin(cat, cats)
+        gk, gd, gc = idx.search_arrays(q[i:i + 1], 10, ef, filter=allow)
+        ok, od, oc, cnt = g.search_batch(q[i:i + 1], 10, ef, allow=allow.astype(np.uint8))
+        n_expand += cnt.n_expand
+        assert gc[0] == oc[0] and np.array_equal(gk[0, :gc[0]], ok[0, :oc[0]]), i
+        assert same_bits(gd[0, :gc[0]], od[0, :oc[0]]), i
+        tk, td = O.brute_search(O.IP, keys, b, q[i], 10, allow=allow.astype(np.uint8))
+        gpu_found.append((gk[0], gc[0]))
+        ora_found.append((ok[0], oc[0]))
+        truth.append(tk)
+    st = idx.search_stats()
+    assert st["n_overflow"] == 0 and st["n_expand"] == n_expand, st
+    if ef <= 64:
+        assert st["n_cand_spill"] > 0, st  # the pool was exercised
+    tk = np.stack(truth)
+    tc = np.full(60, 10, np.uint32)
+    rg = O.recall_at_k(tk, tc, np.stack([x[0] for x in gpu_found]),
+                       np.array([x[1] for x in gpu_found], np.uint32), 10)
+    ro = O.recall_at_k(tk, tc, np.stack([x[0] for x in ora_found]),
+                       np.array([x[1] for x in ora_found], np.uint32), 10)
+    assert abs(rg - ro) <= 0.005, (rg, ro)
