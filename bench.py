@@ -51,6 +51,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=10000)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--f64-check", type=int, default=500)
+    ap.add_argument("--quick", action="store_true",
+                    help="search measurement only: no precision variants, serving or CPU "
+                         "baseline (for profiler runs)")
+    ap.add_argument("--launch-shape", default="",
+                    help="CAP,WPB: visited-set capacity and waves per block instead of the "
+                         "autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each query is "
@@ -151,14 +157,20 @@ def main():
     chosen = next((s for s in sweep if s["recall"] >= args.target), sweep[-1])
     ef = chosen["ef"]
 
-    # autotune the launch shape for this ef (untimed)
+    # launch shape for this ef: given, or autotuned (untimed)
     cap, wpb = ctypes.c_uint32(), ctypes.c_int()
-    st = H.lib.hh_index_autotune(h, q_dev.ptr, nq, k, ef, 0,
-                                 out_k.ptr, out_d.ptr, out_c.ptr, status.ptr,
-                                 sptr, ctypes.byref(cap), ctypes.byref(wpb))
-    if st:
-        raise RuntimeError(L.last_error())
-    log(f"autotune ef={ef}: visited-set capacity {cap.value}, {wpb.value} waves per block")
+    if args.launch_shape:
+        cap.value, wpb.value = (int(x) for x in args.launch_shape.split(","))
+        if H.lib.hh_index_set_launch_shape(h, ef, cap.value, wpb.value):
+            raise RuntimeError(L.last_error())
+        log(f"launch shape ef={ef}: visited-set capacity {cap.value}, {wpb.value} waves per block")
+    else:
+        st = H.lib.hh_index_autotune(h, q_dev.ptr, nq, k, ef, 0,
+                                     out_k.ptr, out_d.ptr, out_c.ptr, status.ptr,
+                                     sptr, ctypes.byref(cap), ctypes.byref(wpb))
+        if st:
+            raise RuntimeError(L.last_error())
+        log(f"autotune ef={ef}: visited-set capacity {cap.value}, {wpb.value} waves per block")
 
     # timed region
     for _ in range(args.warmup):
@@ -196,8 +208,12 @@ def main():
     log(f"search kernel {kern_ms:.3f} ms/launch; {n_dist / nq:.0f} distance evaluations/query; "
         f"{achieved:.0f} GB/s of gathered rows")
 
+    variants, serve, cpu = {}, None, None
+    if args.quick:
+        emit(args, locals())
+        return
+
     # precision variants at the same ef
-    variants = {}
     idx.enable_bf16()
     for name, prec in (("bf16", H.Precision.BF16),):
         run(ef, prec)
@@ -215,7 +231,6 @@ def main():
     log(f"int8 ef={ef}: recall={r:.4f} qps={nq / sec:,.0f}")
 
     # serving loop: Poisson arrivals replayed by the native load generator
-    serve = None
     try:
         rate = min(0.5 * qps, 400_000.0)
         nreq = 50_000
@@ -240,7 +255,6 @@ def main():
         log(f"serving measurement failed: {e}")
 
     # CPU baseline: the oracle's restatement of search_batch on host cores
-    cpu = None
     try:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -278,6 +292,35 @@ def main():
     except Exception as e:  # the baseline is reported, never required
         log(f"cpu baseline unavailable: {e}")
 
+    emit(args, locals())
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "search_c2_1m128_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per search launch (FETCH_SIZE + WRITE_SIZE counter passes,
+    calibrated on a copy kernel) from the committed counter summary of this
+    workload, or None when it is absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def emit(args, v):
+    """Prints the one JSON line of the run from main()'s values."""
+    qps, total, nq, k, ef, rec = v["qps"], v["total"], v["nq"], v["k"], v["ef"], v["rec"]
+    n, dim, achieved, cpu, step_ms = v["n"], v["dim"], v["achieved"], v["cpu"], v["step_ms"]
+    kern_ms, n_dist, alg_bytes, adj_bytes = v["kern_ms"], v["n_dist"], v["alg_bytes"], v["adj_bytes"]
+    sweep, variants, serve = v["sweep"], v["variants"], v["serve"]
+    bf_s, f64_agree, mfma_used, mfma_redo = v["bf_s"], v["f64_agree"], v["mfma_used"], v["mfma_redo"]
+    stream_s, gemm_tflops, build_s, cap, wpb = (v["stream_s"], v["gemm_tflops"], v["build_s"],
+                                                v["cap"], v["wpb"])
     line = {
         "metric": "QPS at recall@10>=0.95, 1Mx128 L2 (GPU FP32 search_batch)",
         "value": round(qps, 1),
@@ -297,7 +340,8 @@ def main():
                    "queries": nq, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic()},
+        "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if measured_traffic() else None,
         "cpu_baseline": cpu,
         "step_ms_p50": round(float(np.percentile(step_ms, 50)), 3),
         "step_ms_p99": round(float(np.percentile(step_ms, 99)), 3),

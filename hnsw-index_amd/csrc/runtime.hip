@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cmath>
 #include <cstring>
 #include <condition_variable>
@@ -267,6 +269,28 @@ int num_cus(int device) {
   return cus;
 }
 
+// Resident blocks per CU of a kernel at a block size and LDS size, cached
+// per device: launch paths ask for every batch, and serving launches often.
+int occupancy_blocks(int device, const void *kern, int threads, size_t lds, int *out) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void *, int, size_t>, int> cache;
+  const auto key = std::make_tuple(device, kern, threads, lds);
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      *out = it->second;
+      return HH_OK;
+    }
+  }
+  int b = 0;
+  HH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, threads, lds));
+  std::lock_guard<std::mutex> lock(mu);
+  cache[key] = b;
+  *out = b;
+  return HH_OK;
+}
+
 }  // namespace
 
 struct hh_index {
@@ -418,12 +442,20 @@ struct SearchCall {
   int32_t *status;
 };
 
+// Largest max(ef, k) a launch must serve: the call's own for a uniform
+// batch; for a mixed batch the caller's bound (c.ef, 0 = 512).  Mixed-batch
+// queries above the bound fail with HH_UNSUPPORTED in the prepare kernel.
+uint32_t spec_ef_bound(const SearchCall &c) {
+  if (!c.specs) return std::max(c.ef, c.k);
+  return c.ef == 0 ? 512u : std::min<uint32_t>(c.ef, 512u);
+}
+
 int check_call(hh_index *idx, const SearchCall &c) {
   if (!c.queries || !c.out_count || (c.k_stride > 0 && (!c.out_keys || !c.out_dist)))
     return fail(HH_INVALID_ARGUMENT, "null device pointer");
   if (!tmax_for(idx->lay)) return fail(HH_UNSUPPORTED, "search kernels support dim <= 1024");
   if (!c.specs && c.precision > 2) return fail(HH_INVALID_ARGUMENT, "unknown precision");
-  const uint32_t efmax = c.specs ? 512u : std::max(c.ef, c.k);
+  const uint32_t efmax = spec_ef_bound(c);
   if (!slots_for(std::max<uint32_t>(efmax, 1)))
     return fail(HH_UNSUPPORTED, "ef (raised to k) above 512 is not supported");
   if (!c.specs && c.precision == 2 && !idx->bf16)
@@ -432,7 +464,7 @@ int check_call(hh_index *idx, const SearchCall &c) {
 }
 
 int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
-  const uint32_t efmax = std::max<uint32_t>(c.specs ? 512u : std::max(c.ef, c.k), 1);
+  const uint32_t efmax = std::max<uint32_t>(spec_ef_bound(c), 1);
   const int sw = slots_for(efmax);
   // visited-set capacity: 32 entries per unit of ef (a query at ef
   // evaluates roughly 10-15 ef distances on the benchmark data), at least
@@ -452,7 +484,8 @@ int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
     if (idx->tuned_wpb[bucket] && w != idx->tuned_wpb[bucket]) continue;
     if (cfg.lds_words * 4 * w > 65536) continue;
     int b = 0;
-    HH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, cfg.kern, w * 64, cfg.lds_words * 4 * w));
+    const int st = occupancy_blocks(idx->device, cfg.kern, w * 64, cfg.lds_words * 4 * w, &b);
+    if (st) return st;
     if (b * w > best_waves) {
       best_waves = b * w;
       cfg.wpb = w;
@@ -469,7 +502,7 @@ int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
 // Enqueues the prepare and search kernels of one call on `stream`, using
 // workspace `ws` (whose buffers the caller has sized).  No synchronisation.
 int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const SearchConfig &cfg,
-                   hipStream_t stream) {
+                   hipStream_t stream, hipEvent_t before_search = nullptr) {
   hhk::PrepArgs pa{};
   pa.queries = c.queries;
   pa.nq = c.nq;
@@ -486,7 +519,7 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   pa.specs = c.specs;
   pa.k_stride = c.k_stride;
   pa.nfilters = c.nfilters;
-  pa.ef_cap = 512;
+  pa.ef_cap = c.specs ? spec_ef_bound(c) : 512;
   pa.has_codes = idx->codes != nullptr;
   pa.has_bf16 = idx->bf16 != nullptr;
   pa.status = c.status ? c.status : ws.status;
@@ -542,6 +575,7 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.hash_limit = cfg.hash_cap / 2 + cfg.hash_cap / 4;
   a.lds_words = cfg.lds_words;
   void *args[] = {&a};
+  if (before_search) HH_HIP(hipEventRecord(before_search, stream));
   HH_HIP(hipLaunchKernel(cfg.kern, dim3((unsigned)cfg.blocks), dim3(cfg.wpb * 64), args,
                          cfg.lds_words * 4 * cfg.wpb, stream));
   return HH_OK;
@@ -608,8 +642,7 @@ int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, 
   if (!st) st = ws->ensure_spill(cfg.blocks * cfg.wpb, cfg.spill_words);
   if (st) return st;
   HH_HIP(hipStreamWaitEvent(stream, ws->done, 0));
-  if (idx->timing) HH_HIP(hipEventRecord(idx->t0, stream));
-  st = enqueue_search(idx, *ws, c, cfg, stream);
+  st = enqueue_search(idx, *ws, c, cfg, stream, idx->timing ? idx->t0 : nullptr);
   if (st) return st;
   if (idx->timing) HH_HIP(hipEventRecord(idx->t1, stream));
   HH_HIP(hipEventRecord(ws->done, stream));
@@ -928,6 +961,25 @@ int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t
   return HH_OK;
 }
 
+int hh_index_set_launch_shape(hh_index *idx, uint32_t ef, uint32_t hash_cap, int wpb) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  const int sw = slots_for(std::max<uint32_t>(ef, 1));
+  if (!sw) return fail(HH_UNSUPPORTED, "ef above 512 is not supported");
+  const int bucket = sw == 1 ? 0 : sw == 2 ? 1 : sw == 4 ? 2 : 3;
+  if (hash_cap == 0) {
+    idx->tuned_hash_cap[bucket] = 0;
+    idx->tuned_wpb[bucket] = 0;
+    return HH_OK;
+  }
+  if (wpb != 1 && wpb != 2 && wpb != 4)
+    return fail(HH_INVALID_ARGUMENT, "waves per block must be 1, 2 or 4");
+  if (hash_cap < 256u * (uint32_t)sw || (uint64_t)(hash_cap + 128) * 4 * (uint64_t)wpb > 65536)
+    return fail(HH_INVALID_ARGUMENT, "visited-set capacity out of range for this ef and block");
+  idx->tuned_hash_cap[bucket] = hash_cap;
+  idx->tuned_wpb[bucket] = wpb;
+  return HH_OK;
+}
+
 int hh_index_reset_stats(hh_index *idx) {
   if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
   HH_HIP(hipSetDevice(idx->device));
@@ -947,14 +999,14 @@ int hh_search_batch_device(hh_index *idx, const float *queries, uint64_t nq, uin
 }
 
 int hh_search_batch_spec_device(hh_index *idx, const float *queries, uint64_t nq,
-                                const hh_query_spec *specs, uint32_t k_stride,
+                                const hh_query_spec *specs, uint32_t k_stride, uint32_t ef_max,
                                 const uint32_t *filters, uint32_t nfilters, uint64_t *out_keys,
                                 float *out_dist, uint32_t *out_count, int32_t *status,
                                 void *stream) {
   if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
   if (!specs) return fail(HH_INVALID_ARGUMENT, "null specs");
-  return launch_search(idx, queries, nq, 0, 0, 0, nullptr, specs, k_stride, filters, nfilters,
-                       out_keys, out_dist, out_count, status, (hipStream_t)stream);
+  return launch_search(idx, queries, nq, 0, ef_max, 0, nullptr, specs, k_stride, filters,
+                       nfilters, out_keys, out_dist, out_count, status, (hipStream_t)stream);
 }
 
 int hh_search_batch(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
@@ -1058,7 +1110,11 @@ int hh_search_batch_spec(hh_index *idx, const float *queries, uint64_t nq,
     cleanup();
     return fail(HH_DEVICE_ERROR, hipGetErrorString(e));
   }
-  st = launch_search(idx, dq, nq, 0, 0, 0, nullptr, dsp, k_stride, df, df ? nfilters : 0, dk, dd,
+  uint32_t ef_max = 1;
+  for (uint64_t i = 0; i < nq; i++)
+    ef_max = std::max(ef_max, std::max(specs[i].ef, specs[i].k));
+  ef_max = std::min<uint32_t>(ef_max, 512);  // larger ones fail per query
+  st = launch_search(idx, dq, nq, 0, ef_max, 0, nullptr, dsp, k_stride, df, df ? nfilters : 0, dk, dd,
                      dc, ds, s);
   if (st) {
     (void)hipStreamSynchronize(s);
@@ -1104,7 +1160,7 @@ int hh_search_plan_create(hh_index *idx, const float *queries, uint64_t nq, uint
   if (!idx || !out) return fail(HH_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
   if (nq == 0) return fail(HH_INVALID_ARGUMENT, "empty batch");
-  const SearchCall c{queries, nq, specs ? 0u : k, specs ? 0u : ef, specs ? 0u : precision,
+  const SearchCall c{queries, nq, specs ? 0u : k, ef, specs ? 0u : precision,
                      specs ? nullptr : allow_bits, specs, specs ? k_stride : k,
                      specs ? filters : nullptr, specs ? nfilters : 0u, out_keys, out_dist,
                      out_count, status};

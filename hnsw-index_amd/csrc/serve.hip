@@ -166,14 +166,19 @@ void hh_server::run() {
 
 int hh_server::launch(Slot &s) {
   const size_t nb = s.batch.size();
+  uint32_t ef_max = 1;
   for (size_t i = 0; i < nb; i++) {
     std::memcpy(s.h_q + i * dim, s.batch[i].query.data(), dim * sizeof(float));
     s.h_s[i] = s.batch[i].spec;
+    ef_max = std::max(ef_max, std::max(s.h_s[i].ef, s.h_s[i].k));
   }
+  // the launch is sized for this batch's largest ef, not the largest
+  // supported one (larger specs fail per request)
+  ef_max = std::min<uint32_t>(ef_max, 512);
   if (hipMemcpyAsync(s.d_q, s.h_q, nb * dim * 4, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
       hipMemcpyAsync(s.d_s, s.h_s, nb * sizeof(hh_query_spec), hipMemcpyHostToDevice, s.stream) != hipSuccess)
     return fail(HH_DEVICE_ERROR, "staging copy failed");
-  int st = hh_search_batch_spec_device(idx, s.d_q, nb, s.d_s, k_max, d_filters, nfilters, s.d_k,
+  int st = hh_search_batch_spec_device(idx, s.d_q, nb, s.d_s, k_max, ef_max, d_filters, nfilters, s.d_k,
                                        s.d_d, s.d_c, s.d_st, s.stream);
   if (st) return st;
   if (hipMemcpyAsync(s.h_k, s.d_k, nb * k_max * 8, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||

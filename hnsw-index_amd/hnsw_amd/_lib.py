@@ -81,7 +81,7 @@ _sig("hh_index_autotune", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, ctypes.P
      ctypes.POINTER(INT))
 _sig("hh_index_last_kernel_ms", INT, P, ctypes.POINTER(ctypes.c_float))
 _sig("hh_search_batch_device", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, P)
-_sig("hh_search_batch_spec_device", INT, P, P, U64, P, U32, P, U32, P, P, P, P, P)
+_sig("hh_search_batch_spec_device", INT, P, P, U64, P, U32, U32, P, U32, P, P, P, P, P)
 _sig("hh_search_batch_spec", INT, P, P, U64, P, U32, P, U32, P, P, P, P)
 _sig("hh_search_plan_create", INT, P, P, U64, U32, U32, U32, P, P, U32, P, U32, P, P, P, P,
      ctypes.POINTER(P))
@@ -99,6 +99,7 @@ _sig("hh_search_batch", INT, P, P, U64, U32, U32, U32, P, P, P, P, ctypes.POINTE
 _sig("hh_brute_upload", INT, U32, U32, P, P, U64, INT, ctypes.POINTER(P), ctypes.POINTER(U64))
 _sig("hh_brute_free", None, P)
 _sig("hh_brute_len", U64, P)
+_sig("hh_index_set_launch_shape", I32, P, U32, U32, INT)
 _sig("hh_device_alloc", I32, INT, U64, ctypes.POINTER(P))
 _sig("hh_device_free", I32, P)
 _sig("hh_memcpy", I32, P, P, U64, INT, P)

@@ -184,6 +184,11 @@ int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t
                       uint32_t precision, uint64_t *out_keys, float *out_dist,
                       uint32_t *out_count, int32_t *status, void *stream,
                       uint32_t *chosen_hash_cap, int *chosen_wpb);
+/* Sets the launch shape of an ef bucket directly (for example one found by
+ * hh_index_autotune in an earlier process): visited-set capacity in
+ * entries (>= 256 per 64 of ef, LDS per block <= 64 KiB) and waves per
+ * block (1, 2 or 4).  hash_cap 0 restores the default choice. */
+int hh_index_set_launch_shape(hh_index *idx, uint32_t ef, uint32_t hash_cap, int wpb);
 /* Times the search kernel of every following launch with a pair of HIP
  * events on the launch stream (no reference counterpart: measurement). */
 int hh_index_set_timing(hh_index *idx, int on);
@@ -216,10 +221,12 @@ typedef struct hh_query_spec {
 } hh_query_spec;
 
 /* Mixed batch: query i uses specs[i] (device array); its results start at
- * out_keys + i*k_stride (k_stride >= every spec's k).  filters: device
- * array of nfilters bitmaps of ceil(n/32) words each, or NULL. */
+ * out_keys + i*k_stride (k_stride >= every spec's k).  ef_max bounds
+ * max(ef, k) over the batch and sizes the launch (0 = 512, the largest
+ * supported); a query above it gets HH_UNSUPPORTED.  filters: device array
+ * of nfilters bitmaps of ceil(n/32) words each, or NULL. */
 int hh_search_batch_spec_device(hh_index *idx, const float *queries, uint64_t nq,
-                                const hh_query_spec *specs, uint32_t k_stride,
+                                const hh_query_spec *specs, uint32_t k_stride, uint32_t ef_max,
                                 const uint32_t *filters, uint32_t nfilters, uint64_t *out_keys,
                                 float *out_dist, uint32_t *out_count, int32_t *status,
                                 void *stream);
@@ -236,8 +243,8 @@ int hh_search_batch_spec(hh_index *idx, const float *queries, uint64_t nq,
  * is captured once in a HIP graph and replayed: for callers that issue the
  * same batch shape repeatedly on fixed device buffers.  Arguments as in
  * hh_search_batch_device (specs != NULL selects the mixed form, as
- * hh_search_batch_spec_device; then k, ef, precision and allow_bits are
- * ignored).  The plan owns its workspace; its launches are ordered with an
+ * hh_search_batch_spec_device; then k, precision and allow_bits are ignored
+ * and ef is the batch's ef_max).  The plan owns its workspace; its launches are ordered with an
  * event, so launches of one plan never overlap.  A plan captures the index
  * as it is at creation: re-create it after hh_index_update_deleted or
  * hh_index_enable_bf16. */
