@@ -163,14 +163,15 @@ def main():
         cap.value, wpb.value = (int(x) for x in args.launch_shape.split(","))
         if H.lib.hh_index_set_launch_shape(h, ef, cap.value, wpb.value):
             raise RuntimeError(L.last_error())
-        log(f"launch shape ef={ef}: visited-set capacity {cap.value}, {wpb.value} waves per block")
+        how = "launch shape"
     else:
         st = H.lib.hh_index_autotune(h, q_dev.ptr, nq, k, ef, 0,
                                      out_k.ptr, out_d.ptr, out_c.ptr, status.ptr,
                                      sptr, ctypes.byref(cap), ctypes.byref(wpb))
         if st:
             raise RuntimeError(L.last_error())
-        log(f"autotune ef={ef}: visited-set capacity {cap.value}, {wpb.value} waves per block")
+        how = "autotune"
+    log(f"{how} ef={ef}: visited-set capacity {cap.value}, {wpb.value} waves per block")
 
     # timed region
     for _ in range(args.warmup):

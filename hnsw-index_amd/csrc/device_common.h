@@ -122,10 +122,39 @@ __device__ __forceinline__ float to_f<uint16_t>(uint16_t v) {
 }
 
 // Loads lane i's share of the row at p (f32 or bf16 elements).
-template <int TMAX, bool EXTRA, class E>
+// EXACT: the dimension is exactly 32 * TMAX (D = 128, 256, 1024), so every
+// 4-block is full and there is no remainder: straight-line loads with no
+// shape branches, which lets a caller issue several rows' loads before
+// waiting for any of them.
+template <int TMAX, bool EXTRA, class E, bool EXACT = false>
 __device__ __forceinline__ void load_q(const E *p_, const Shape &s, int i,
                                        QLanes<TMAX, EXTRA> &x) {
   gptr<E> p = G(p_);
+  if constexpr (EXACT) {
+    static_assert(!EXTRA && TMAX % 4 == 0, "exact rows have no remainder");
+#pragma unroll
+    for (int b = 0; b < TMAX / 4; b++) {
+      gptr<E> blk = p + b * 128;
+      if constexpr (sizeof(E) == 4) {
+        const float4 a = *(gptr<float4>)(blk + 4 * i);
+        const float4 c = *(gptr<float4>)(blk + 64 + 4 * i);
+        x.lo[4 * b] = a.x; x.lo[4 * b + 1] = a.y; x.lo[4 * b + 2] = a.z; x.lo[4 * b + 3] = a.w;
+        x.hi[4 * b] = c.x; x.hi[4 * b + 1] = c.y; x.hi[4 * b + 2] = c.z; x.hi[4 * b + 3] = c.w;
+      } else {
+        const uint2 a = *(gptr<uint2>)(blk + 4 * i);
+        const uint2 c = *(gptr<uint2>)(blk + 64 + 4 * i);
+        x.lo[4 * b] = __uint_as_float(a.x << 16);
+        x.lo[4 * b + 1] = __uint_as_float(a.x & 0xFFFF0000u);
+        x.lo[4 * b + 2] = __uint_as_float(a.y << 16);
+        x.lo[4 * b + 3] = __uint_as_float(a.y & 0xFFFF0000u);
+        x.hi[4 * b] = __uint_as_float(c.x << 16);
+        x.hi[4 * b + 1] = __uint_as_float(c.x & 0xFFFF0000u);
+        x.hi[4 * b + 2] = __uint_as_float(c.y << 16);
+        x.hi[4 * b + 3] = __uint_as_float(c.y & 0xFFFF0000u);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int b = 0; b < (TMAX + 3) / 4; b++) {
     const int t0 = 4 * b;
@@ -182,7 +211,7 @@ __device__ __forceinline__ void load_q(const E *p_, const Shape &s, int i,
 // (src/kernels/x86.rs:47-69 dot, :96-120 l2): one fused multiply-add per
 // element in t order, the 8-wide remainder into accumulator lanes 0..7,
 // and the unfused tail product (x86.rs:75-77, 126-129).
-template <int TMAX, bool EXTRA, bool L2>
+template <int TMAX, bool EXTRA, bool L2, bool EXACT = false>
 __device__ __forceinline__ void quarter_acc(const QLanes<TMAX, EXTRA> &q,
                                             const QLanes<TMAX, EXTRA> &x, const Shape &s, int i,
                                             float &lo, float &hi, float &tail) {
@@ -190,7 +219,7 @@ __device__ __forceinline__ void quarter_acc(const QLanes<TMAX, EXTRA> &q,
   hi = 0.0f;
 #pragma unroll
   for (int t = 0; t < TMAX; t++) {
-    if (t < s.T) {
+    if (EXACT || t < s.T) {
       if (L2) {
         const float dl = q.lo[t] - x.lo[t], dh = q.hi[t] - x.hi[t];
         lo = __builtin_fmaf(dl, dl, lo);

@@ -82,6 +82,9 @@ __global__ void to_bf16_kernel(const float *src, uint16_t *dst, uint64_t count) 
 // resident wavefront).  Only a query that visits more than this many
 // not-yet-expanded nodes drops candidates (counted in n_overflow).
 constexpr uint32_t kCandPool = 8192;
+// Nodes a query may mark in its HBM visited bitset (after leaving the LDS
+// table) and still clear word by word at its end; beyond, the whole bitset.
+constexpr uint32_t kSpillLog = 4096;
 
 int grid_for(uint64_t work) {
   uint64_t g = (work + 255) / 256;
@@ -142,6 +145,7 @@ struct Workspace {
   unsigned long long *stats = nullptr;
   uint32_t *spill = nullptr;
   uint64_t *cpool = nullptr;  // per-wave candidate pools (kCandPool entries)
+  uint32_t *slog = nullptr;   // per-wave spill logs (kSpillLog entries)
   uint64_t spill_slots = 0;
   // brute-force partial lists
   float *part_d = nullptr;
@@ -194,6 +198,7 @@ struct Workspace {
     dfree(stats);
     dfree(spill);
     dfree(cpool);
+    dfree(slog);
     dfree(part_d);
     dfree(part_d64);
     dfree(part_k);
@@ -229,8 +234,10 @@ struct Workspace {
     HH_HIP(hipEventSynchronize(done));
     dfree(spill);
     dfree(cpool);
+    dfree(slog);
     int st = dalloc(&spill, slots * words);
     if (!st) st = dalloc(&cpool, slots * kCandPool);
+    if (!st) st = dalloc(&slog, slots * kSpillLog);
     if (st) {
       spill_slots = 0;
       return st;
@@ -379,25 +386,32 @@ int pack_deleted(const uint8_t *deleted, uint64_t n, std::vector<uint32_t> &bits
   return HH_OK;
 }
 
-template <int TMAX, bool EX, int SW>
+template <int TMAX, bool EX, bool XA, int SW>
 void *search_kernel_ptr() {
-  return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, SW>);
+  return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW>);
 }
 
-void *pick_search_kernel(int tmax, bool extra, int sw) {
-#define HH_PICK(TM, EX)                                     \
-  if (tmax == TM && extra == EX) {                          \
-    if (sw == 1) return search_kernel_ptr<TM, EX, 1>();     \
-    if (sw == 2) return search_kernel_ptr<TM, EX, 2>();     \
-    if (sw == 4) return search_kernel_ptr<TM, EX, 4>();     \
-    if (sw == 8) return search_kernel_ptr<TM, EX, 8>();     \
+// The search kernel instance for a row layout and list size.
+void *pick_search_kernel(const hh::RowLayout &lay, int sw) {
+  const int tmax = tmax_for(lay);
+  const bool extra = lay.U > 0 || lay.R > 0;
+  const bool exact = !extra && (int)lay.T == tmax;
+#define HH_PICK(TM, EX, XA)                                      \
+  if (tmax == TM && extra == EX && exact == XA) {                \
+    if (sw == 1) return search_kernel_ptr<TM, EX, XA, 1>();      \
+    if (sw == 2) return search_kernel_ptr<TM, EX, XA, 2>();      \
+    if (sw == 4) return search_kernel_ptr<TM, EX, XA, 4>();      \
+    if (sw == 8) return search_kernel_ptr<TM, EX, XA, 8>();      \
   }
-  HH_PICK(4, false)
-  HH_PICK(4, true)
-  HH_PICK(8, false)
-  HH_PICK(8, true)
-  HH_PICK(32, false)
-  HH_PICK(32, true)
+  HH_PICK(4, false, false)
+  HH_PICK(4, true, false)
+  HH_PICK(4, false, true)
+  HH_PICK(8, false, false)
+  HH_PICK(8, true, false)
+  HH_PICK(8, false, true)
+  HH_PICK(32, false, false)
+  HH_PICK(32, true, false)
+  HH_PICK(32, false, true)
 #undef HH_PICK
   return nullptr;
 }
@@ -466,17 +480,15 @@ int check_call(hh_index *idx, const SearchCall &c) {
 int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
   const uint32_t efmax = std::max<uint32_t>(spec_ef_bound(c), 1);
   const int sw = slots_for(efmax);
-  // visited-set capacity: 32 entries per unit of ef (a query at ef
-  // evaluates roughly 10-15 ef distances on the benchmark data), at least
-  // 2048 so the result arrays fit in it afterwards; past 3/4 full the query
-  // continues on its HBM bitset.
+  // visited-set capacity: 2048 entries (8 KiB of LDS per wavefront, which
+  // keeps the register-limited 16 wavefronts per CU resident, and holds the
+  // result arrays afterwards); past 3/4 full a query continues on its HBM
+  // bitset.  hh_index_autotune trades this against occupancy per ef.
   const int bucket = sw == 1 ? 0 : sw == 2 ? 1 : sw == 4 ? 2 : 3;
-  cfg.hash_cap = idx->tuned_hash_cap[bucket]
-                     ? idx->tuned_hash_cap[bucket]
-                     : std::min<uint32_t>(16384, std::max<uint32_t>(2048, efmax * 32));
+  cfg.hash_cap = idx->tuned_hash_cap[bucket] ? idx->tuned_hash_cap[bucket] : 2048u;
   cfg.hash_cap = std::max<uint32_t>(cfg.hash_cap, 256u * (uint32_t)sw);
   cfg.lds_words = cfg.hash_cap + 128;
-  cfg.kern = pick_search_kernel(tmax_for(idx->lay), idx->lay.U > 0 || idx->lay.R > 0, sw);
+  cfg.kern = pick_search_kernel(idx->lay, sw);
   // waves per block: the tuned value, else whichever of 1, 2, 4 keeps the
   // most waves resident
   int blocks_per_cu = 1, best_waves = 0;
@@ -570,6 +582,8 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.spill_words = cfg.spill_words;
   a.cpool = ws.cpool;
   a.cpool_cap = kCandPool;
+  a.spill_log = ws.slog;
+  a.spill_log_cap = kSpillLog;
   a.stats = idx->stats_ws.stats;
   a.hash_cap = cfg.hash_cap;
   a.hash_limit = cfg.hash_cap / 2 + cfg.hash_cap / 4;
@@ -922,7 +936,7 @@ int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t
   float best_ms = 1e30f;
   uint32_t best_cap = 0;
   int best_wpb = 0;
-  const uint32_t caps[] = {1024, 1536, 2048, 3072, 4096, 6144, 8192, 12288, 16384};
+  const uint32_t caps[] = {1024, 1536, 1792, 2048, 3072, 4096, 6144, 8192, 12288, 16384};
   int st = HH_OK;
   for (uint32_t cap : caps) {
     if (cap < 256u * (uint32_t)sw || cap * 4 < efmax * 16) continue;

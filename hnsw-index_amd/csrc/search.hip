@@ -51,81 +51,123 @@ __device__ __forceinline__ void pool_push(gmut<uint64_t> pool, bool has_pool, ui
   pn += room ? 1 : 0;
 }
 
+// f32 or bf16 rows: B rows per quarter per round, all of a round's loads
+// issued before any is used (rows past cnt repeat the last id, so every
+// lane loads unconditionally and no branch splits the loads).
+template <int TMAX>
+constexpr int rows_per_quarter() { return TMAX <= 4 ? 4 : (TMAX <= 8 ? 2 : 1); }
+
+template <int TMAX, bool EXTRA, bool EXACT, class E>
+__device__ __forceinline__ void gather_round(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
+                                             const E *rows, const uint32_t *ids, int cnt,
+                                             int base, float *out) {
+  const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
+  const Shape s{P.T, P.U, P.R, P.stride};
+  constexpr int B = rows_per_quarter<TMAX>();
+  {
+    uint32_t rid[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const int row = base + 4 * b + quarter;
+      rid[b] = ids[row < cnt ? row : cnt - 1];
+    }
+    QLanes<TMAX, EXTRA> x[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      if constexpr (!EXACT) {
+#pragma unroll
+        for (int t = 0; t < TMAX; t++) x[b].lo[t] = x[b].hi[t] = 0.0f;
+        x[b].e[0] = x[b].r = 0.0f;
+        if constexpr (EXTRA) x[b].e[1] = x[b].e[2] = 0.0f;
+      }
+      load_q<TMAX, EXTRA, E, EXACT>(rows + (size_t)rid[b] * P.stride, s, i, x[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const int row = base + 4 * b + quarter;
+      float lo, hi, tail;
+      if (P.metric == 0) quarter_acc<TMAX, EXTRA, true, EXACT>(q, x[b], s, i, lo, hi, tail);
+      else quarter_acc<TMAX, EXTRA, false, EXACT>(q, x[b], s, i, lo, hi, tail);
+      const float h = finish_metric(quarter_reduce<EXTRA>(lo, hi, tail, s.R, quarter), P.metric);
+      if (i == 0 && row < cnt) out[row] = h;
+    }
+  }
+}
+
+// The first round is straight-line code, so its loads issue right after
+// the caller's (no loop-header wait for earlier loads in flight).
+template <int TMAX, bool EXTRA, bool EXACT, class E>
+__device__ __forceinline__ void gather_rows(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
+                                            const E *rows, const uint32_t *ids, int cnt,
+                                            float *out) {
+  constexpr int R = 4 * rows_per_quarter<TMAX>();
+  gather_round<TMAX, EXTRA, EXACT, E>(P, q, rows, ids, cnt, 0, out);
+  for (int base = R; base < cnt; base += R)
+    gather_round<TMAX, EXTRA, EXACT, E>(P, q, rows, ids, cnt, base, out);
+}
+
 // Distances from the query to rows ids[0..cnt) (ids in LDS), written to
 // out[0..cnt) (LDS), with traversal precision `prec` (0 exact f32,
 // 1 int8 codes, 2 bf16 rows).  All 64 lanes must call it.
 template <int TMAX>
 constexpr int code_chunks() { return (TMAX * 32 + 127) / 128; }
 
-template <int TMAX, bool EXTRA>
+template <int TMAX, bool EXTRA, bool EXACT>
 __device__ __forceinline__ void gather_dists(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
                                              const uint2 (&qc)[code_chunks<TMAX>()], int prec,
                                              const uint32_t *ids, int cnt, float *out) {
   const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
-  const Shape s{P.T, P.U, P.R, P.stride};
   if (prec == 1) {
     // int8 codes of the crate's quantizer: exact integer sums
     // (src/kernels/int8.rs:13-48), rescaled as src/quantization.rs:85-92.
+    // Four rows per quarter per round, loads first.
+    constexpr int CC = code_chunks<TMAX>();
     const int chunks = (int)(P.cstride >> 7);
     const float s2 = __fmul_rn(P.sq_scale, P.sq_scale);
-    for (int base = 0; base < cnt; base += 4) {
-      const int row = base + quarter;
-      int acc = 0;
-      if (row < cnt) {
-        gptr<uint2> cr = (gptr<uint2>)(P.codes + (size_t)ids[row] * P.cstride);
+    for (int base = 0; base < cnt; base += 16) {
+      uint2 cb[4][CC];
 #pragma unroll
-        for (int c = 0; c < code_chunks<TMAX>(); c++) {
-          if (c < chunks) {
-            const uint2 b = cr[c * 16 + i];
-            const uint2 a = qc[c];
-            if (P.metric == 0) {
-              acc += __builtin_amdgcn_sdot4((int)a.x, (int)a.x, 0, false) -
-                     2 * __builtin_amdgcn_sdot4((int)a.x, (int)b.x, 0, false) +
-                     __builtin_amdgcn_sdot4((int)b.x, (int)b.x, 0, false);
-              acc += __builtin_amdgcn_sdot4((int)a.y, (int)a.y, 0, false) -
-                     2 * __builtin_amdgcn_sdot4((int)a.y, (int)b.y, 0, false) +
-                     __builtin_amdgcn_sdot4((int)b.y, (int)b.y, 0, false);
-            } else {
-              acc += __builtin_amdgcn_sdot4((int)a.x, (int)b.x, 0, false);
-              acc += __builtin_amdgcn_sdot4((int)a.y, (int)b.y, 0, false);
-            }
+      for (int b = 0; b < 4; b++) {
+        const int row = base + 4 * b + quarter;
+        gptr<uint2> cr = (gptr<uint2>)(P.codes + (size_t)ids[row < cnt ? row : cnt - 1] * P.cstride);
+#pragma unroll
+        for (int c = 0; c < CC; c++)
+          cb[b][c] = (EXACT || c < chunks) ? (uint2)cr[c * 16 + i] : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const int row = base + 4 * b + quarter;
+        int acc = 0;
+#pragma unroll
+        for (int c = 0; c < CC; c++) {
+          const uint2 bb = cb[b][c];
+          const uint2 a = qc[c];
+          if (P.metric == 0) {
+            acc += __builtin_amdgcn_sdot4((int)a.x, (int)a.x, 0, false) -
+                   2 * __builtin_amdgcn_sdot4((int)a.x, (int)bb.x, 0, false) +
+                   __builtin_amdgcn_sdot4((int)bb.x, (int)bb.x, 0, false);
+            acc += __builtin_amdgcn_sdot4((int)a.y, (int)a.y, 0, false) -
+                   2 * __builtin_amdgcn_sdot4((int)a.y, (int)bb.y, 0, false) +
+                   __builtin_amdgcn_sdot4((int)bb.y, (int)bb.y, 0, false);
+          } else {
+            acc += __builtin_amdgcn_sdot4((int)a.x, (int)bb.x, 0, false);
+            acc += __builtin_amdgcn_sdot4((int)a.y, (int)bb.y, 0, false);
           }
         }
-      }
-      acc = quarter_sum_i(acc);
-      if (i == 0 && row < cnt) {
-        float v;
-        if (P.metric == 0) v = __fmul_rn(__uint2float_rn((uint32_t)acc), s2);
-        else if (P.metric == 1) v = __fmul_rn(-__int2float_rn(acc), s2);
-        else v = __fsub_rn(1.0f, __fmul_rn(__int2float_rn(acc), s2));
-        out[row] = v;
+        acc = quarter_sum_i(acc);
+        if (i == 0 && row < cnt) {
+          float v;
+          if (P.metric == 0) v = __fmul_rn(__uint2float_rn((uint32_t)acc), s2);
+          else if (P.metric == 1) v = __fmul_rn(-__int2float_rn(acc), s2);
+          else v = __fsub_rn(1.0f, __fmul_rn(__int2float_rn(acc), s2));
+          out[row] = v;
+        }
       }
     }
     return;
   }
-  constexpr int B = TMAX <= 4 ? 4 : (TMAX <= 8 ? 2 : 1);
-  for (int base = 0; base < cnt; base += 4 * B) {
-    QLanes<TMAX, EXTRA> x[B];
-#pragma unroll
-    for (int b = 0; b < B; b++) {
-      const int row = base + 4 * b + quarter;
-      if (row < cnt) {
-        if (prec == 2) load_q<TMAX, EXTRA, uint16_t>(P.bf16 + (size_t)ids[row] * P.stride, s, i, x[b]);
-        else load_q<TMAX, EXTRA, float>(P.rows + (size_t)ids[row] * P.stride, s, i, x[b]);
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < B; b++) {
-      const int row = base + 4 * b + quarter;
-      float lo = 0.0f, hi = 0.0f, tail = 0.0f;
-      if (row < cnt) {
-        if (P.metric == 0) quarter_acc<TMAX, EXTRA, true>(q, x[b], s, i, lo, hi, tail);
-        else quarter_acc<TMAX, EXTRA, false>(q, x[b], s, i, lo, hi, tail);
-      }
-      const float h = finish_metric(quarter_reduce<EXTRA>(lo, hi, tail, s.R, quarter), P.metric);
-      if (i == 0 && row < cnt) out[row] = h;
-    }
-  }
+  if (prec == 2) gather_rows<TMAX, EXTRA, EXACT, uint16_t>(P, q, P.bf16, ids, cnt, out);
+  else gather_rows<TMAX, EXTRA, EXACT, float>(P, q, P.rows, ids, cnt, out);
 }
 
 __device__ __forceinline__ bool admit(const SearchArgs &P, const uint32_t *allow, uint32_t id) {
@@ -134,7 +176,8 @@ __device__ __forceinline__ bool admit(const SearchArgs &P, const uint32_t *allow
   return true;
 }
 
-template <int TMAX, bool EXTRA, int SW>
+// EXACT: D == 32 * TMAX (branch-free row loads, see load_q).
+template <int TMAX, bool EXTRA, bool EXACT, int SW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? 4 : 2)))
 hnsw_search_kernel(const SearchArgs P) {
   constexpr int SC = 2 * SW;
@@ -146,6 +189,7 @@ hnsw_search_kernel(const SearchArgs P) {
   uint32_t *st_id = table + P.hash_cap;
   float *st_d = reinterpret_cast<float *>(st_id + 64);
   uint32_t *spill = P.spill ? P.spill + (size_t)slot * P.spill_words : nullptr;
+  gmut<uint32_t> slog = GM(P.spill_log) + (size_t)slot * P.spill_log_cap;
   const uint32_t hcap = P.hash_cap;
   const Shape shp{P.T, P.U, P.R, P.stride};
   gptr<uint32_t> adj0 = G(P.adj0);
@@ -210,12 +254,13 @@ hnsw_search_kernel(const SearchArgs P) {
     wave_sync();
     int mode = 0;
     uint32_t vcount = 0;
+    uint32_t slog_n = 0;  // nodes logged since the visited set left LDS
 
     // ---- entry point and greedy descent (search_graph, search.rs:146-161)
     uint32_t cur = P.entry;
     st_id[0] = cur;
     wave_sync();
-    gather_dists<TMAX, EXTRA>(P, q, qc, (int)prec, st_id, 1, st_d);
+    gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, 1, st_d);
     wave_sync();
     float best = rlf(st_d[0], 0);
     st.up++;
@@ -230,7 +275,7 @@ hnsw_search_kernel(const SearchArgs P) {
           const uint32_t nid = lane < c ? blk[1 + base + lane] : kInvalid;
           if (lane < c) st_id[lane] = nid;
           wave_sync();
-          gather_dists<TMAX, EXTRA>(P, q, qc, (int)prec, st_id, c, st_d);
+          gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, c, st_d);
           wave_sync();
           const float dv = lane < c ? st_d[lane] : 0.0f;
           // first list position holding the IEEE-smallest distance
@@ -281,7 +326,7 @@ hnsw_search_kernel(const SearchArgs P) {
       worst = best;
     }
     const bool pf_ok = P.m0 <= 64;
-    uint32_t pf_id = kInvalid, pf_adj = kInvalid;
+    uint32_t pf_id = kInvalid, pf_adj = kInvalid, pf2_id = kInvalid, pf2_adj = kInvalid;
 
     for (;;) {
       if (csize == 0) {
@@ -331,34 +376,67 @@ hnsw_search_kernel(const SearchArgs P) {
       const size_t arow = (size_t)cid * P.m0;
       for (uint32_t base = 0; base < P.m0; base += 64) {
         const int c = (int)(P.m0 - base < 64 ? P.m0 - base : 64);
+        const bool last = base + 64 >= P.m0;
         uint32_t nid;
         if (base == 0 && cid == pf_id) nid = pf_adj;
+        else if (base == 0 && cid == pf2_id) nid = pf2_adj;
         else nid = lane < c ? adj0[arow + base + lane] : kInvalid;
-        if (base == 0 && pf_ok) {
-          // prefetch the adjacency row of the next candidate in line
-          if (csize > 0) {
-            float nd;
-            slist_get<SC>(C, 0, nd, pf_id);
-            pf_adj = lane < (int)P.m0 ? adj0[(size_t)pf_id * P.m0 + lane] : kInvalid;
-          } else {
-            pf_id = kInvalid;
-          }
-        }
         bool isnew = false;
         if (nid != kInvalid) isnew = mode == 0 ? lds_visit(table, hcap, nid) : bit_visit(spill, nid);
         const uint64_t nm = __ballot(isnew);
         const int nn = popc64(nm);
+        if (mode == 1 && nn > 0) {
+          // log nodes marked in the bitset so the query's end clears just
+          // their words (a full clear past the log's capacity)
+          if (slog_n + (uint32_t)nn <= P.spill_log_cap && isnew) slog[slog_n + mbcnt(nm)] = nid;
+          slog_n += (uint32_t)nn;
+        }
+        // The adjacency row of the next candidate is fetched while this
+        // expansion's rows are gathered and inserted: first C's front (the
+        // next pop unless a new neighbour beats it), below the best new one.
+        if (last) {
+          pf_id = kInvalid;
+          pf2_id = kInvalid;
+          if (pf_ok && csize > 0) {
+            float nd;
+            slist_get<SC>(C, 0, nd, pf_id);
+            pf_adj = lane < (int)P.m0 ? adj0[(size_t)pf_id * P.m0 + lane] : kInvalid;
+          }
+        }
         if (nn == 0) continue;
         if (isnew) st_id[mbcnt(nm)] = nid;
         wave_sync();
         const uint32_t myid = lane < nn ? st_id[lane] : kInvalid;
         const bool adm = lane < nn && admit(P, allow, myid);
-        gather_dists<TMAX, EXTRA>(P, q, qc, (int)prec, st_id, nn, st_d);
+        gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, nn, st_d);
         wave_sync();
         const float myd = lane < nn ? st_d[lane] : 0.0f;
         const uint64_t am = __ballot(adm);
         vcount += nn;
         st.dist += nn;
+        if (last && pf_ok) {
+          // smallest new neighbour that will enter C (under the current
+          // worst result; later insertions can only exclude more)
+          const bool enters = lane < nn && (wsize < (int)ef || myd < worst);
+          uint64_t key = enters ? ckey(myd, myid) : ~0ull;
+#pragma unroll
+          for (int m = 1; m < 64; m <<= 1) {
+            const uint64_t o = ((uint64_t)(uint32_t)__shfl_xor((int)(key >> 32), m, 64) << 32) |
+                               (uint32_t)__shfl_xor((int)(uint32_t)key, m, 64);
+            key = o < key ? o : key;
+          }
+          uint64_t front = ~0ull;
+          if (csize > 0) {
+            float fd;
+            uint32_t fi;
+            slist_get<SC>(C, 0, fd, fi);
+            front = ckey(fd, fi);
+          }
+          if (key < front) {
+            pf2_id = (uint32_t)key;
+            pf2_adj = lane < (int)P.m0 ? adj0[(size_t)pf2_id * P.m0 + lane] : kInvalid;
+          }
+        }
         uint64_t cm = wsize >= (int)ef ? __ballot(lane < nn && myd < worst) : __ballot(lane < nn);
         while (cm) {
           const int b = __ffsll((unsigned long long)cm) - 1;
@@ -417,6 +495,21 @@ hnsw_search_kernel(const SearchArgs P) {
       }
     }
 
+    if (mode == 1) {
+      // clear this query's bits: the LDS table's nodes and the logged ones
+      __threadfence();
+      for (uint32_t t = lane; t < hcap; t += 64) {
+        const uint32_t v = table[t];
+        if (v != kInvalid) GM(spill)[v >> 5] = 0u;
+      }
+      if (slog_n <= P.spill_log_cap) {
+        for (uint32_t j = lane; j < slog_n; j += 64) GM(spill)[slog[j] >> 5] = 0u;
+      } else {
+        for (uint32_t w = lane; w < P.spill_words; w += 64) GM(spill)[w] = 0u;
+      }
+      __threadfence();
+    }
+
     // ---- results: optional exact re-rank, keys, (distance, key) order
     float *res_d = reinterpret_cast<float *>(table);
     uint32_t *res_id = table + 64 * SW;
@@ -433,7 +526,7 @@ hnsw_search_kernel(const SearchArgs P) {
     if (prec != 0) {
       for (int base = 0; base < wsize; base += 64) {
         const int c = wsize - base < 64 ? wsize - base : 64;
-        gather_dists<TMAX, EXTRA>(P, q, qc, 0, res_id + base, c, res_d + base);
+        gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, 0, res_id + base, c, res_d + base);
         wave_sync();
       }
       st.rr += wsize;
@@ -466,10 +559,6 @@ hnsw_search_kernel(const SearchArgs P) {
       }
     }
     GM(P.out_count)[qi] = (uint32_t)(wsize < (int)k ? wsize : (int)k);
-    if (mode == 1) {
-      for (uint32_t w = lane; w < P.spill_words; w += 64) GM(spill)[w] = 0u;
-      __threadfence();
-    }
     wave_sync();
   }
   if (lane == 0 && P.stats) {
@@ -546,16 +635,19 @@ __global__ void prepare_queries_kernel(const PrepArgs A) {
     for (uint32_t e = A.dim; e < A.cstride; e++) code[e] = 0;
 }
 
-#define HH_SEARCH_INST(TM, EX)                                                      \
-  template __global__ void hnsw_search_kernel<TM, EX, 1>(const SearchArgs);         \
-  template __global__ void hnsw_search_kernel<TM, EX, 2>(const SearchArgs);         \
-  template __global__ void hnsw_search_kernel<TM, EX, 4>(const SearchArgs);         \
-  template __global__ void hnsw_search_kernel<TM, EX, 8>(const SearchArgs);
-HH_SEARCH_INST(4, false)
-HH_SEARCH_INST(4, true)
-HH_SEARCH_INST(8, false)
-HH_SEARCH_INST(8, true)
-HH_SEARCH_INST(32, false)
-HH_SEARCH_INST(32, true)
+#define HH_SEARCH_INST(TM, EX, XA)                                                  \
+  template __global__ void hnsw_search_kernel<TM, EX, XA, 1>(const SearchArgs);     \
+  template __global__ void hnsw_search_kernel<TM, EX, XA, 2>(const SearchArgs);     \
+  template __global__ void hnsw_search_kernel<TM, EX, XA, 4>(const SearchArgs);     \
+  template __global__ void hnsw_search_kernel<TM, EX, XA, 8>(const SearchArgs);
+HH_SEARCH_INST(4, false, false)
+HH_SEARCH_INST(4, true, false)
+HH_SEARCH_INST(4, false, true)
+HH_SEARCH_INST(8, false, false)
+HH_SEARCH_INST(8, true, false)
+HH_SEARCH_INST(8, false, true)
+HH_SEARCH_INST(32, false, false)
+HH_SEARCH_INST(32, true, false)
+HH_SEARCH_INST(32, false, true)
 
 }  // namespace hhk

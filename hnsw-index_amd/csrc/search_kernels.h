@@ -47,6 +47,8 @@ struct SearchArgs {
   uint64_t spill_words;
   uint64_t *cpool;              // per-wave candidate pools (cpool_cap each)
   uint32_t cpool_cap;
+  uint32_t *spill_log;          // per-wave ids marked in the bitset (spill_log_cap each)
+  uint32_t spill_log_cap;
   unsigned long long *stats;    // 7 counters
   uint32_t hash_cap, hash_limit, lds_words;
 };
@@ -69,7 +71,7 @@ struct PrepArgs {
   uint32_t *counter;
 };
 
-template <int TMAX, bool EXTRA, int SW>
+template <int TMAX, bool EXTRA, bool EXACT, int SW>
 __global__ void hnsw_search_kernel(const SearchArgs P);
 __global__ void prepare_queries_kernel(const PrepArgs A);
 

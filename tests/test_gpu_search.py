@@ -175,11 +175,23 @@ def test_bf16_traversal_reranks_exactly():
 
 
 def test_large_ef_spills_visited_set_and_stays_exact():
-    b, q = O.dataset_gaussian(20000, 8, 16, 31)
-    idx = build(b, threads=8)
-    (gk, gd, gc), (ok, od, oc), st, cnt = _compare(idx, q, 50, 512)
-    assert np.array_equal(gk, ok) and same_bits(gd, od)
-    assert st["n_overflow"] == 0
+    """Queries that outgrow the LDS visited table continue on their HBM
+    bitset and clear it afterwards (word by word from the spill log, or
+    whole once the log overflows); every later query on the same
+    wavefront must start clean, so all results stay the oracle's."""
+    b, q = O.dataset_gaussian(20000, 40, 16, 31)
+    idx = build(b, m=32, threads=8)
+    h = idx.device_index()
+    for cap, wpb in ((0, 0), (2048, 2)):  # the table spills past 1536 nodes
+        print(f"case cap={cap} wpb={wpb}", flush=True)
+        assert H.lib.hh_index_set_launch_shape(h, 512, cap, wpb) == 0
+        (gk, gd, gc), (ok, od, oc), st, cnt = _compare(idx, q, 50, 512)
+        assert np.array_equal(gk, ok) and same_bits(gd, od)
+        assert st["n_overflow"] == 0 and st["n_visited_spill"] > 0
+        assert st["n_dist"] + st["n_upper"] == cnt.n_dist
+        # mostly more than 4096 nodes marked after the spill: full clears
+        assert cnt.n_dist > 40 * (4096 + 1536)
+    assert H.lib.hh_index_set_launch_shape(h, 512, 0, 0) == 0
 
 
 def test_mixed_batch_equals_individual_searches():
@@ -285,3 +297,28 @@ def test_selective_filters_match_the_oracle_exactly(ef):
     ro = O.recall_at_k(tk, tc, np.stack([x[0] for x in ora_found]),
                        np.array([x[1] for x in ora_found], np.uint32), 10)
     assert abs(rg - ro) <= 0.005, (rg, ro)
+
+
+def test_launch_shapes_do_not_change_results():
+    """Visited-set capacity (down to one that spills almost every query to
+    its HBM bitset) and waves per block only change speed: results,
+    distances and counters stay the oracle's, for the exact-layout kernel
+    (D = 128) and the general one (D = 100)."""
+    for dim in (128, 100):
+        b, q = O.dataset_gaussian(3000, 64, dim, 31)
+        idx = build(b)
+        h = idx.device_index()
+        for ef, shapes in ((64, [(256, 1), (1024, 1), (1536, 2), (2048, 4)]),
+                           (100, [(512, 1), (2048, 2), (4096, 1)])):
+            print(f"case dim={dim} ef={ef}", flush=True)
+            for cap, wpb in shapes:
+                assert H.lib.hh_index_set_launch_shape(h, ef, cap, wpb) == 0, (cap, wpb)
+                (gk, gd, gc), (ok, od, oc), st, cnt = _compare(idx, q, 10, ef)
+                assert np.array_equal(gk, ok) and same_bits(gd, od) and np.array_equal(gc, oc), \
+                    (dim, ef, cap, wpb)
+                assert st["n_dist"] + st["n_upper"] == cnt.n_dist and st["n_expand"] == cnt.n_expand
+                if cap <= 512:
+                    assert st["n_visited_spill"] > 0  # the bitset path ran
+            assert H.lib.hh_index_set_launch_shape(h, ef, 0, 0) == 0
+        assert H.lib.hh_index_set_launch_shape(h, 64, 2048, 3) != 0
+        assert H.lib.hh_index_set_launch_shape(h, 300, 512, 1) != 0
