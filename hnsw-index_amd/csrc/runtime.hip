@@ -125,6 +125,11 @@ int tmax_for(const hh::RowLayout &lay) {
   if (lay.T <= 32) return 32;
   return 0;
 }
+// LDS words per wavefront of the search kernel: the visited table, the
+// 64-entry id and distance staging, C (128 * sw candidate words) and W
+// (64 * sw result words), candidate words being 2 words each.
+uint32_t lds_words_for(uint32_t hash_cap, int sw) { return hash_cap + 128 + 384u * (uint32_t)sw; }
+
 int slots_for(uint32_t ef) {
   if (ef <= 64) return 1;
   if (ef <= 128) return 2;
@@ -487,7 +492,7 @@ int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
   const int bucket = sw == 1 ? 0 : sw == 2 ? 1 : sw == 4 ? 2 : 3;
   cfg.hash_cap = idx->tuned_hash_cap[bucket] ? idx->tuned_hash_cap[bucket] : 2048u;
   cfg.hash_cap = std::max<uint32_t>(cfg.hash_cap, 256u * (uint32_t)sw);
-  cfg.lds_words = cfg.hash_cap + 128;
+  cfg.lds_words = lds_words_for(cfg.hash_cap, sw);
   cfg.kern = pick_search_kernel(idx->lay, sw);
   // waves per block: the tuned value, else whichever of 1, 2, 4 keeps the
   // most waves resident
@@ -941,7 +946,7 @@ int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t
   for (uint32_t cap : caps) {
     if (cap < 256u * (uint32_t)sw || cap * 4 < efmax * 16) continue;
     for (int w : {1, 2, 4}) {
-      if ((cap + 128) * 4 * (uint32_t)w > 65536) continue;
+      if ((uint64_t)lds_words_for(cap, sw) * 4 * (uint32_t)w > 65536) continue;
       idx->tuned_hash_cap[bucket] = cap;
       idx->tuned_wpb[bucket] = w;
       float ms_best = 1e30f;
@@ -987,7 +992,7 @@ int hh_index_set_launch_shape(hh_index *idx, uint32_t ef, uint32_t hash_cap, int
   }
   if (wpb != 1 && wpb != 2 && wpb != 4)
     return fail(HH_INVALID_ARGUMENT, "waves per block must be 1, 2 or 4");
-  if (hash_cap < 256u * (uint32_t)sw || (uint64_t)(hash_cap + 128) * 4 * (uint64_t)wpb > 65536)
+  if (hash_cap < 256u * (uint32_t)sw || (uint64_t)lds_words_for(hash_cap, sw) * 4 * (uint64_t)wpb > 65536)
     return fail(HH_INVALID_ARGUMENT, "visited-set capacity out of range for this ef and block");
   idx->tuned_hash_cap[bucket] = hash_cap;
   idx->tuned_wpb[bucket] = wpb;

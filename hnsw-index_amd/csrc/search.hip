@@ -51,6 +51,146 @@ __device__ __forceinline__ void pool_push(gmut<uint64_t> pool, bool has_pool, ui
   pn += room ? 1 : 0;
 }
 
+// ---------------------------------------------------------------- lists
+// C (candidates) and W (results) are sorted arrays of candidate words
+// (`ckey`) in the wave's LDS.  A neighbour batch is merged in one pass: the
+// list is read into registers (element i in lane i % 64, chunk i / 64), every
+// element's merged position is its rank, counted with ballots against the
+// batch's few new elements (a scalar loop over them, independent steps),
+// then all lanes write.  The caller issues the merges with all 64 lanes
+// (wave-uniform arguments).
+
+// Rank of each batch element (lanes in `amask`, word `akey`) among the
+// list elements e[0..CPL) (valid where chunk*64 + lane < size), plus the
+// number of batch elements ordering before it; and, for every list
+// element, its index plus the number of batch elements ordering before it.
+template <int CPL>
+__device__ __forceinline__ uint32_t merge_ranks(const uint64_t (&e)[CPL], uint32_t (&ep)[CPL],
+                                                uint32_t size, uint64_t amask, uint64_t akey) {
+  const int lane = lane_id();
+  uint32_t ap = 0;
+#pragma unroll
+  for (int c = 0; c < CPL; c++) ep[c] = c * 64 + lane;
+  for (uint64_t m = amask; m; m &= m - 1) {
+    const int j = __ffsll((unsigned long long)m) - 1;
+    const uint64_t kj = rl64(akey, j);
+    uint32_t r = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+      const bool valid = c * 64 + lane < (int)size;
+      r += (uint32_t)popc64(__ballot(valid && e[c] < kj));
+      ep[c] += kj < e[c] ? 1u : 0u;
+    }
+    ap += (kj < akey ? 1u : 0u) + (lane == j ? r : 0u);
+  }
+  return ap;
+}
+
+// W <- the `cap` smallest of W[0..size) (held in e[]) and the batch
+// elements in `amask`; returns the new worst (largest) word.
+template <int WPL>
+__device__ __forceinline__ uint64_t w_merge(uint64_t *W, const uint64_t (&e)[WPL], uint32_t &size,
+                                            uint32_t cap, uint64_t amask, uint64_t akey) {
+  const int lane = lane_id();
+  const bool mine = (amask >> lane) & 1;
+  uint32_t ep[WPL];
+  const uint32_t ap = merge_ranks<WPL>(e, ep, size, amask, akey);
+  const uint32_t total = size + (uint32_t)popc64(amask);
+  const uint32_t nsize = total < cap ? total : cap;
+  uint64_t last = ~0ull;
+  const uint64_t la = __ballot(mine && ap == nsize - 1);
+  if (la) last = rl64(akey, __ffsll((unsigned long long)la) - 1);
+#pragma unroll
+  for (int c = 0; c < WPL; c++) {
+    const bool valid = c * 64 + lane < (int)size;
+    const uint64_t lb = __ballot(valid && ep[c] == nsize - 1);
+    if (lb) last = rl64(e[c], __ffsll((unsigned long long)lb) - 1);
+    if (valid && ep[c] < nsize) W[ep[c]] = e[c];
+  }
+  if (mine && ap < nsize) W[ap] = akey;
+  size = nsize;
+  return last;
+}
+
+// C[head..head+size) (held in e[]) merged with the batch elements in
+// `amask` into C[0..): the merged order's first `keep` entries stay, keep =
+// min(ccap, entries below pmin); the rest go to the pool unless dead
+// (results full and distance above the worst), and pmin stays a lower
+// bound of the pool.  With results full, dead entries are also trimmed from
+// C's end.
+template <int CPL>
+__device__ __forceinline__ void c_merge(uint64_t *C, const uint64_t (&e)[CPL], uint32_t &head,
+                                        uint32_t &size, uint32_t ccap, uint64_t amask,
+                                        uint64_t akey, bool wfull, float worst,
+                                        gmut<uint64_t> pool, bool has_pool, uint32_t pool_cap,
+                                        uint32_t &pn, uint64_t &pmin, unsigned long long &ovf,
+                                        unsigned long long &moved) {
+  const int lane = lane_id();
+  const bool mine = (amask >> lane) & 1;
+  uint32_t ep[CPL];
+  const uint32_t ap = merge_ranks<CPL>(e, ep, size, amask, akey);
+  // every entry of C is below pmin; batch entries at or above it go out
+  const uint32_t below = size + (uint32_t)popc64(__ballot(mine && akey < pmin));
+  const uint32_t keep = below < ccap ? below : ccap;
+  wave_sync();  // every lane has read C before any writes
+  // the smallest entry leaving C bounds the pool from below
+  const uint64_t at_keep_a = __ballot(mine && ap == keep);
+  uint64_t first_out = at_keep_a ? rl64(akey, __ffsll((unsigned long long)at_keep_a) - 1) : ~0ull;
+  uint32_t live = 0;  // kept entries that are not dead
+#pragma unroll
+  for (int c = 0; c < CPL; c++) {
+    const bool valid = c * 64 + lane < (int)size;
+    const uint64_t b = __ballot(valid && ep[c] == keep);
+    if (b) first_out = rl64(e[c], __ffsll((unsigned long long)b) - 1);
+    const bool dead = wfull && ckey_d(e[c]) > worst;
+    if (valid && ep[c] < keep) C[ep[c]] = e[c];
+    live += (uint32_t)popc64(__ballot(valid && ep[c] < keep && !dead));
+    const bool out = valid && ep[c] >= keep && !dead;
+    const uint64_t om = __ballot(out);
+    if (om) {
+      const uint32_t room = has_pool ? pool_cap - pn : 0u;
+      const uint32_t slot = (uint32_t)mbcnt(om);
+      if (out && slot < room) pool[pn + slot] = e[c];
+      const uint32_t no = (uint32_t)popc64(om), put = no < room ? no : room;
+      pn += put;
+      moved += put;
+      ovf += no - put;
+    }
+  }
+  {
+    const bool dead = wfull && ckey_d(akey) > worst;
+    if (mine && ap < keep) C[ap] = akey;
+    live += (uint32_t)popc64(__ballot(mine && ap < keep && !dead));
+    const bool out = mine && ap >= keep && !dead;
+    const uint64_t om = __ballot(out);
+    if (om) {
+      const uint32_t room = has_pool ? pool_cap - pn : 0u;
+      const uint32_t slot = (uint32_t)mbcnt(om);
+      if (out && slot < room) pool[pn + slot] = akey;
+      const uint32_t no = (uint32_t)popc64(om), put = no < room ? no : room;
+      pn += put;
+      moved += put;
+      ovf += no - put;
+    }
+  }
+  if (first_out < pmin && !(wfull && ckey_d(first_out) > worst)) pmin = first_out;
+  wave_sync();
+  head = 0;
+  size = wfull ? live : keep;  // dead entries form C's tail: trimmed
+}
+
+// Loads list entries [base, base + size) into registers (element i in lane
+// i % 64, chunk i / 64; ~0 past the end).
+template <int CPL>
+__device__ __forceinline__ void list_load(const uint64_t *L, uint32_t size, uint64_t (&e)[CPL]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int c = 0; c < CPL; c++) {
+    const uint32_t i = c * 64 + lane;
+    e[c] = i < size ? L[i] : ~0ull;
+  }
+}
+
 // f32 or bf16 rows: B rows per quarter per round, all of a round's loads
 // issued before any is used (rows past cnt repeat the last id, so every
 // lane loads unconditionally and no branch splits the loads).
@@ -180,7 +320,6 @@ __device__ __forceinline__ bool admit(const SearchArgs &P, const uint32_t *allow
 template <int TMAX, bool EXTRA, bool EXACT, int SW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? 4 : 2)))
 hnsw_search_kernel(const SearchArgs P) {
-  constexpr int SC = 2 * SW;
   extern __shared__ uint32_t lds[];
   const int lane = lane_id(), i16 = lane & 15;
   const int wave = threadIdx.x >> 6;
@@ -188,6 +327,8 @@ hnsw_search_kernel(const SearchArgs P) {
   uint32_t *table = lds + (size_t)wave * P.lds_words;
   uint32_t *st_id = table + P.hash_cap;
   float *st_d = reinterpret_cast<float *>(st_id + 64);
+  uint64_t *Ck = reinterpret_cast<uint64_t *>(st_d + 64);  // 128 * SW candidate words
+  uint64_t *Wk = Ck + 128 * SW;                              // 64 * SW result words
   uint32_t *spill = P.spill ? P.spill + (size_t)slot * P.spill_words : nullptr;
   gmut<uint32_t> slog = GM(P.spill_log) + (size_t)slot * P.spill_log_cap;
   const uint32_t hcap = P.hash_cap;
@@ -227,7 +368,7 @@ hnsw_search_kernel(const SearchArgs P) {
     // Candidate-list capacity depends on the query's own ef only (twice
     // its 64-multiple bucket), never on the kernel instance, so a query
     // returns the same results alone, in a mixed batch or when served.
-    const int ccap = ef <= 64 ? 128 : ef <= 128 ? 256 : ef <= 256 ? 512 : 1024;
+    const uint32_t ccap = ef <= 64 ? 128 : ef <= 128 ? 256 : ef <= 256 ? 512 : 1024;
     const uint32_t *allow = P.allow;
     if (P.specs) allow = filt == 0xFFFFFFFFu ? nullptr : P.filters + (size_t)filt * P.filter_words;
     st.q++;
@@ -305,73 +446,55 @@ hnsw_search_kernel(const SearchArgs P) {
     }
 
     // ---- layer 0 (search_layer, search.rs:81-142)
-    SList<SW> W;
-    SList<SC> C;
-#pragma unroll
-    for (int s = 0; s < SW; s++) { W.d[s] = 0.0f; W.id[s] = kInvalid; }
-#pragma unroll
-    for (int s = 0; s < SC; s++) { C.d[s] = 0.0f; C.id[s] = kInvalid; }
-    int wsize = 0, csize = 0;
+    // C and W: sorted candidate words in LDS (C from `head`, W from 0).
+    uint32_t head = 0, csize = 0, wsize = 0;
     float worst = 0.0f;
     uint32_t pn = 0;            // pooled candidates
-    uint64_t pmin = ~0ull;      // smallest pooled candidate
+    uint64_t pmin = ~0ull;      // lower bound of the pooled candidates
     lds_visit(table, hcap, cur);
     wave_sync();
     vcount = 1;
-    slist_insert_at<SC>(C, 0, best, cur);
+    const uint64_t ep_key = ckey(best, cur);
+    Ck[0] = ep_key;
     csize = 1;
     if (admit(P, allow, cur)) {
-      slist_insert_at<SW>(W, 0, best, cur);
+      Wk[0] = ep_key;
       wsize = 1;
       worst = best;
     }
+    wave_sync();
     const bool pf_ok = P.m0 <= 64;
     uint32_t pf_id = kInvalid, pf_adj = kInvalid, pf2_id = kInvalid, pf2_adj = kInvalid;
 
     for (;;) {
       if (csize == 0) {
         if (pn == 0) break;
-        // refill C with the smallest pooled entries; the rest stay pooled
-        // (compacted in place: each processed entry writes at most one)
+        // refill C from the pool in chunks of 64, each merged like a
+        // neighbour batch; what does not fit goes back to the pool, which
+        // is compacted in place (a chunk writes at most as many entries as
+        // it read)
         const uint32_t total = pn;
-        uint32_t w = 0;
-        uint64_t nmin = ~0ull;
+        const bool wfull = wsize >= ef;
+        pn = 0;
+        pmin = ~0ull;
+        head = 0;
         for (uint32_t base = 0; base < total; base += 64) {
           const uint32_t c = total - base < 64 ? total - base : 64;
-          const uint64_t mine = lane < (int)c ? pool[base + lane] : ~0ull;
-          for (uint32_t j = 0; j < c; j++) {
-            const uint64_t x = rl64(mine, (int)j);
-            const float xd = ckey_d(x);
-            if (wsize >= (int)ef && xd > worst) continue;
-            uint64_t out = x;
-            if (csize == ccap) {
-              float md;
-              uint32_t mid;
-              slist_get<SC>(C, csize - 1, md, mid);
-              const uint64_t cmax = ckey(md, mid);
-              if (x < cmax) {
-                slist_insert_at<SC>(C, slist_rank<SC>(C, csize, xd, (uint32_t)x), xd, (uint32_t)x);
-                out = cmax;
-              }
-              pool[w] = out;
-              w++;
-              nmin = out < nmin ? out : nmin;
-            } else {
-              slist_insert_at<SC>(C, slist_rank<SC>(C, csize, xd, (uint32_t)x), xd, (uint32_t)x);
-              csize++;
-            }
-          }
+          const uint64_t x = lane < (int)c ? (uint64_t)pool[base + lane] : ~0ull;
+          const uint64_t live = __ballot(lane < (int)c && !(wfull && ckey_d(x) > worst));
+          uint64_t ce[2 * SW];
+          list_load<2 * SW>(Ck + head, csize, ce);
+          c_merge<2 * SW>(Ck, ce, head, csize, ccap, live, x, wfull, worst, pool, has_pool,
+                          P.cpool_cap, pn, pmin, st.ovf, st.cspill);
         }
-        pn = w;
-        pmin = nmin;
         if (csize == 0) break;
       }
-      float cd;
-      uint32_t cid;
-      slist_get<SC>(C, 0, cd, cid);
-      slist_pop_front<SC>(C);
+      const uint64_t top = Ck[head];
+      const float cd = ckey_d(top);
+      const uint32_t cid = (uint32_t)top;
+      head++;
       csize--;
-      if (wsize >= (int)ef && cd > worst) break;
+      if (wsize >= ef && cd > worst) break;
       st.exp++;
       const size_t arow = (size_t)cid * P.m0;
       for (uint32_t base = 0; base < P.m0; base += 64) {
@@ -392,14 +515,13 @@ hnsw_search_kernel(const SearchArgs P) {
           slog_n += (uint32_t)nn;
         }
         // The adjacency row of the next candidate is fetched while this
-        // expansion's rows are gathered and inserted: first C's front (the
+        // expansion's rows are gathered and merged: first C's front (the
         // next pop unless a new neighbour beats it), below the best new one.
         if (last) {
           pf_id = kInvalid;
           pf2_id = kInvalid;
           if (pf_ok && csize > 0) {
-            float nd;
-            slist_get<SC>(C, 0, nd, pf_id);
+            pf_id = (uint32_t)Ck[head];
             pf_adj = lane < (int)P.m0 ? adj0[(size_t)pf_id * P.m0 + lane] : kInvalid;
           }
         }
@@ -414,73 +536,51 @@ hnsw_search_kernel(const SearchArgs P) {
         const uint64_t am = __ballot(adm);
         vcount += nn;
         st.dist += nn;
-        if (last && pf_ok) {
-          // smallest new neighbour that will enter C (under the current
-          // worst result; later insertions can only exclude more)
-          const bool enters = lane < nn && (wsize < (int)ef || myd < worst);
-          uint64_t key = enters ? ckey(myd, myid) : ~0ull;
+        // Which new neighbours the crate's sequential loop accepts
+        // (search.rs:118-132): neighbour t passes iff |W_t| < ef or
+        // d_t < worst_t, and worst_t is the ef-th smallest distance of W and
+        // the admitted neighbours before t.  So t passes iff fewer than ef
+        // of those are at or below d_t.  (Neighbours at or above the worst
+        // at the batch start can never pass.)
+        const uint64_t sm = __ballot(lane < nn && (wsize < ef || myd < worst));
+        const bool surv = (sm >> lane) & 1;
+        uint64_t we[SW];
+        list_load<SW>(Wk, wsize, we);
+        uint32_t below = 0;
+        for (uint64_t m = sm; m; m &= m - 1) {
+          const int j = __ffsll((unsigned long long)m) - 1;
+          const float dj = rlf(myd, j);
+          uint32_t cw = 0;  // results at or below d_j
+#pragma unroll
+          for (int c = 0; c < SW; c++)
+            cw += (uint32_t)popc64(__ballot(c * 64 + lane < (int)wsize && !(ckey_d(we[c]) > dj)));
+          below += (lane == j ? cw : 0u) + (((am >> j) & 1) && j < lane && !(dj > myd) ? 1u : 0u);
+        }
+        const bool acc = surv && below < ef;
+        const uint64_t amask = __ballot(acc);
+        const uint64_t akey = acc ? ckey(myd, myid) : ~0ull;
+        if (last && pf_ok && amask) {
+          // the smallest accepted neighbour, if it beats C's front, is the
+          // next candidate: fetch its adjacency row during the merges
+          uint64_t key = akey;
 #pragma unroll
           for (int m = 1; m < 64; m <<= 1) {
             const uint64_t o = ((uint64_t)(uint32_t)__shfl_xor((int)(key >> 32), m, 64) << 32) |
                                (uint32_t)__shfl_xor((int)(uint32_t)key, m, 64);
             key = o < key ? o : key;
           }
-          uint64_t front = ~0ull;
-          if (csize > 0) {
-            float fd;
-            uint32_t fi;
-            slist_get<SC>(C, 0, fd, fi);
-            front = ckey(fd, fi);
-          }
-          if (key < front) {
+          if (csize == 0 || key < Ck[head]) {
             pf2_id = (uint32_t)key;
             pf2_adj = lane < (int)P.m0 ? adj0[(size_t)pf2_id * P.m0 + lane] : kInvalid;
           }
         }
-        uint64_t cm = wsize >= (int)ef ? __ballot(lane < nn && myd < worst) : __ballot(lane < nn);
-        while (cm) {
-          const int b = __ffsll((unsigned long long)cm) - 1;
-          cm &= cm - 1;
-          const float di = rlf(myd, b);
-          const uint32_t ii = rl(myid, b);
-          if (!(wsize < (int)ef || di < worst)) continue;
-          if ((am >> b) & 1) {
-            const int p = slist_rank<SW>(W, wsize, di, ii);
-            slist_insert_at<SW>(W, p, di, ii);
-            if (wsize < (int)ef) wsize++;
-            float wd;
-            uint32_t wi;
-            slist_get<SW>(W, wsize - 1, wd, wi);
-            worst = wd;
-          }
-          if (csize == ccap && wsize >= (int)ef) csize = slist_count_not_above<SC>(C, csize, worst);
-          const uint64_t x = ckey(di, ii);
-          bool to_c;
-          if (csize == 0) {
-            to_c = pn == 0 || x < pmin;
-          } else {
-            float md;
-            uint32_t mid;
-            slist_get<SC>(C, csize - 1, md, mid);
-            const uint64_t cmax = ckey(md, mid);
-            if (x < cmax) {
-              to_c = true;
-              if (csize == ccap)  // C's largest moves out
-                pool_push(pool, has_pool, P.cpool_cap, wsize >= (int)ef && md > worst, cmax, pn,
-                          pmin, st.ovf, st.cspill);
-            } else {
-              to_c = csize < ccap && (pn == 0 || x < pmin);
-            }
-          }
-          if (to_c) {
-            slist_insert_at<SC>(C, slist_rank<SC>(C, csize, di, ii), di, ii);
-            if (csize < ccap) csize++;
-          } else {
-            pool_push(pool, has_pool, P.cpool_cap, wsize >= (int)ef && di > worst, x, pn, pmin,
-                      st.ovf, st.cspill);
-          }
+        if (amask) {
+          uint64_t ce[2 * SW];
+          list_load<2 * SW>(Ck + head, csize, ce);
+          if (amask & am) worst = ckey_d(w_merge<SW>(Wk, we, wsize, ef, amask & am, akey));
+          c_merge<2 * SW>(Ck, ce, head, csize, ccap, amask, akey, wsize >= ef, worst, pool,
+                          has_pool, P.cpool_cap, pn, pmin, st.ovf, st.cspill);
         }
-        wave_sync();
         if (mode == 0 && vcount > P.hash_limit && spill) {
           for (uint32_t t = lane; t < hcap; t += 64) {
             const uint32_t v = table[t];
@@ -514,13 +614,10 @@ hnsw_search_kernel(const SearchArgs P) {
     float *res_d = reinterpret_cast<float *>(table);
     uint32_t *res_id = table + 64 * SW;
     uint32_t *res_key = table + 128 * SW;  // 2 words per entry
-#pragma unroll
-    for (int s = 0; s < SW; s++) {
-      const int idx = lane * SW + s;
-      if (idx < wsize) {
-        res_d[idx] = W.d[s];
-        res_id[idx] = W.id[s];
-      }
+    for (uint32_t idx = lane; idx < wsize; idx += 64) {
+      const uint64_t w = Wk[idx];
+      res_d[idx] = ckey_d(w);
+      res_id[idx] = (uint32_t)w;
     }
     wave_sync();
     if (prec != 0) {
