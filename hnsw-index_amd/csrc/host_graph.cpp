@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -840,6 +841,376 @@ double hh_recall_at_k(uint64_t nq, const uint64_t *truth, const uint32_t *truth_
         }
   }
   return expected == 0 ? 1.0 : (double)hits / (double)expected;
+}
+
+}  // extern "C"
+
+// ==================================================================== files
+// The crate's versioned, checksummed index format, version 1
+// (src/hnsw/io.rs:1-28 describes the layout; writer :103-167, loader
+// :218-540).  Little-endian throughout; a CRC-32 (IEEE) of every preceding
+// byte closes the file.  Loading treats the input as untrusted: the
+// envelope is checked first, every column is located before anything is
+// parsed, the adjacency section is walked without allocating, and the heap
+// the graph would take is compared with the caller's limit before any graph
+// storage is allocated.
+
+namespace {
+
+// CRC-32/ISO-HDLC: reflected polynomial 0xEDB88320, initial and final
+// complement (src/crc32.rs:1-7), one 256-entry table.
+struct CrcTable {
+  uint32_t t[256];
+  CrcTable() {
+    for (uint32_t i = 0; i < 256; i++) {
+      uint32_t c = i;
+      for (int b = 0; b < 8; b++) c = (c & 1) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
+      t[i] = c;
+    }
+  }
+};
+const CrcTable &crc_table() {
+  static const CrcTable table;
+  return table;
+}
+uint32_t crc32_update(uint32_t crc, const uint8_t *p, size_t n) {
+  const uint32_t *t = crc_table().t;
+  for (size_t i = 0; i < n; i++) crc = t[(crc ^ p[i]) & 0xFF] ^ (crc >> 8);
+  return crc;
+}
+
+constexpr uint8_t kMagic[8] = {'H', 'N', 'S', 'W', 'I', 'D', 'X', 0};
+constexpr uint32_t kVersion = 1;
+constexpr size_t kHeaderLen = 56, kTrailerLen = 4;
+constexpr double kLevelSumSlack = 1024.0;     // src/hnsw/io.rs:69-71
+constexpr double kUnitNormTolerance = 1e-3;   // src/hnsw/io.rs:72-73
+constexpr uint64_t kMemoryFactor = 64;        // src/hnsw/io.rs:74-81
+constexpr uint64_t kMemoryFloor = 64ull << 20;
+
+struct Out {
+  std::vector<uint8_t> b;
+  void bytes(const void *p, size_t n) {
+    const uint8_t *s = (const uint8_t *)p;
+    b.insert(b.end(), s, s + n);
+  }
+  void u8(uint8_t v) { b.push_back(v); }
+  void u32(uint32_t v) { bytes(&v, 4); }
+  void u64(uint64_t v) { bytes(&v, 8); }
+  void f64(double v) { bytes(&v, 8); }
+};
+
+// Bounds-checked reader over the checksummed body.
+struct In {
+  const uint8_t *p;
+  size_t len, pos;
+  bool take(size_t n, const uint8_t **out) {
+    if (n > len - pos) return false;
+    *out = p + pos;
+    pos += n;
+    return true;
+  }
+  template <class T>
+  bool get(T *v) {
+    const uint8_t *s;
+    if (!take(sizeof(T), &s)) return false;
+    std::memcpy(v, s, sizeof(T));
+    return true;
+  }
+};
+
+int format_error(hh_format_info *info, uint32_t kind, const char *what, const std::string &msg) {
+  if (info) {
+    info->kind = kind;
+    std::snprintf(info->what, sizeof info->what, "%s", what ? what : "");
+  }
+  return fail(HH_FORMAT_ERROR, "invalid index file: " + msg);
+}
+int truncated(hh_format_info *info) {
+  return format_error(info, HH_FORMAT_TRUNCATED, "", "truncated");
+}
+int invalid(hh_format_info *info, const char *what) {
+  return format_error(info, HH_FORMAT_INVALID, what, std::string("invalid ") + what);
+}
+
+// Heap bytes of a loaded graph in this library's layout: rows, keys,
+// levels, tombstones, per-node lock and upper-block offset, layer-0 block,
+// key-map entry, upper blocks, int8 codes and quantizer offsets.
+uint64_t graph_heap_bytes(uint64_t n, uint32_t dim, uint32_t m, uint32_t m0, uint64_t level_sum,
+                          bool quantized) {
+  const uint64_t per_node = 4ull * dim + 8 + 1 + 1 + sizeof(std::mutex) + 8 +
+                            4ull * (m0 + 1) + 48 + (quantized ? dim : 0);
+  return n * per_node + level_sum * 4ull * (m + 1) + (quantized ? 4ull * dim : 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t hh_crc32(const uint8_t *data, uint64_t len) {
+  if (!data) len = 0;
+  return ~crc32_update(~0u, data, (size_t)len);
+}
+
+int hh_graph_info(const hh_graph *g, uint32_t *dim, uint32_t *metric, hh_params *params) {
+  if (!g) return fail(HH_INVALID_ARGUMENT, "null argument");
+  if (dim) *dim = g->dim;
+  if (metric) *metric = g->metric;
+  if (params) *params = g->p;
+  return HH_OK;
+}
+
+uint64_t hh_graph_memory_usage(const hh_graph *g) {
+  if (!g) return 0;
+  uint64_t level_sum = 0;
+  for (uint8_t l : g->levels) level_sum += l;
+  return graph_heap_bytes(g->n(), g->dim, g->p.m, g->p.m0, level_sum, g->quantized);
+}
+
+// HnswIndex::to_bytes / write_to (src/hnsw/mod.rs:611-646, io.rs:103-167).
+int hh_graph_to_bytes(const hh_graph *g, uint8_t *buf, uint64_t cap, uint64_t *len) {
+  if (!g || !len) return fail(HH_INVALID_ARGUMENT, "null argument");
+  hh_graph *gm = const_cast<hh_graph *>(g);  // link reads use the graph's atomics
+  Out w;
+  w.bytes(kMagic, 8);
+  w.u32(kVersion);
+  const uint64_t e = g->entry.load(std::memory_order_acquire);
+  w.u8((uint8_t)g->metric);
+  w.u8(g->p.selection ? 1 : 0);
+  w.u8(g->p.selection ? (uint8_t)((g->p.extend_candidates ? 1 : 0) | (g->p.keep_pruned ? 2 : 0)) : 0);
+  w.u8(g->quantized ? 1 : 0);
+  w.u32(g->dim);
+  w.u32(g->p.m);
+  w.u32(g->p.m0);
+  w.u32(g->p.ef_construction);
+  w.f64(g->p.level_multiplier);
+  w.u64(g->p.seed);
+  w.u32((uint32_t)g->n());
+  w.u32(e == kNoEntry ? 0xFFFFFFFFu : (uint32_t)e);
+  w.bytes(g->keys.data(), g->keys.size() * 8);
+  w.bytes(g->levels.data(), g->levels.size());
+  w.bytes(g->deleted.data(), g->deleted.size());
+  w.bytes(g->vectors.data(), g->vectors.size() * 4);
+  std::vector<uint32_t> nb;
+  for (uint32_t id = 0; id < (uint32_t)g->n(); id++)
+    for (int layer = 0; layer <= (int)g->levels[id]; layer++) {
+      gm->read_into(id, layer, nb);
+      w.u32((uint32_t)nb.size());
+      w.bytes(nb.data(), nb.size() * 4);
+    }
+  if (g->quantized) {
+    w.bytes(g->sq_offset.data(), g->sq_offset.size() * 4);
+    float s = g->sq_scale;
+    w.bytes(&s, 4);
+  }
+  const uint32_t crc = hh_crc32(w.b.data(), w.b.size());
+  w.u32(crc);
+  *len = w.b.size();
+  if (!buf) return HH_OK;
+  if (cap < w.b.size()) return fail(HH_INVALID_ARGUMENT, "buffer too small for the index file");
+  std::memcpy(buf, w.b.data(), w.b.size());
+  return HH_OK;
+}
+
+// HnswIndex::from_bytes / from_bytes_with_limit (src/hnsw/mod.rs:666-700,
+// io.rs:218-540).  max_memory HH_DEFAULT_MEMORY_LIMIT selects the crate's
+// default: 64 times the input size, at least 64 MiB.
+int hh_graph_from_bytes(const uint8_t *bytes, uint64_t len, uint64_t max_memory, hh_graph **out,
+                        hh_format_info *info) {
+  if (!out || (!bytes && len)) return fail(HH_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (info) std::memset(info, 0, sizeof *info);
+  // ---- envelope (io.rs:244-273)
+  if (len < 12) {
+    const size_t pre = len < 8 ? (size_t)len : 8;
+    if (std::memcmp(bytes, kMagic, pre) == 0) return truncated(info);
+    return format_error(info, HH_FORMAT_BAD_MAGIC, "", "missing HNSWIDX magic bytes");
+  }
+  if (std::memcmp(bytes, kMagic, 8) != 0)
+    return format_error(info, HH_FORMAT_BAD_MAGIC, "", "missing HNSWIDX magic bytes");
+  uint32_t version;
+  std::memcpy(&version, bytes + 8, 4);
+  if (version != kVersion) {
+    if (info) info->version = version;
+    return format_error(info, HH_FORMAT_UNSUPPORTED_VERSION, "",
+                        "unsupported format version " + std::to_string(version));
+  }
+  if (len < kHeaderLen + kTrailerLen) return truncated(info);
+  const size_t body = (size_t)len - kTrailerLen;
+  uint32_t stored;
+  std::memcpy(&stored, bytes + body, 4);
+  const uint32_t computed = hh_crc32(bytes, body);
+  if (stored != computed) {
+    if (info) {
+      info->stored = stored;
+      info->computed = computed;
+    }
+    return format_error(info, HH_FORMAT_CHECKSUM_MISMATCH, "", "checksum mismatch");
+  }
+  In r{bytes, body, 12};
+  // ---- header (io.rs:287-358)
+  uint8_t metric, sel, flags, quant;
+  r.get(&metric);
+  r.get(&sel);
+  r.get(&flags);
+  r.get(&quant);
+  if (metric > 2) return invalid(info, "metric");
+  if (!((sel == 0 && flags == 0) || (sel == 1 && flags <= 3))) return invalid(info, "neighbor selection");
+  if (quant > 1) return invalid(info, "quantized flag");
+  uint32_t dim, m, m0, efc, n, entry_id;
+  double lm;
+  uint64_t seed;
+  r.get(&dim);
+  if (dim == 0 || dim > 65536) return invalid(info, "dimension");
+  r.get(&m);
+  r.get(&m0);
+  r.get(&efc);
+  r.get(&lm);
+  r.get(&seed);
+  if (m < 2 || m > 512 || m0 < m || m0 > 2048 || efc < 1 || efc > 65536 ||
+      !(std::isfinite(lm) && lm > 0.0 && lm <= 10.0))
+    return invalid(info, "parameters");
+  r.get(&n);
+  if ((uint64_t)n > kMaxNodes) return invalid(info, "node count");
+  r.get(&entry_id);
+  // ---- columns, located before anything is parsed (io.rs:372-402)
+  const uint8_t *kcol, *lcol, *fcol, *vcol;
+  if (!r.take((size_t)n * 8, &kcol) || !r.take(n, &lcol) || !r.take(n, &fcol) ||
+      !r.take((size_t)n * 4 * dim, &vcol))
+    return truncated(info);
+  uint64_t level_sum = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    if (lcol[i] > kMaxLevel) return invalid(info, "node level");
+    level_sum += lcol[i];
+  }
+  for (uint32_t i = 0; i < n; i++)
+    if (fcol[i] > 1) return invalid(info, "tombstone flag");
+  if ((double)level_sum > 2.0 * ((double)n / std::expm1(1.0 / lm)) + kLevelSumSlack)
+    return invalid(info, "implausible level distribution");
+  // ---- adjacency walked without allocating (io.rs:407-420)
+  {
+    In s = r;
+    for (uint32_t i = 0; i < n; i++)
+      for (int layer = 0; layer <= (int)lcol[i]; layer++) {
+        uint32_t cnt;
+        if (!s.get(&cnt)) return truncated(info);
+        if (cnt > (layer == 0 ? m0 : m)) return invalid(info, "neighbor count exceeds capacity");
+        const uint8_t *skip;
+        if (!s.take((size_t)cnt * 4, &skip)) return truncated(info);
+      }
+  }
+  // ---- memory bound, before any graph storage (io.rs:425-450)
+  const uint64_t limit = max_memory == HH_DEFAULT_MEMORY_LIMIT
+                             ? std::max<uint64_t>(len * kMemoryFactor, kMemoryFloor)
+                             : max_memory;
+  const uint64_t required = graph_heap_bytes(n, dim, m, m0, level_sum, quant == 1);
+  if (required > limit) {
+    if (info) {
+      info->required = required;
+      info->limit = limit;
+    }
+    return fail(HH_MEMORY_LIMIT_EXCEEDED,
+                "loading the index needs " + std::to_string(required) +
+                    " bytes of memory, more than the limit of " + std::to_string(limit));
+  }
+  // ---- vectors: the checks insert applies (io.rs:453-475)
+  for (uint32_t i = 0; i < n; i++) {
+    const uint8_t *v = vcol + (size_t)i * 4 * dim;
+    double norm_sq = 0.0;
+    for (uint32_t e = 0; e < dim; e++) {
+      float x;
+      std::memcpy(&x, v + 4 * e, 4);
+      if (!std::isfinite(x)) return invalid(info, "vector out of range");
+      norm_sq += (double)x * (double)x;
+    }
+    if (norm_sq > kMaxSquaredNorm) return invalid(info, "vector out of range");
+    if (metric == HH_COSINE && std::fabs(norm_sq - 1.0) > kUnitNormTolerance)
+      return invalid(info, "cosine vector is not unit length");
+  }
+  hh_params p{};
+  p.m = m;
+  p.m0 = m0;
+  p.ef_construction = efc;
+  p.selection = sel;
+  p.extend_candidates = flags & 1;
+  p.keep_pruned = (flags >> 1) & 1;
+  p.level_multiplier = lm;
+  p.seed = seed;
+  std::unique_ptr<hh_graph> g(new (std::nothrow) hh_graph());
+  if (!g) return fail(HH_OUT_OF_MEMORY, "allocation failed");
+  g->dim = dim;
+  g->metric = metric;
+  g->p = p;
+  g->keys.resize(n);
+  if (n) std::memcpy(g->keys.data(), kcol, (size_t)n * 8);
+  g->levels.assign(lcol, lcol + n);
+  g->deleted.assign(fcol, fcol + n);
+  g->vectors.resize((size_t)n * dim);
+  if (n) std::memcpy(g->vectors.data(), vcol, (size_t)n * 4 * dim);
+  g->layer0.assign((size_t)n * (m0 + 1), 0);
+  g->upper_off.resize(n);
+  uint64_t off = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    g->upper_off[i] = off;
+    off += (uint64_t)lcol[i] * (m + 1);
+  }
+  g->upper.assign(off, 0);
+  g->locks.reset(new std::mutex[std::max<uint32_t>(n, 1)]);
+  g->locks_cap = n;
+  // ---- links (io.rs:495-520)
+  std::vector<uint32_t> ids, sorted;
+  for (uint32_t id = 0; id < n; id++)
+    for (int layer = 0; layer <= (int)lcol[id]; layer++) {
+      uint32_t cnt;
+      r.get(&cnt);
+      ids.resize(cnt);
+      for (uint32_t j = 0; j < cnt; j++) {
+        uint32_t nb;
+        r.get(&nb);
+        if (!(nb < n && nb != id && (int)lcol[nb] >= layer)) return invalid(info, "neighbor id");
+        ids[j] = nb;
+      }
+      sorted = ids;
+      std::sort(sorted.begin(), sorted.end());
+      if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        return invalid(info, "duplicate neighbor id");
+      g->set(id, layer, ids.data(), cnt);
+    }
+  // ---- quantizer (io.rs:229-236; ScalarQuantizer::from_parts,
+  // src/quantization.rs:58-62)
+  if (quant) {
+    const uint8_t *oc;
+    float scale;
+    if (!r.take((size_t)dim * 4, &oc)) return truncated(info);
+    g->sq_offset.resize(dim);
+    std::memcpy(g->sq_offset.data(), oc, (size_t)dim * 4);
+    for (float o : g->sq_offset)
+      if (!std::isfinite(o)) return invalid(info, "quantizer offset");
+    if (!r.get(&scale)) return truncated(info);
+    if (!(std::isfinite(scale) && scale > 0.0f)) return invalid(info, "quantizer");
+    g->sq_scale = scale;
+    g->codes.resize((size_t)n * dim);
+    for (uint32_t i = 0; i < n; i++)
+      sq_encode(g->sq_offset.data(), g->sq_scale, dim, g->vec(i), g->codes.data() + (size_t)i * dim);
+    g->quantized = true;
+  }
+  if (r.pos != r.len)
+    return format_error(info, HH_FORMAT_TRAILING_BYTES, "", "trailing bytes after the index");
+  // ---- key map and entry point (io.rs:522-540)
+  g->key_to_id.reserve(n);
+  for (uint32_t id = 0; id < n; id++)
+    if (!g->deleted[id] && !g->key_to_id.emplace(g->keys[id], id).second)
+      return invalid(info, "duplicate live key");
+  if (n == 0) {
+    if (entry_id != 0xFFFFFFFFu) return invalid(info, "entry point of an empty index");
+    g->entry.store(kNoEntry);
+  } else {
+    uint8_t top = 0;
+    for (uint32_t i = 0; i < n; i++) top = std::max(top, lcol[i]);
+    if (entry_id >= n || lcol[entry_id] != top) return invalid(info, "entry point");
+    g->entry.store(((uint64_t)top << 32) | entry_id, std::memory_order_release);
+  }
+  *out = g.release();
+  return HH_OK;
 }
 
 }  // extern "C"

@@ -49,6 +49,19 @@ class ZeroVector(Error): status = 5
 class DuplicateKey(Error): status = 6
 class CapacityExceeded(Error): status = 7
 class QuantizationDisabled(Error): status = 8
+class FormatError(Error):
+    """Error::Format (src/error.rs:56-75): `.kind` is one of "BadMagic",
+    "UnsupportedVersion", "Truncated", "ChecksumMismatch", "Invalid",
+    "TrailingBytes"; `.what`, `.version`, `.stored`, `.computed` as the
+    variant carries."""
+    status = 9
+
+
+class MemoryLimitExceeded(Error):
+    """Error::MemoryLimitExceeded: `.required` and `.limit` in bytes."""
+    status = 10
+
+
 class DeviceError(Error): status = 100
 class OutOfMemory(Error): status = 101
 class Unsupported(Error): status = 102
@@ -57,7 +70,8 @@ class InvalidArgument(Error): status = 103
 
 _BY_STATUS = {c.status: c for c in (DimensionMismatch, InvalidDimension, InvalidParameter,
                                      NonFiniteVector, ZeroVector, DuplicateKey,
-                                     CapacityExceeded, QuantizationDisabled, DeviceError,
+                                     CapacityExceeded, QuantizationDisabled, FormatError,
+                                     MemoryLimitExceeded, DeviceError,
                                      OutOfMemory, Unsupported, InvalidArgument)}
 
 
@@ -231,6 +245,66 @@ class HnswIndex:
                 lib.hh_graph_free(self._g)
         except Exception:
             pass
+
+    # -- persistence (src/hnsw/mod.rs:611-700, src/hnsw/io.rs)
+    def to_bytes(self):
+        """The crate's version-1 index file, byte for byte."""
+        n = ctypes.c_uint64()
+        _check(lib.hh_graph_to_bytes(self._g, None, 0, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        _check(lib.hh_graph_to_bytes(self._g, buf.ctypes.data, n.value, ctypes.byref(n)))
+        return buf.tobytes()
+
+    def write_to(self, f):
+        data = self.to_bytes()
+        if isinstance(f, (str, os.PathLike)):
+            with open(f, "wb") as fh:
+                fh.write(data)
+        else:
+            f.write(data)
+
+    @classmethod
+    def from_bytes(cls, data, max_memory=None, device=0):
+        """Loads an index file, validating it as untrusted input; raises
+        FormatError or MemoryLimitExceeded (max_memory None = the crate's
+        default limit, 64x the input size and at least 64 MiB)."""
+        buf = np.frombuffer(bytes(data), np.uint8)
+        h = ctypes.c_void_p()
+        info = L.FormatInfo()
+        limit = (1 << 64) - 1 if max_memory is None else min(max(int(max_memory), 0), (1 << 64) - 2)
+        st = lib.hh_graph_from_bytes(buf.ctypes.data if buf.size else None, buf.size, limit,
+                                     ctypes.byref(h), ctypes.byref(info))
+        if st:
+            kinds = {1: "BadMagic", 2: "UnsupportedVersion", 3: "Truncated",
+                     4: "ChecksumMismatch", 5: "Invalid", 6: "TrailingBytes"}
+            _check(st, kind=kinds.get(info.kind), what=info.what.decode(), version=info.version,
+                   stored=info.stored, computed=info.computed, required=info.required,
+                   limit=info.limit)
+        self = cls.__new__(cls)
+        dim, metric = ctypes.c_uint32(), ctypes.c_uint32()
+        p = L.Params()
+        _check(lib.hh_graph_info(h, ctypes.byref(dim), ctypes.byref(metric), ctypes.byref(p)))
+        params = HnswParams(p.m)
+        params._p = p
+        self._g = h
+        self._dim = dim.value
+        self._metric = Metric(metric.value)
+        self._params = params
+        self._device = device
+        self._idx = None
+        self._dirty = True
+        self._bf16 = False
+        return self
+
+    @classmethod
+    def read_from(cls, f, **kw):
+        if isinstance(f, (str, os.PathLike)):
+            with open(f, "rb") as fh:
+                return cls.from_bytes(fh.read(), **kw)
+        return cls.from_bytes(f.read(), **kw)
+
+    def memory_usage(self):
+        return int(lib.hh_graph_memory_usage(self._g))
 
     # -- properties (src/hnsw/mod.rs:210-264)
     def dim(self): return self._dim

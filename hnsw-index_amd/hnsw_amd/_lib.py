@@ -36,6 +36,11 @@ class GraphDesc(ctypes.Structure):
                 ("codes", P), ("sq_offset", P), ("sq_scale", ctypes.c_float), ("live", U64)]
 
 
+class FormatInfo(ctypes.Structure):
+    _fields_ = [("kind", U32), ("version", U32), ("stored", U32), ("computed", U32),
+                ("required", U64), ("limit", U64), ("what", ctypes.c_char * 48)]
+
+
 class SearchStats(ctypes.Structure):
     _fields_ = [("queries", U64), ("n_dist", U64), ("n_upper", U64), ("n_expand", U64),
                 ("n_rerank", U64), ("n_overflow", U64), ("n_visited_spill", U64),
@@ -99,6 +104,11 @@ _sig("hh_search_batch", INT, P, P, U64, U32, U32, U32, P, P, P, P, ctypes.POINTE
 _sig("hh_brute_upload", INT, U32, U32, P, P, U64, INT, ctypes.POINTER(P), ctypes.POINTER(U64))
 _sig("hh_brute_free", None, P)
 _sig("hh_brute_len", U64, P)
+_sig("hh_graph_info", I32, P, ctypes.POINTER(U32), ctypes.POINTER(U32), P)
+_sig("hh_graph_memory_usage", U64, P)
+_sig("hh_graph_to_bytes", I32, P, P, U64, ctypes.POINTER(U64))
+_sig("hh_graph_from_bytes", I32, P, U64, U64, ctypes.POINTER(P), P)
+_sig("hh_crc32", U32, P, U64)
 _sig("hh_index_set_launch_shape", I32, P, U32, U32, INT)
 _sig("hh_device_alloc", I32, INT, U64, ctypes.POINTER(P))
 _sig("hh_device_free", I32, P)

@@ -38,6 +38,8 @@ typedef enum hh_status {
   HH_DUPLICATE_KEY = 6,          /* Error::DuplicateKey */
   HH_CAPACITY_EXCEEDED = 7,      /* Error::CapacityExceeded */
   HH_QUANTIZATION_DISABLED = 8,  /* Error::QuantizationDisabled */
+  HH_FORMAT_ERROR = 9,           /* Error::Format: see hh_format_info */
+  HH_MEMORY_LIMIT_EXCEEDED = 10, /* Error::MemoryLimitExceeded */
   HH_DEVICE_ERROR = 100,         /* a HIP call failed */
   HH_OUT_OF_MEMORY = 101,        /* device allocation refused */
   HH_UNSUPPORTED = 102,          /* shape outside what the kernels handle */
@@ -117,6 +119,46 @@ int hh_graph_enable_quantization(hh_graph *g);
 int hh_graph_disable_quantization(hh_graph *g);
 /* HnswIndex::len / node_count / stats (src/hnsw/mod.rs:236,705,562-583). */
 uint64_t hh_graph_len(const hh_graph *g);
+/* Dimension, metric tag and parameters of a graph (HnswIndex::dim / metric
+ * / params, src/hnsw/mod.rs:210-230). */
+int hh_graph_info(const hh_graph *g, uint32_t *dim, uint32_t *metric, hh_params *params);
+/* Heap bytes the graph holds in this library's layout (the counterpart of
+ * HnswIndex::memory_usage, src/hnsw/mod.rs:590-609; the figure a file load
+ * compares with its memory limit). */
+uint64_t hh_graph_memory_usage(const hh_graph *g);
+
+/* The crate's index file, version 1 (src/hnsw/io.rs:1-28).  FormatError
+ * kinds (src/error.rs:56-75): */
+typedef enum hh_format_kind {
+  HH_FORMAT_BAD_MAGIC = 1,
+  HH_FORMAT_UNSUPPORTED_VERSION = 2,
+  HH_FORMAT_TRUNCATED = 3,
+  HH_FORMAT_CHECKSUM_MISMATCH = 4,
+  HH_FORMAT_INVALID = 5,        /* what names the field (the crate's Invalid(&str)) */
+  HH_FORMAT_TRAILING_BYTES = 6,
+} hh_format_kind;
+typedef struct hh_format_info {
+  uint32_t kind;                /* hh_format_kind for HH_FORMAT_ERROR */
+  uint32_t version;             /* UnsupportedVersion */
+  uint32_t stored, computed;    /* ChecksumMismatch */
+  uint64_t required, limit;     /* HH_MEMORY_LIMIT_EXCEEDED */
+  char what[48];                /* Invalid */
+} hh_format_info;
+/* HnswIndex::to_bytes / write_to (src/hnsw/mod.rs:611-646): *len receives
+ * the file size; the bytes are written when buf holds at least that many. */
+int hh_graph_to_bytes(const hh_graph *g, uint8_t *buf, uint64_t cap, uint64_t *len);
+/* HnswIndex::from_bytes / from_bytes_with_limit (src/hnsw/mod.rs:666-700):
+ * validates the untrusted input as the crate does, in the crate's order,
+ * and never allocates graph storage before the file is known to contain the
+ * whole graph and its heap fits max_memory (HH_DEFAULT_MEMORY_LIMIT = the
+ * crate's default: 64 times the input size, at least 64 MiB).  info
+ * (optional) receives the error details. */
+#define HH_DEFAULT_MEMORY_LIMIT (~(uint64_t)0)
+int hh_graph_from_bytes(const uint8_t *bytes, uint64_t len, uint64_t max_memory, hh_graph **out,
+                        hh_format_info *info);
+/* CRC-32 (IEEE 802.3, reflected 0xEDB88320) of the index files
+ * (src/crc32.rs). */
+uint32_t hh_crc32(const uint8_t *data, uint64_t len);
 uint64_t hh_graph_node_count(const hh_graph *g);
 
 /* ------------------------------------------------------------------ */

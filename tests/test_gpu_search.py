@@ -322,3 +322,18 @@ def test_launch_shapes_do_not_change_results():
             assert H.lib.hh_index_set_launch_shape(h, ef, 0, 0) == 0
         assert H.lib.hh_index_set_launch_shape(h, 64, 2048, 3) != 0
         assert H.lib.hh_index_set_launch_shape(h, 300, 512, 1) != 0
+
+
+def test_loaded_index_file_searches_like_the_original():
+    """An index written in the crate's file format and loaded back searches
+    identically on the GPU (f32, int8 + re-rank, with tombstones)."""
+    b, q = O.dataset_gaussian(3000, 50, 32, 44)
+    idx = build(b, threads=4)
+    for key in (5, 77, 1234):
+        idx.remove(key)
+    idx.enable_quantization()
+    loaded = H.HnswIndex.from_bytes(idx.to_bytes())
+    for prec in (H.Precision.F32, H.Precision.INT8):
+        ek, ed, ec = idx.search_arrays(q, 10, 48, precision=prec)
+        gk, gd, gc = loaded.search_arrays(q, 10, 48, precision=prec)
+        assert np.array_equal(ec, gc) and np.array_equal(ek, gk) and same_bits(ed, gd), prec
