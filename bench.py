@@ -37,6 +37,36 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def measure_quantizer(H, idx, reps):
+    """The int8 quantizer trained and applied on the device over the
+    resident rows (reps timed runs of hh_index_quantize, wall time including
+    the parameter read-back), installed in the host graph, then the host
+    library's quantizer on the same vectors on one core; both must agree bit
+    for bit."""
+    h = idx.device_index()
+    dim, n = idx.dim(), idx.node_count()
+    off = np.zeros(dim, np.float32)
+    sc = ctypes.c_float()
+    dev_s = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        H._check(H.lib.hh_index_quantize(h, off.ctypes.data, ctypes.byref(sc)))
+        dev_s.append(time.perf_counter() - t0)
+    idx.enable_quantization(on_device=True)
+    t0 = time.perf_counter()
+    H._check(H.lib.hh_graph_enable_quantization(idx._g))
+    host_s = time.perf_counter() - t0
+    ga = idx.graph_arrays()
+    same = bool(np.array_equal(ga["sq_offset"].view(np.uint32), off.view(np.uint32))
+                and np.float32(ga["sq_scale"]) == np.float32(sc.value))
+    if not same:
+        raise SystemExit("device quantizer differs from the host quantizer")
+    log(f"quantizer: device {min(dev_s) * 1e3:.2f} ms, host {host_s * 1e3:.0f} ms "
+        f"({n} x {dim}), parameters identical")
+    return {"device_s": round(min(dev_s), 5), "host_s": round(host_s, 4), "rows": n, "dim": dim,
+            "identical_parameters": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +84,9 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="search measurement only: no precision variants, serving or CPU "
                          "baseline (for profiler runs)")
+    ap.add_argument("--quantizer-only", action="store_true",
+                    help="time only the device quantizer on the uploaded index (for "
+                         "profiler runs) and print its JSON line")
     ap.add_argument("--launch-shape", default="",
                     help="CAP,WPB: visited-set capacity and waves per block instead of the "
                          "autotuner's choice")
@@ -95,6 +128,14 @@ def main():
     out_c, status = Buffer(nq * 4), Buffer(nq * 4)
     log("uploading the index")
     h = idx.device_index()
+    if args.quantizer_only:
+        q = measure_quantizer(H, idx, max(args.steps, 1))
+        print(json.dumps({"metric": "int8 quantizer train+encode, rows/s", "unit": "rows/s",
+                          "value": round(n / q["device_s"]), "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "f64 sums, f32 codes",
+                          "data": "synthetic", "config": {"workload": f"{n} x {dim} rows"},
+                          "quantizer": q}), flush=True)
+        return
 
     # ground truth: exact GPU brute force, checked against the FP64 oracle
     log("ground truth: GPU brute force over all rows")
@@ -222,13 +263,11 @@ def main():
         sec = timed(ef, prec, max(3, args.steps // 2))
         variants[name] = {"recall": round(r, 4), "qps": round(nq / sec)}
         log(f"{name} ef={ef}: recall={r:.4f} qps={nq / sec:,.0f}")
-    log("int8: training the quantizer and re-uploading")
-    idx.enable_quantization()
-    h = idx.device_index()
+    quant = measure_quantizer(H, idx, 3)
     run(ef, H.Precision.INT8)
     r = recall()
     sec = timed(ef, H.Precision.INT8, max(3, args.steps // 2))
-    variants["int8"] = {"recall": round(r, 4), "qps": round(nq / sec)}
+    variants["int8"] = {"recall": round(r, 4), "qps": round(nq / sec), "quantizer": quant}
     log(f"int8 ef={ef}: recall={r:.4f} qps={nq / sec:,.0f}")
 
     # serving loop: Poisson arrivals replayed by the native load generator

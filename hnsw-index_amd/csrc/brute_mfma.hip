@@ -294,8 +294,9 @@ __global__ void query_norms_kernel(const float *qprep, uint64_t nq, uint32_t str
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) GM(p)[i] = v;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    GM(p)[i] = v;
 }
 
 template __global__ void brute_mfma_kernel<8>(const MfmaArgs);

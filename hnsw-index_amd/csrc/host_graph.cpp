@@ -695,7 +695,9 @@ int graph_view(const hh_graph *g, hh_graph_desc *d) {
   d->upper_off = g->upper_off.data();
   d->upper = g->upper.data();
   d->deleted = g->deleted.data();
-  d->codes = g->quantized ? g->codes.data() : nullptr;
+  // a quantized empty graph still reports its quantizer (non-null codes)
+  static const int8_t kNoCodes = 0;
+  d->codes = g->quantized ? (g->codes.empty() ? &kNoCodes : g->codes.data()) : nullptr;
   d->sq_offset = g->quantized ? g->sq_offset.data() : nullptr;
   d->sq_scale = g->sq_scale;
   d->live = g->key_to_id.size();
@@ -792,6 +794,26 @@ int hh_graph_enable_quantization(hh_graph *g) {
   for (size_t i = 0; i < g->n(); i++)
     sq_encode(g->sq_offset.data(), g->sq_scale, g->dim, g->vec((uint32_t)i),
               g->codes.data() + i * g->dim);
+  g->quantized = true;
+  return HH_OK;
+}
+
+int hh_graph_set_quantizer(hh_graph *g, const float *offset, float scale, const int8_t *codes) {
+  if (!g || (!offset && g->dim)) return fail(HH_INVALID_ARGUMENT, "null argument");
+  // the validity rule of ScalarQuantizer::from_parts (src/quantization.rs:58-62)
+  bool valid = std::isfinite(scale) && scale > 0.0f;
+  for (uint32_t d = 0; d < g->dim && valid; d++) valid = std::isfinite(offset[d]);
+  if (!valid) return fail(HH_INVALID_PARAMETER, "invalid quantizer parameters");
+  g->sq_offset.assign(offset, offset + g->dim);
+  g->sq_scale = scale;
+  g->codes.resize(g->vectors.size());
+  if (codes) {
+    if (!g->codes.empty()) std::memcpy(g->codes.data(), codes, g->codes.size());
+  } else {
+    for (size_t i = 0; i < g->n(); i++)
+      sq_encode(g->sq_offset.data(), g->sq_scale, g->dim, g->vec((uint32_t)i),
+                g->codes.data() + i * g->dim);
+  }
   g->quantized = true;
   return HH_OK;
 }

@@ -1,0 +1,178 @@
+// quantize.hip — the crate's scalar quantizer trained and applied on the
+// GPU over the resident rows (ScalarQuantizer::train / encode_into,
+// src/quantization.rs:32-81), bit-identical to the host quantizer:
+//
+//   offset[d] = (f32)(Σ_r (f64)x[r][d] / rows)   L2 only, the sum taken over
+//                                                the rows in order, in f64;
+//                                                0 for the other metrics
+//   max_abs   = max over every x of |x − offset| (f32)
+//   scale     = max_abs / 127, or 1 when max_abs is 0 or not finite
+//   code      = clamp(round_half_away((x − offset) · (1/scale)), ±127)
+//
+// The per-dimension sum keeps the crate's sequential f64 rounding: one lane
+// per dimension adds its column in row order.  That chain of n dependent
+// adds is the kernel's critical path, so the rows are spread over many
+// blocks by position (16 row positions = 64 bytes of each row per block) and
+// all 256 threads of a block stage the next tile of rows while one wave
+// adds the current one.  The maximum is order-free (a grid-wide max over
+// float bit patterns, which order like the non-negative values) and
+// encoding is elementwise; both stream the rows once.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "quantize_kernels.h"
+
+using namespace hhd;
+
+namespace hhk {
+
+// Dimension stored at row position p: the inverse of the accumulator-major
+// row layout (value 32t + j lives at (t/4)*128 + j*tw + t%4 with
+// tw = min(4, T − 4(t/4)); from 32T on the values are in natural order and
+// positions past the dimension are zero padding).
+__device__ __forceinline__ uint32_t dim_at(uint32_t p, uint32_t T) {
+  if (p >= 32u * T) return p;
+  const uint32_t b = p >> 7, w = p & 127;
+  const uint32_t tw = T - 4 * b < 4 ? T - 4 * b : 4;
+  const uint32_t j = w / tw, t = 4 * b + w % tw;
+  return 32 * t + j;
+}
+
+constexpr int kPos = 16;     // row positions per block of sq_mean_kernel
+constexpr int kTile = 512;   // rows per staged tile (32 KiB)
+constexpr int kLoads = kTile * (kPos / 4) / 256;  // float4 loads per thread per tile
+
+// Load k of one tile of rows r0..: 4 threads per row, kTile rows in
+// kLoads loads per thread; rows past the end read row n-1 (the sum stops at
+// n; n >= 1).
+__device__ __forceinline__ float4 sq_tile_load(const SqArgs &a, uint32_t p0, uint32_t t,
+                                               uint64_t r0, int k) {
+  const uint32_t i = t + 256 * k, r = i >> 2, c = i & 3;
+  const uint64_t row = r0 + r < a.n ? r0 + r : a.n - 1;
+  return *(gptr<float4>)(a.rows + row * a.stride + p0 + 4 * c);
+}
+
+__global__ void __launch_bounds__(256) sq_mean_kernel(SqArgs a) {
+  static_assert(kLoads == 8, "one register per load below");
+  __shared__ float tile[kTile * kPos];
+  float4 *tile4 = reinterpret_cast<float4 *>(tile);
+  const uint32_t p0 = blockIdx.x * kPos;
+  const uint32_t t = threadIdx.x;
+  float4 x0, x1, x2, x3, x4, x5, x6, x7;
+#define HH_SQ_FETCH(R0)                  \
+  x0 = sq_tile_load(a, p0, t, R0, 0);    \
+  x1 = sq_tile_load(a, p0, t, R0, 1);    \
+  x2 = sq_tile_load(a, p0, t, R0, 2);    \
+  x3 = sq_tile_load(a, p0, t, R0, 3);    \
+  x4 = sq_tile_load(a, p0, t, R0, 4);    \
+  x5 = sq_tile_load(a, p0, t, R0, 5);    \
+  x6 = sq_tile_load(a, p0, t, R0, 6);    \
+  x7 = sq_tile_load(a, p0, t, R0, 7);
+  double sum = 0.0;
+  HH_SQ_FETCH(0)
+  for (uint64_t r0 = 0; r0 < a.n; r0 += kTile) {
+    tile4[t] = x0;
+    tile4[t + 256] = x1;
+    tile4[t + 512] = x2;
+    tile4[t + 768] = x3;
+    tile4[t + 1024] = x4;
+    tile4[t + 1280] = x5;
+    tile4[t + 1536] = x6;
+    tile4[t + 1792] = x7;
+    __syncthreads();
+    if (r0 + kTile < a.n) {
+      HH_SQ_FETCH(r0 + kTile)
+    }
+    if (t < kPos) {
+      const uint32_t nr = a.n - r0 < (uint64_t)kTile ? (uint32_t)(a.n - r0) : (uint32_t)kTile;
+      uint32_t r = 0;
+      // 32 reads in flight, then the 32 dependent adds of the chain
+      for (; r + 32 <= nr; r += 32) {
+        float v[32];
+#pragma unroll
+        for (int u = 0; u < 32; u++) v[u] = tile[(r + u) * kPos + t];
+#pragma unroll
+        for (int u = 0; u < 32; u++) sum = __dadd_rn(sum, (double)v[u]);
+      }
+      for (; r < nr; r++) sum = __dadd_rn(sum, (double)tile[r * kPos + t]);
+    }
+    __syncthreads();
+  }
+#undef HH_SQ_FETCH
+  if (t < kPos) {
+    const uint32_t p = p0 + t;
+    const uint32_t d = dim_at(p, a.T);
+    const float o = d < a.dim ? (float)(sum / (double)a.n) : 0.0f;
+    GM(a.offset_pos)[p] = o;
+    if (d < a.dim) GM(a.offset)[d] = o;
+  }
+}
+
+// Row r, position group c (4 positions) of a row-chunked flat loop: blocks
+// take chunks of kChunk rows so the index arithmetic stays in 32 bits.
+constexpr uint32_t kChunk = 1024;
+
+__global__ void __launch_bounds__(256) sq_max_kernel(SqArgs a) {
+  __shared__ uint32_t red[4];
+  const uint32_t s4 = a.stride / 4;
+  uint32_t m = 0;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * kChunk; r0 < a.n; r0 += (uint64_t)gridDim.x * kChunk) {
+    const uint32_t nr = a.n - r0 < kChunk ? (uint32_t)(a.n - r0) : kChunk;
+    const uint32_t units = nr * s4;
+    gptr<float4> base = (gptr<float4>)(a.rows + r0 * a.stride);
+    for (uint32_t u = threadIdx.x; u < units; u += 256) {
+      const uint32_t c = u % s4;
+      const float4 x = base[u];
+      const float4 o = *(gptr<float4>)(a.offset_pos + 4 * c);
+      // padding positions hold 0 with offset 0: |0 - 0| leaves the max alone
+      m = max(m, __float_as_uint(fabsf(__fsub_rn(x.x, o.x))));
+      m = max(m, __float_as_uint(fabsf(__fsub_rn(x.y, o.y))));
+      m = max(m, __float_as_uint(fabsf(__fsub_rn(x.z, o.z))));
+      m = max(m, __float_as_uint(fabsf(__fsub_rn(x.w, o.w))));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t b = max(max(red[0], red[1]), max(red[2], red[3]));
+    atomicMax(a.max_bits, b);
+  }
+}
+
+__device__ __forceinline__ float sq_scale_of(uint32_t max_bits) {
+  const float max_abs = __uint_as_float(max_bits);
+  return (max_abs > 0.0f && isfinite(max_abs)) ? max_abs / 127.0f : 1.0f;
+}
+
+__device__ __forceinline__ int8_t sq_code(float x, float o, float inv) {
+  float c = roundf(__fmul_rn(__fsub_rn(x, o), inv));
+  c = fminf(127.0f, fmaxf(-127.0f, c));
+  return (int8_t)c;
+}
+
+// codes[r*cstride + d] for d < dim, natural order; the caller zero-fills
+// the buffer first so the padding to cstride stays zero.
+__global__ void __launch_bounds__(256) sq_encode_kernel(SqArgs a) {
+  const float inv = 1.0f / sq_scale_of(*G(a.max_bits));
+  const uint32_t s4 = a.stride / 4;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * kChunk; r0 < a.n; r0 += (uint64_t)gridDim.x * kChunk) {
+    const uint32_t nr = a.n - r0 < kChunk ? (uint32_t)(a.n - r0) : kChunk;
+    const uint32_t units = nr * s4;
+    gptr<float4> base = (gptr<float4>)(a.rows + r0 * a.stride);
+    for (uint32_t u = threadIdx.x; u < units; u += 256) {
+      const uint32_t r = u / s4, c = u - r * s4;
+      const float4 x = base[u];
+      const float4 o = *(gptr<float4>)(a.offset_pos + 4 * c);
+      const float xs[4] = {x.x, x.y, x.z, x.w}, os[4] = {o.x, o.y, o.z, o.w};
+      int8_t *row = a.codes + (r0 + r) * a.cstride;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t d = dim_at(4 * c + k, a.T);
+        if (d < a.dim) GM(row)[d] = sq_code(xs[k], os[k], inv);
+      }
+    }
+  }
+}
+
+}  // namespace hhk

@@ -17,6 +17,7 @@
 
 #include "brute_kernels.h"
 #include "hh_internal.h"
+#include "quantize_kernels.h"
 #include "search_kernels.h"
 
 using hh::fail;
@@ -799,6 +800,18 @@ int hh_index_upload(const hh_graph_desc *d, int device, hh_index **out) {
   if (st) return st;
   HH_HIP(hipStreamCreateWithFlags(&idx->stream, hipStreamNonBlocking));
   if (d->n == 0) {
+    // an empty index may still hold a quantizer (searches then succeed
+    // with no results instead of reporting quantization disabled)
+    if (d->codes) {
+      if (!d->sq_offset) return fail(HH_INVALID_ARGUMENT, "codes without quantizer offsets");
+      idx->cstride = (d->dim + 127) / 128 * 128;
+      st = dalloc(&idx->codes, idx->cstride);
+      if (!st) st = dalloc(&idx->sq_offset, d->dim);
+      if (st) return st;
+      HH_HIP(hipMemset(idx->codes, 0, idx->cstride));
+      HH_HIP(hipMemcpy(idx->sq_offset, d->sq_offset, d->dim * 4, hipMemcpyHostToDevice));
+      idx->sq_scale = d->sq_scale;
+    }
     *out = idx.release();
     return HH_OK;
   }
@@ -883,6 +896,87 @@ int hh_index_enable_bf16(hh_index *idx) {
   to_bf16_kernel<<<grid_for(count), 256>>>(idx->rows, idx->bf16, count);
   HH_HIP(hipGetLastError());
   HH_HIP(hipDeviceSynchronize());
+  return HH_OK;
+}
+
+int hh_index_quantize(hh_index *idx, float *offset, float *scale) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  HH_HIP(hipSetDevice(idx->device));
+  int st = drain(idx);
+  if (st) return st;
+  const uint32_t dim = idx->dim, stride = idx->lay.stride;
+  const uint32_t cstride = (dim + 127) / 128 * 128;
+  const uint64_t n = idx->n;
+  hipStream_t s = idx->stream;
+  // new buffers first: the current quantizer stays whole if one fails
+  int8_t *codes = nullptr;
+  float *off = nullptr, *off_pos = nullptr;
+  uint32_t *mx = nullptr;
+  st = dalloc(&codes, std::max<uint64_t>(n * cstride, cstride));
+  if (!st) st = dalloc(&off, dim);
+  if (!st) st = dalloc(&off_pos, stride);
+  if (!st) st = dalloc(&mx, 1);
+  auto release = [&] {
+    dfree(codes);
+    dfree(off);
+    dfree(off_pos);
+    dfree(mx);
+  };
+  if (st) {
+    release();
+    return st;
+  }
+  hhk::SqArgs a{idx->rows, n, stride, dim, idx->lay.T, cstride, off, off_pos, mx, codes};
+  const uint64_t code_words = std::max<uint64_t>(n * cstride, cstride) / 4;
+  hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(code_words)), dim3(256), 0, s,
+                     reinterpret_cast<uint32_t *>(codes), code_words, 0u);
+  hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(1), dim3(256), 0, s, mx, (uint64_t)1, 0u);
+  if (idx->metric == HH_L2 && n > 0) {
+    hipLaunchKernelGGL(hhk::sq_mean_kernel, dim3(stride / 16), dim3(256), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(dim)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t *>(off), (uint64_t)dim, 0u);
+    hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(stride)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t *>(off_pos), (uint64_t)stride, 0u);
+  }
+  if (n > 0) {
+    const int grid = (int)std::min<uint64_t>((n + 1023) / 1024, 4096);
+    hipLaunchKernelGGL(hhk::sq_max_kernel, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(hhk::sq_encode_kernel, dim3(grid), dim3(256), 0, s, a);
+  }
+  uint32_t bits = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&bits, mx, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && offset && dim) e = hipMemcpy(offset, off, (size_t)dim * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    release();
+    return fail(HH_DEVICE_ERROR, std::string("device quantizer: ") + hipGetErrorString(e));
+  }
+  // the scale rule of the training kernels, evaluated the same way here
+  float max_abs;
+  std::memcpy(&max_abs, &bits, 4);
+  const float sc = (max_abs > 0.0f && std::isfinite(max_abs)) ? max_abs / 127.0f : 1.0f;
+  dfree(idx->codes);
+  dfree(idx->sq_offset);
+  idx->codes = codes;
+  idx->sq_offset = off;
+  idx->cstride = cstride;
+  idx->sq_scale = sc;
+  codes = nullptr;
+  off = nullptr;
+  release();
+  if (scale) *scale = sc;
+  return HH_OK;
+}
+
+int hh_index_download_codes(hh_index *idx, int8_t *codes) {
+  if (!idx || !codes) return fail(HH_INVALID_ARGUMENT, "null argument");
+  if (!idx->codes) return fail(HH_UNSUPPORTED, "no quantizer codes resident");
+  if (idx->n == 0) return HH_OK;
+  HH_HIP(hipSetDevice(idx->device));
+  HH_HIP(hipMemcpy2D(codes, idx->dim, idx->codes, idx->cstride, idx->dim, idx->n,
+                     hipMemcpyDeviceToHost));
   return HH_OK;
 }
 

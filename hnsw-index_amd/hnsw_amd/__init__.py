@@ -352,9 +352,33 @@ class HnswIndex:
             _check(lib.hh_index_update_deleted(self._idx, _ptr(v["deleted"]), len(self)))
         return bool(r.value)
 
-    def enable_quantization(self):
-        _check(lib.hh_graph_enable_quantization(self._g))
-        self._dirty = True
+    def enable_quantization(self, on_device=False):
+        """HnswIndex::enable_quantization (src/hnsw/mod.rs:544-547).  With
+        on_device the quantizer is trained and the codes encoded by the GPU
+        over the resident rows (bit-identical parameters and codes), and the
+        host graph receives both, so later inserts encode with it as the
+        crate does."""
+        if not on_device:
+            _check(lib.hh_graph_enable_quantization(self._g))
+            self._dirty = True
+            return
+        idx = self.device_index()
+        off = np.zeros(self._dim, np.float32)
+        sc = ctypes.c_float()
+        _check(lib.hh_index_quantize(idx, _ptr(off), ctypes.byref(sc)))
+        codes = np.zeros((self.node_count(), self._dim), np.int8)
+        if codes.size:
+            _check(lib.hh_index_download_codes(idx, _ptr(codes)))
+        _check(lib.hh_graph_set_quantizer(self._g, _ptr(off), sc.value,
+                                          _ptr(codes) if codes.size else None))
+        # the resident index already holds exactly these codes
+
+    def quantizer(self):
+        """(offset, scale) of the trained quantizer, or None."""
+        v = self.graph_arrays()
+        if v["codes"] is None:
+            return None
+        return v["sq_offset"].copy(), float(v["sq_scale"])
 
     def disable_quantization(self):
         _check(lib.hh_graph_disable_quantization(self._g))
@@ -376,7 +400,7 @@ class HnswIndex:
 
         def arr(ptr, ctype, count):
             if not ptr or count == 0:
-                return np.zeros(0, dtype=np.ctypeslib.as_array(ctypes.pointer(ctype())).dtype)
+                return np.zeros(0, dtype=np.dtype(ctype))
             return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), (count,))
 
         levels = arr(d.levels, ctypes.c_uint8, n)

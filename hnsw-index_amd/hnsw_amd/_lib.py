@@ -71,6 +71,7 @@ _sig("hh_graph_free", None, P)
 _sig("hh_graph_insert_batch", INT, P, P, P, U64, INT, ctypes.POINTER(U64))
 _sig("hh_graph_remove", INT, P, U64, ctypes.POINTER(INT))
 _sig("hh_graph_enable_quantization", INT, P)
+_sig("hh_graph_set_quantizer", INT, P, P, ctypes.c_float, P)
 _sig("hh_graph_disable_quantization", INT, P)
 _sig("hh_graph_len", U64, P)
 _sig("hh_graph_node_count", U64, P)
@@ -79,6 +80,8 @@ _sig("hh_index_upload", INT, ctypes.POINTER(GraphDesc), INT, ctypes.POINTER(P))
 _sig("hh_index_free", None, P)
 _sig("hh_index_update_deleted", INT, P, P, U64)
 _sig("hh_index_enable_bf16", INT, P)
+_sig("hh_index_quantize", INT, P, P, ctypes.POINTER(ctypes.c_float))
+_sig("hh_index_download_codes", INT, P, P)
 _sig("hh_index_stats", INT, P, ctypes.POINTER(SearchStats))
 _sig("hh_index_reset_stats", INT, P)
 _sig("hh_index_set_timing", INT, P, INT)

@@ -117,6 +117,12 @@ int hh_graph_remove(hh_graph *g, uint64_t key, int *removed);
 /* HnswIndex::enable_quantization (src/hnsw/mod.rs:544-547). */
 int hh_graph_enable_quantization(hh_graph *g);
 int hh_graph_disable_quantization(hh_graph *g);
+/* Installs a trained quantizer (ScalarQuantizer::from_parts,
+ * src/quantization.rs:58-62: offset finite, scale finite and > 0, else
+ * HH_INVALID_PARAMETER) with the n*dim codes of the stored vectors, or
+ * encodes them here when codes is NULL.  Used with hh_index_quantize to
+ * give the host graph the quantizer trained on the device. */
+int hh_graph_set_quantizer(hh_graph *g, const float *offset, float scale, const int8_t *codes);
 /* HnswIndex::len / node_count / stats (src/hnsw/mod.rs:236,705,562-583). */
 uint64_t hh_graph_len(const hh_graph *g);
 /* Dimension, metric tag and parameters of a graph (HnswIndex::dim / metric
@@ -202,6 +208,18 @@ int hh_index_update_deleted(hh_index *idx, const uint8_t *deleted, uint64_t live
 /* Converts the resident f32 rows to bf16 (round to nearest even) on the
  * device, enabling HH_PREC_BF16. */
 int hh_index_enable_bf16(hh_index *idx);
+/* Trains the crate's scalar quantizer on the resident rows and encodes
+ * them on the device, enabling HH_PREC_INT8 (ScalarQuantizer::train and
+ * encode_into, src/quantization.rs:32-81, as HnswIndex::enable_quantization
+ * applies them, src/hnsw/mod.rs:544-547).  The trained parameters are
+ * bit-identical to hh_graph_enable_quantization's on the same vectors:
+ * per-dimension f64 sums in row order (L2), the f32 maximum of |x - offset|
+ * and the same rounding of every code.  offset (dim floats) and scale
+ * receive the parameters when not NULL.  Replaces a resident quantizer. */
+int hh_index_quantize(hh_index *idx, float *offset, float *scale);
+/* Copies the resident codes to host memory as n*dim bytes (the crate's
+ * code layout), for hh_graph_set_quantizer.  HH_UNSUPPORTED without codes. */
+int hh_index_download_codes(hh_index *idx, int8_t *codes);
 
 /* Search statistics accumulated by the kernels since the last reset. */
 typedef struct hh_search_stats {
@@ -288,8 +306,8 @@ int hh_search_batch_spec(hh_index *idx, const float *queries, uint64_t nq,
  * hh_search_batch_spec_device; then k, precision and allow_bits are ignored
  * and ef is the batch's ef_max).  The plan owns its workspace; its launches are ordered with an
  * event, so launches of one plan never overlap.  A plan captures the index
- * as it is at creation: re-create it after hh_index_update_deleted or
- * hh_index_enable_bf16. */
+ * as it is at creation: re-create it after hh_index_update_deleted,
+ * hh_index_enable_bf16 or hh_index_quantize. */
 typedef struct hh_search_plan hh_search_plan;
 int hh_search_plan_create(hh_index *idx, const float *queries, uint64_t nq, uint32_t k,
                           uint32_t ef, uint32_t precision, const uint32_t *allow_bits,

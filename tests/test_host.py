@@ -83,6 +83,26 @@ def test_host_quantizer_matches_the_oracle_bit_for_bit():
         assert np.array_equal(v["codes"], O.sq_encode_rows(off, scale, v["vectors"]))
 
 
+def test_installed_quantizer_is_validated_and_encodes_like_the_oracle():
+    """ScalarQuantizer::from_parts validity (src/quantization.rs:58-62) and
+    encode_into of every stored vector when no codes are given."""
+    import ctypes
+    from hnsw_amd import _lib as L
+    b, _ = O.dataset_gaussian(300, 1, 20, 8)
+    idx = build(b, H.Metric.L2, threads=2)
+    off, scale = O.sq_train(int(H.Metric.L2), idx.graph_arrays()["vectors"])
+    g = idx._g
+    for bad_off, bad_scale in [(off, 0.0), (off, -1.0), (off, float("inf")), (off, float("nan")),
+                               (np.where(np.arange(20) == 3, np.inf, off).astype(np.float32), scale)]:
+        assert L.lib.hh_graph_set_quantizer(g, bad_off.ctypes.data, bad_scale, None) == \
+            H.InvalidParameter.status
+        assert not idx.is_quantized()
+    assert L.lib.hh_graph_set_quantizer(g, off.ctypes.data, float(scale), None) == 0
+    v = idx.graph_arrays()
+    assert np.array_equal(v["sq_offset"], off) and np.float32(v["sq_scale"]) == scale
+    assert np.array_equal(v["codes"], O.sq_encode_rows(off, scale, v["vectors"]))
+
+
 def _assert_invariants(v):
     """src/hnsw/tests.rs:28-53"""
     n, m, m0 = v["n"], v["m"], v["m0"]
