@@ -13,8 +13,9 @@
 // per dimension adds its column in row order.  That chain of n dependent
 // adds is the kernel's critical path, so the rows are spread over many
 // blocks by position (16 row positions = 64 bytes of each row per block) and
-// all 256 threads of a block stage the next tile of rows while one wave
-// adds the current one.  The maximum is order-free (a grid-wide max over
+// all 256 threads of a block load, widen to f64 and stage the next tile of
+// rows while 16 lanes add the current one (one LDS read and one add per
+// row on the chain).  The maximum is order-free (a grid-wide max over
 // float bit patterns, which order like the non-negative values) and
 // encoding is elementwise; both stream the rows once.
 #include <hip/hip_runtime.h>
@@ -39,12 +40,12 @@ __device__ __forceinline__ uint32_t dim_at(uint32_t p, uint32_t T) {
 }
 
 constexpr int kPos = 16;     // row positions per block of sq_mean_kernel
-constexpr int kTile = 512;   // rows per staged tile (32 KiB)
+constexpr int kTile = 256;   // rows per staged tile (as f64: 32 KiB, two buffers)
 constexpr int kLoads = kTile * (kPos / 4) / 256;  // float4 loads per thread per tile
 
 // Load k of one tile of rows r0..: 4 threads per row, kTile rows in
-// kLoads loads per thread; rows past the end read row n-1 (the sum stops at
-// n; n >= 1).
+// kLoads loads per thread; rows past the end read row n-1 (never summed:
+// the sum stops at n; n >= 1).
 __device__ __forceinline__ float4 sq_tile_load(const SqArgs &a, uint32_t p0, uint32_t t,
                                                uint64_t r0, int k) {
   const uint32_t i = t + 256 * k, r = i >> 2, c = i & 3;
@@ -52,53 +53,68 @@ __device__ __forceinline__ float4 sq_tile_load(const SqArgs &a, uint32_t p0, uin
   return *(gptr<float4>)(a.rows + row * a.stride + p0 + 4 * c);
 }
 
+// The loading threads also widen to f64, so the summing lanes issue one
+// LDS read and one add per row.
+__device__ __forceinline__ void sq_tile_store(double *buf, uint32_t t, int k, float4 x) {
+  const uint32_t i = t + 256 * k, r = i >> 2, c = i & 3;
+  double2 *dst = reinterpret_cast<double2 *>(buf + r * kPos + 4 * c);
+  dst[0] = make_double2((double)x.x, (double)x.y);
+  dst[1] = make_double2((double)x.z, (double)x.w);
+}
+
 __global__ void __launch_bounds__(256) sq_mean_kernel(SqArgs a) {
-  static_assert(kLoads == 8, "one register per load below");
-  __shared__ float tile[kTile * kPos];
-  float4 *tile4 = reinterpret_cast<float4 *>(tile);
+  static_assert(kLoads == 4, "one register per load below");
+  __shared__ double tile[2][kTile * kPos];
   const uint32_t p0 = blockIdx.x * kPos;
   const uint32_t t = threadIdx.x;
-  float4 x0, x1, x2, x3, x4, x5, x6, x7;
-#define HH_SQ_FETCH(R0)                  \
-  x0 = sq_tile_load(a, p0, t, R0, 0);    \
-  x1 = sq_tile_load(a, p0, t, R0, 1);    \
-  x2 = sq_tile_load(a, p0, t, R0, 2);    \
-  x3 = sq_tile_load(a, p0, t, R0, 3);    \
-  x4 = sq_tile_load(a, p0, t, R0, 4);    \
-  x5 = sq_tile_load(a, p0, t, R0, 5);    \
-  x6 = sq_tile_load(a, p0, t, R0, 6);    \
-  x7 = sq_tile_load(a, p0, t, R0, 7);
+  float4 x0, x1, x2, x3;
+#define HH_SQ_FETCH(R0)               \
+  x0 = sq_tile_load(a, p0, t, R0, 0); \
+  x1 = sq_tile_load(a, p0, t, R0, 1); \
+  x2 = sq_tile_load(a, p0, t, R0, 2); \
+  x3 = sq_tile_load(a, p0, t, R0, 3);
+#define HH_SQ_STORE(BUF)           \
+  sq_tile_store(BUF, t, 0, x0);    \
+  sq_tile_store(BUF, t, 1, x1);    \
+  sq_tile_store(BUF, t, 2, x2);    \
+  sq_tile_store(BUF, t, 3, x3);
   double sum = 0.0;
   HH_SQ_FETCH(0)
-  for (uint64_t r0 = 0; r0 < a.n; r0 += kTile) {
-    tile4[t] = x0;
-    tile4[t + 256] = x1;
-    tile4[t + 512] = x2;
-    tile4[t + 768] = x3;
-    tile4[t + 1024] = x4;
-    tile4[t + 1280] = x5;
-    tile4[t + 1536] = x6;
-    tile4[t + 1792] = x7;
-    __syncthreads();
+  HH_SQ_STORE(tile[0])
+  if ((uint64_t)kTile < a.n) {
+    HH_SQ_FETCH((uint64_t)kTile)
+  }
+  __syncthreads();
+  // Iteration j: lanes 0..15 add tile j (buffer j&1) while every thread
+  // stores tile j+1 (held in registers) into the other buffer and loads
+  // tile j+2.  The buffer written in iteration j was last read in
+  // iteration j-1, before the barrier that ends it.
+  uint32_t j = 0;
+  for (uint64_t r0 = 0; r0 < a.n; r0 += kTile, j++) {
     if (r0 + kTile < a.n) {
-      HH_SQ_FETCH(r0 + kTile)
+      HH_SQ_STORE(tile[(j + 1) & 1])
+      if (r0 + 2 * kTile < a.n) {
+        HH_SQ_FETCH(r0 + 2 * kTile)
+      }
     }
     if (t < kPos) {
+      const double *buf = tile[j & 1];
       const uint32_t nr = a.n - r0 < (uint64_t)kTile ? (uint32_t)(a.n - r0) : (uint32_t)kTile;
       uint32_t r = 0;
-      // 32 reads in flight, then the 32 dependent adds of the chain
-      for (; r + 32 <= nr; r += 32) {
-        float v[32];
+      // 16 reads in flight, then the 16 dependent adds of the chain
+      for (; r + 16 <= nr; r += 16) {
+        double v[16];
 #pragma unroll
-        for (int u = 0; u < 32; u++) v[u] = tile[(r + u) * kPos + t];
+        for (int u = 0; u < 16; u++) v[u] = buf[(r + u) * kPos + t];
 #pragma unroll
-        for (int u = 0; u < 32; u++) sum = __dadd_rn(sum, (double)v[u]);
+        for (int u = 0; u < 16; u++) sum = __dadd_rn(sum, v[u]);
       }
-      for (; r < nr; r++) sum = __dadd_rn(sum, (double)tile[r * kPos + t]);
+      for (; r < nr; r++) sum = __dadd_rn(sum, buf[r * kPos + t]);
     }
     __syncthreads();
   }
 #undef HH_SQ_FETCH
+#undef HH_SQ_STORE
   if (t < kPos) {
     const uint32_t p = p0 + t;
     const uint32_t d = dim_at(p, a.T);
