@@ -8,12 +8,14 @@
 // query indices from a counter, so uneven per-query work (ef, filters)
 // balances itself.
 //
-// Per wave: the candidate list C and the result list W are sorted register
-// lists (element i in lane i/S); the visited set is an LDS hash table that
-// moves to a per-wave HBM bitset if a query visits more nodes than the
-// table holds; neighbour rows are gathered four at a time, one per 16-lane
-// quarter, 512 contiguous bytes per row at D=128.  The adjacency row of the
-// next candidate is fetched while the current one is expanded.
+// Per wave: the candidate list C and the result list W are sorted arrays of
+// candidate words in LDS, each neighbour batch merged in one pass; C spills
+// its largest entries to a per-wave HBM pool; the visited set is an LDS hash
+// table that moves to a per-wave HBM bitset if a query visits more nodes
+// than the table holds; neighbour rows are gathered in rounds of 16, one row
+// per 16-lane quarter at a time, 512 contiguous bytes per row at D=128.  The
+// adjacency row of the next candidate is fetched while the current one is
+// expanded.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
