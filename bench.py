@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--quantizer-only", action="store_true",
                     help="time only the device quantizer on the uploaded index (for "
                          "profiler runs) and print its JSON line")
+    ap.add_argument("--one-pass", action="store_true",
+                    help="search each batch in one pass in query order instead of the "
+                         "two-pass longest-first order")
     ap.add_argument("--launch-shape", default="",
                     help="CAP,WPB: visited-set capacity and waves per block instead of the "
                          "autotuner's choice")
@@ -128,6 +131,8 @@ def main():
     out_c, status = Buffer(nq * 4), Buffer(nq * 4)
     log("uploading the index")
     h = idx.device_index()
+    if args.one_pass:
+        idx.set_query_order(False)
     if args.quantizer_only:
         q = measure_quantizer(H, idx, max(args.steps, 1))
         print(json.dumps({"metric": "int8 quantizer train+encode, rows/s", "unit": "rows/s",
@@ -247,8 +252,9 @@ def main():
     adj_bytes = st["n_expand"] * 32 * 4
     kern_ms = float(np.median(kms))
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
-    log(f"search kernel {kern_ms:.3f} ms/launch; {n_dist / nq:.0f} distance evaluations/query; "
-        f"{achieved:.0f} GB/s of gathered rows")
+    log(f"search kernel {kern_ms:.3f} ms/launch; {n_dist / nq:.0f} distance evaluations/query "
+        f"({st['n_upper'] / nq:.0f} in the upper layers), {st['n_expand'] / nq:.1f} layer-0 "
+        f"expansions/query; {achieved:.0f} GB/s of gathered rows")
 
     variants, serve, cpu = {}, None, None
     if args.quick:
@@ -387,6 +393,8 @@ def emit(args, v):
         "step_ms_p99": round(float(np.percentile(step_ms, 99)), 3),
         "search_kernel_ms": round(kern_ms, 4),
         "dist_evals_per_query": round(n_dist / nq, 1),
+        "upper_evals_per_query": round(v["st"]["n_upper"] / nq, 1),
+        "expansions_per_query": round(v["st"]["n_expand"] / nq, 1),
         "gathered_bytes_per_launch": alg_bytes,
         "adjacency_bytes_per_launch": adj_bytes,
         "sweep_f32": sweep,

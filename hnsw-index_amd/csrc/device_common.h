@@ -82,6 +82,11 @@ __device__ __forceinline__ int32_t tkey(float f) {
   int32_t i = __float_as_int(f);
   return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1);
 }
+// Bin of a query's descent distance for the longest-first query order:
+// the top 12 bits of the total-order key (an eighth of a binade).
+__device__ __forceinline__ uint32_t order_bin(float f) {
+  return ((uint32_t)tkey(f) ^ 0x80000000u) >> 20;
+}
 // Candidate order: (distance total order, node id).
 __device__ __forceinline__ bool cless(float da, uint32_t ia, float db, uint32_t ib) {
   int32_t a = tkey(da), b = tkey(db);

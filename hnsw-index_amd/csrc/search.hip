@@ -318,6 +318,14 @@ __device__ __forceinline__ bool admit(const SearchArgs &P, const uint32_t *allow
   return true;
 }
 
+// Phase-1 output of one query (all lanes store the same word; lane 0
+// counts the query in its order bin).
+__device__ __forceinline__ void seed_query(const SearchArgs &P, uint32_t qi, int lane,
+                                           uint32_t entry, float best) {
+  GM(P.seed)[qi] = ((uint64_t)__float_as_uint(best) << 32) | entry;
+  atomicAdd(P.hist + order_bin(best), lane == 0 ? 1u : 0u);
+}
+
 // EXACT: D == 32 * TMAX (branch-free row loads, see load_q).
 template <int TMAX, bool EXTRA, bool EXACT, int SW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? 4 : 2)))
@@ -344,13 +352,24 @@ hnsw_search_kernel(const SearchArgs P) {
   gmut<uint64_t> pool = GM(P.cpool) + (size_t)slot * P.cpool_cap;
   const bool has_pool = P.cpool != nullptr;
 
+  // Phase 1 (descent only, nearly uniform work per query) takes queries
+  // statically, without the shared counter.
+  uint32_t static_next = slot;
+  const uint32_t total_waves = gridDim.x * (blockDim.x >> 6);
   for (;;) {
     // Side effects in this loop run on every lane with wave-uniform values
     // (never under `lane == 0` branches): correlated lane-0 branches around
     // the back edge let the compiler run lanes a loop iteration apart, which
     // breaks the readlane broadcasts below.  Only lane 0 adds 1 here.
-    const uint32_t qi = rl(atomicAdd(P.counter, lane == 0 ? 1u : 0u), 0);
+    uint32_t qi;
+    if (P.phase == 1) {
+      qi = static_next;
+      static_next += total_waves;
+    } else {
+      qi = rl(atomicAdd(P.counter, lane == 0 ? 1u : 0u), 0);
+    }
     if (qi >= P.nq) break;
+    if (P.phase == 2) qi = rl(G(P.order)[qi], 0);
     uint32_t k = P.k, ef = P.ef, prec = P.precision, filt = 0xFFFFFFFFu;
     if (P.specs) {
       const hh_query_spec sp = G(P.specs)[qi];
@@ -364,6 +383,7 @@ hnsw_search_kernel(const SearchArgs P) {
     const int status = (int)rl((uint32_t)G(P.status)[qi], 0);
     if (status != 0 || k == 0 || P.live == 0 || P.top < 0) {
       GM(P.out_count)[qi] = 0;
+      if (P.phase == 1) seed_query(P, qi, lane, kInvalid, -INFINITY);
       continue;
     }
     if (ef < k) ef = k;
@@ -373,7 +393,7 @@ hnsw_search_kernel(const SearchArgs P) {
     const uint32_t ccap = ef <= 64 ? 128 : ef <= 128 ? 256 : ef <= 256 ? 512 : 1024;
     const uint32_t *allow = P.allow;
     if (P.specs) allow = filt == 0xFFFFFFFFu ? nullptr : P.filters + (size_t)filt * P.filter_words;
-    st.q++;
+    if (P.phase != 1) st.q++;
 
     // query, in the row layout, plus its int8 code words
     QLanes<TMAX, EXTRA> q;
@@ -393,61 +413,84 @@ hnsw_search_kernel(const SearchArgs P) {
         if (c < chunks) qc[c] = cq[c * 16 + i16];
     }
 
-    for (uint32_t t = lane; t < hcap; t += 64) table[t] = kInvalid;
-    wave_sync();
     int mode = 0;
     uint32_t vcount = 0;
     uint32_t slog_n = 0;  // nodes logged since the visited set left LDS
 
     // ---- entry point and greedy descent (search_graph, search.rs:146-161)
-    uint32_t cur = P.entry;
-    st_id[0] = cur;
-    wave_sync();
-    gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, 1, st_d);
-    wave_sync();
-    float best = rlf(st_d[0], 0);
-    st.up++;
-    for (int layer = P.top; layer >= 1; layer--) {
-      for (;;) {
-        const uint32_t from = cur;
-        gptr<uint32_t> blk = G(P.upper) + G(P.upper_off)[from] + (size_t)(layer - 1) * (P.m + 1);
-        uint32_t cnt = rl(blk[0], 0);
-        if (cnt > P.m) cnt = P.m;
-        for (uint32_t base = 0; base < cnt; base += 64) {
-          const int c = (int)(cnt - base < 64 ? cnt - base : 64);
-          const uint32_t nid = lane < c ? blk[1 + base + lane] : kInvalid;
-          if (lane < c) st_id[lane] = nid;
-          wave_sync();
-          gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, c, st_d);
-          wave_sync();
-          const float dv = lane < c ? st_d[lane] : 0.0f;
-          // first list position holding the IEEE-smallest distance
-          float md = dv;
-          int mi = lane < c ? lane : 64;
+    uint32_t cur;
+    float best;
+    if (P.phase == 2) {
+      const uint64_t sd = G(P.seed)[qi];
+      cur = rl((uint32_t)sd, 0);
+      best = __uint_as_float(rl((uint32_t)(sd >> 32), 0));
+    } else {
+      cur = P.entry;
+      // The upper-layer block offset of the current node is always known
+      // before its block is needed: the entry's is loaded with the entry's
+      // row, a neighbour's with the neighbours' rows.  So each greedy step
+      // costs two dependent round trips (block, rows).
+      uint64_t cur_off = P.top >= 1 ? (uint64_t)G(P.upper_off)[cur] : 0;
+      st_id[0] = cur;
+      wave_sync();
+      gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, 1, st_d);
+      wave_sync();
+      best = rlf(st_d[0], 0);
+      st.up++;
+      for (int layer = P.top; layer >= 1; layer--) {
+        for (;;) {
+          const uint32_t from = cur;
+          gptr<uint32_t> blk = G(P.upper) + cur_off + (size_t)(layer - 1) * (P.m + 1);
+          // the count and the first 64 slots in one round trip (slots past
+          // the count are inside the block, m+1 words)
+          const uint32_t cnt_word = blk[0];
+          const uint32_t slot0 = lane < (int)P.m ? blk[1 + lane] : kInvalid;
+          uint32_t cnt = rl(cnt_word, 0);
+          if (cnt > P.m) cnt = P.m;
+          for (uint32_t base = 0; base < cnt; base += 64) {
+            const int c = (int)(cnt - base < 64 ? cnt - base : 64);
+            const uint32_t nid = lane < c ? (base == 0 ? slot0 : blk[1 + base + lane]) : kInvalid;
+            const uint64_t noff = lane < c ? (uint64_t)G(P.upper_off)[nid] : 0;
+            if (lane < c) st_id[lane] = nid;
+            wave_sync();
+            gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, c, st_d);
+            wave_sync();
+            const float dv = lane < c ? st_d[lane] : 0.0f;
+            // first list position holding the IEEE-smallest distance
+            float md = dv;
+            int mi = lane < c ? lane : 64;
 #pragma unroll
-          for (int m = 1; m < 64; m <<= 1) {
-            const float od = __shfl_xor(md, m, 64);
-            const int oi = __shfl_xor(mi, m, 64);
-            const bool take = oi < 64 && (mi == 64 || od < md || (!(md < od) && oi < mi));
-            if (take) {
-              md = od;
-              mi = oi;
+            for (int m = 1; m < 64; m <<= 1) {
+              const float od = __shfl_xor(md, m, 64);
+              const int oi = __shfl_xor(mi, m, 64);
+              const bool take = oi < 64 && (mi == 64 || od < md || (!(md < od) && oi < mi));
+              if (take) {
+                md = od;
+                mi = oi;
+              }
             }
+            md = rlf(md, 0);
+            mi = (int)rl((uint32_t)mi, 0);
+            st.up += c;
+            if (md < best) {
+              best = md;
+              cur = rl(nid, mi);
+              cur_off = ((uint64_t)rl((uint32_t)(noff >> 32), mi) << 32) | rl((uint32_t)noff, mi);
+            }
+            wave_sync();
           }
-          md = rlf(md, 0);
-          mi = (int)rl((uint32_t)mi, 0);
-          st.up += c;
-          if (md < best) {
-            best = md;
-            cur = rl(nid, mi);
-          }
-          wave_sync();
+          if (cur == from) break;
         }
-        if (cur == from) break;
+      }
+      if (P.phase == 1) {
+        seed_query(P, qi, lane, cur, best);
+        continue;
       }
     }
 
     // ---- layer 0 (search_layer, search.rs:81-142)
+    for (uint32_t t = lane; t < hcap; t += 64) table[t] = kInvalid;
+    wave_sync();
     // C and W: sorted candidate words in LDS (C from `head`, W from 0).
     uint32_t head = 0, csize = 0, wsize = 0;
     float worst = 0.0f;
@@ -669,6 +712,49 @@ hnsw_search_kernel(const SearchArgs P) {
     atomicAdd(&P.stats[5], st.ovf);
     atomicAdd(&P.stats[6], st.spill);
     atomicAdd(&P.stats[7], st.cspill);
+  }
+}
+
+// Longest-first query order for the two-pass launch.  Bins are taken from
+// the largest descent distance down; queries within a bin in any order.
+__global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *seed, uint64_t nq,
+                                                             uint32_t *hist, uint32_t *order,
+                                                             uint32_t *counter) {
+  static_assert(kOrderBins == 4 * 1024, "four bins per thread");
+  __shared__ uint32_t off[kOrderBins];
+  __shared__ uint32_t part[1024];
+  const uint32_t t = threadIdx.x;
+  // thread t owns the descending-order bins 4t..4t+3 (bin kOrderBins-1-r)
+  uint32_t c[4], sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    c[i] = G(hist)[kOrderBins - 1 - (4 * t + i)];
+    sum += c[i];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (uint32_t o = 1; o < 1024; o <<= 1) {
+    const uint32_t v = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    off[kOrderBins - 1 - (4 * t + i)] = run;
+    run += c[i];
+  }
+  // every bin was read before the scan's first barrier: clear them, and the
+  // search kernel's query counter, for the next pass
+#pragma unroll
+  for (int i = 0; i < 4; i++) GM(hist)[4 * t + i] = 0u;
+  if (t == 0) GM(counter)[0] = 0u;
+  __syncthreads();
+  for (uint64_t i = t; i < nq; i += 1024) {
+    const float best = __uint_as_float((uint32_t)(G(seed)[i] >> 32));
+    const uint32_t pos = atomicAdd(&off[order_bin(best)], 1u);
+    if (pos < nq) GM(order)[pos] = (uint32_t)i;
   }
 }
 

@@ -431,6 +431,12 @@ class HnswIndex:
             self._bf16 = False
         return self._idx
 
+    def set_query_order(self, longest_first=True):
+        """Large batches (1-16 queries per resident wavefront) run in two
+        passes with the longest queries first (default), or in one pass in
+        query order; results are the same either way."""
+        _check(lib.hh_index_set_query_order(self.device_index(), int(bool(longest_first))))
+
     def enable_bf16(self):
         _check(lib.hh_index_enable_bf16(self.device_index()))
         self._bf16 = True

@@ -337,3 +337,44 @@ def test_loaded_index_file_searches_like_the_original():
         ek, ed, ec = idx.search_arrays(q, 10, 48, precision=prec)
         gk, gd, gc = loaded.search_arrays(q, 10, 48, precision=prec)
         assert np.array_equal(ec, gc) and np.array_equal(ek, gk) and same_bits(ed, gd), prec
+
+
+def test_longest_first_order_returns_what_one_pass_returns():
+    """Batches of 1-16 queries per resident wavefront at ef > 64 run in two passes
+    (upper-layer descent of every query, an order by descending descent
+    distance, then layer 0 in that order).  Results and counters must equal
+    the one-pass launch's, and the oracle's, including queries that end
+    early (k = 0, a NaN query) and mixed specs."""
+    b, q = O.dataset_gaussian(4000, 20000, 24, 31)
+    idx = build(b, H.Metric.L2, m=8, threads=8)
+    assert idx.graph_arrays()["top"] >= 1
+    runs = {}
+    for longest_first in (False, True):
+        print(f"case longest_first={longest_first}", flush=True)
+        idx.set_query_order(longest_first)
+        idx.search_stats(reset=True)
+        res = idx.search_arrays(q, 10, 96)
+        runs[longest_first] = res, idx.search_stats()
+    (k1, d1, c1), s1 = runs[False]
+    (k2, d2, c2), s2 = runs[True]
+    assert np.array_equal(c1, c2) and np.array_equal(k1, k2) and same_bits(d1, d2)
+    for f in ("queries", "n_dist", "n_upper", "n_expand", "n_overflow"):
+        assert s1[f] == s2[f], f
+    g = oracle_graph(idx)
+    ok, od, oc, cnt = g.search_batch(q[:400], 10, 96)
+    assert np.array_equal(k2[:400], ok) and same_bits(d2[:400], od)
+    # mixed specs with queries that end early, in both orders
+    nq = q.shape[0]
+    rng = np.random.default_rng(5)
+    ks = rng.choice([0, 1, 10], nq)
+    efs = rng.integers(10, 200, nq)
+    qq = q.copy()
+    qq[7, 3] = np.nan
+    out = {}
+    for longest_first in (False, True):
+        idx.set_query_order(longest_first)
+        out[longest_first] = idx.search_mixed(qq, ks, efs)
+    for a, b_ in zip(out[False], out[True]):
+        assert np.array_equal(a.view(np.uint8), b_.view(np.uint8))
+    assert out[True][3][7] == 4 and out[True][2][7] == 0
+    assert (out[True][2][ks == 0] == 0).all()
