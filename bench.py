@@ -234,6 +234,32 @@ def main():
     qps = nq * args.steps / total
     rec = recall()
 
+    # the same steps issued on two streams with no wait between them: one
+    # batch's tail (its last, longest queries) overlaps the next batch's start
+    stream2 = Stream()
+    outs2 = [Buffer(nq * k * 8), Buffer(nq * k * 4), Buffer(nq * 4), Buffer(nq * 4)]
+
+    def run2(i, ef_, precision):
+        if i % 2 == 0:
+            return run(ef_, precision)
+        st = H.lib.hh_search_batch_device(
+            h, q_dev.ptr, nq, k, ef_, int(precision), None,
+            outs2[0].ptr, outs2[1].ptr, outs2[2].ptr, outs2[3].ptr, stream2.handle)
+        if st:
+            raise RuntimeError(L.last_error())
+
+    for i in range(args.warmup):
+        run2(i, ef, H.Precision.F32)
+    synchronize()
+    t_pipe = time.perf_counter()
+    for i in range(args.steps):
+        run2(i, ef, H.Precision.F32)
+    synchronize()
+    pipe_qps = nq * args.steps / (time.perf_counter() - t_pipe)
+    same_pipe = bool(np.array_equal(outs2[0].download((nq, k), np.uint64),
+                                    out_k.download((nq, k), np.uint64)))
+    log(f"two streams: {pipe_qps:,.0f} QPS (results identical: {same_pipe})")
+
     # kernel time (HIP events on the launch stream) and algorithmic bytes
     H.lib.hh_index_set_timing(h, 1)
     kms = []
@@ -392,6 +418,7 @@ def emit(args, v):
         "step_ms_p50": round(float(np.percentile(step_ms, 50)), 3),
         "step_ms_p99": round(float(np.percentile(step_ms, 99)), 3),
         "search_kernel_ms": round(kern_ms, 4),
+        "two_stream_qps": round(v["pipe_qps"]),
         "dist_evals_per_query": round(n_dist / nq, 1),
         "upper_evals_per_query": round(v["st"]["n_upper"] / nq, 1),
         "expansions_per_query": round(v["st"]["n_expand"] / nq, 1),

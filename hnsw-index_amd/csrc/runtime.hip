@@ -87,6 +87,11 @@ constexpr uint32_t kCandPool = 8192;
 // table) and still clear word by word at its end; beyond, the whole bitset.
 constexpr uint32_t kSpillLog = 4096;
 
+// prepare_queries_kernel: four queries (one per wavefront) per block
+int prep_grid(uint64_t nq) {
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>((nq + 3) / 4, 16384));
+}
+
 int grid_for(uint64_t work) {
   uint64_t g = (work + 255) / 256;
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, 65536));
@@ -557,7 +562,7 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   pa.has_bf16 = idx->bf16 != nullptr;
   pa.status = c.status ? c.status : ws.status;
   pa.counter = ws.counter;
-  hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(grid_for(c.nq)), dim3(256), 0, stream, pa);
+  hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(prep_grid(c.nq)), dim3(256), 0, stream, pa);
   HH_HIP(hipGetLastError());
 
   hhk::SearchArgs a{};
@@ -1573,7 +1578,7 @@ int brute_launch(hh_brute *b, const float *queries, uint64_t nq, uint32_t k,
   pa.status = status ? status : ws.status;
   pa.counter = nullptr;
   pa.ef_cap = 512;
-  hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(grid_for(nq)), dim3(256), 0, stream, pa);
+  hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(prep_grid(nq)), dim3(256), 0, stream, pa);
   HH_HIP(hipGetLastError());
   hhk::BruteArgs a{};
   a.rows = b->rows;

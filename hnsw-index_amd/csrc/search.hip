@@ -760,23 +760,27 @@ __global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *see
 
 // Validation and preparation of every query (src/vector.rs:22-45,90-104),
 // spec checks (src/hnsw/mod.rs:468-473), int8 encoding of the query
-// (src/quantization.rs:73-81), then the row layout.  One thread per query;
-// the f64 norm is summed left to right as the crate does.
-__global__ void prepare_queries_kernel(const PrepArgs A) {
-  const uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (qi == 0 && A.counter) *A.counter = 0u;
-  if (qi >= A.nq) return;
+// (src/quantization.rs:73-81), then the row layout.  One wavefront per
+// query: lanes load the query coalesced and square it in f64 (exact for f32
+// inputs); the f64 norm is then summed left to right, as the crate does, by
+// a uniform chain over the lanes' squares.
+__device__ __forceinline__ void prepare_one(const PrepArgs &A, uint64_t qi, int lane) {
   gptr<float> v = G(A.queries) + qi * A.dim;
   int status = 0;
   double norm_sq = 0.0;
-  for (uint32_t e = 0; e < A.dim; e++) {
-    const float x = v[e];
-    if (!isfinite(x)) {
+  for (uint32_t base = 0; base < A.dim; base += 64) {
+    const uint32_t e = base + (uint32_t)lane;
+    const float x = e < A.dim ? v[e] : 0.0f;
+    if (__ballot(!isfinite(x))) {
       status = 4;
       break;
     }
-    const double xd = (double)x;
-    norm_sq = __dadd_rn(norm_sq, __dmul_rn(xd, xd));
+    const double sq = __dmul_rn((double)x, (double)x);
+    const int cnt = A.dim - base < 64 ? (int)(A.dim - base) : 64;
+    const int lo = __double2loint(sq), hi = __double2hiint(sq);
+    for (int l = 0; l < cnt; l++)
+      norm_sq = __dadd_rn(norm_sq, __hiloint2double(__builtin_amdgcn_readlane(hi, l),
+                                                    __builtin_amdgcn_readlane(lo, l)));
   }
   if (status == 0 && norm_sq > 3.4028234663852886e38 / 8.0) status = 4;
   if (status == 0 && A.metric == 2 && norm_sq == 0.0) status = 5;
@@ -799,25 +803,43 @@ __global__ void prepare_queries_kernel(const PrepArgs A) {
   gmut<float> out = GM(A.qprep) + qi * A.stride;
   gmut<int8_t> code = A.qcode ? GM(A.qcode) + qi * A.cstride : nullptr;
   const float sinv = 1.0f / A.sq_scale;
-  for (uint32_t e = 0; e < A.dim; e++) {
-    float x = v[e];
-    if (A.metric == 2) x = (float)__dmul_rn((double)x, inv);
+  const bool encode = code && prec == 1;
+  for (uint32_t e = (uint32_t)lane; e < A.stride; e += 64) {
+    float x = 0.0f;
     uint32_t pos = e;
-    if (e < 32u * A.T) {
-      const uint32_t t = e >> 5, jj = e & 31, b = t >> 2;
-      const uint32_t tw = A.T - 4 * b < 4 ? A.T - 4 * b : 4;
-      pos = b * 128 + jj * tw + (t & 3);
+    if (e < A.dim) {
+      x = v[e];
+      if (A.metric == 2) x = (float)__dmul_rn((double)x, inv);
+      if (e < 32u * A.T) {
+        const uint32_t t = e >> 5, jj = e & 31, b = t >> 2;
+        const uint32_t tw = A.T - 4 * b < 4 ? A.T - 4 * b : 4;
+        pos = b * 128 + jj * tw + (t & 3);
+      }
     }
     out[pos] = x;
-    if (code && prec == 1) {
-      float c = roundf(__fmul_rn(__fsub_rn(x, A.sq_offset[e]), sinv));
-      c = fminf(127.0f, fmaxf(-127.0f, c));
-      code[e] = (int8_t)c;
+  }
+  if (encode) {
+    for (uint32_t e = (uint32_t)lane; e < A.cstride; e += 64) {
+      int8_t cv = 0;
+      if (e < A.dim) {
+        float x = v[e];
+        if (A.metric == 2) x = (float)__dmul_rn((double)x, inv);
+        float c = roundf(__fmul_rn(__fsub_rn(x, A.sq_offset[e]), sinv));
+        c = fminf(127.0f, fmaxf(-127.0f, c));
+        cv = (int8_t)c;
+      }
+      code[e] = cv;
     }
   }
-  for (uint32_t e = A.dim; e < A.stride; e++) out[e] = 0.0f;
-  if (code && prec == 1)
-    for (uint32_t e = A.dim; e < A.cstride; e++) code[e] = 0;
+}
+
+// grid: any number of 256-thread blocks (four queries per block at a time)
+__global__ void __launch_bounds__(256) prepare_queries_kernel(const PrepArgs A) {
+  const int lane = (int)(threadIdx.x & 63);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && A.counter) GM(A.counter)[0] = 0u;
+  for (uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); qi < A.nq;
+       qi += (uint64_t)gridDim.x * 4)
+    prepare_one(A, qi, lane);
 }
 
 #define HH_SEARCH_INST(TM, EX, XA)                                                  \
