@@ -157,7 +157,6 @@ struct Workspace {
   // longest-first query order of the two-pass launch
   uint64_t *seed = nullptr;
   uint32_t *order = nullptr;
-  uint32_t *hist = nullptr;   // hhk::kOrderBins counters, zero between launches
   uint32_t *spill = nullptr;
   uint64_t *cpool = nullptr;  // per-wave candidate pools (kCandPool entries)
   uint32_t *slog = nullptr;   // per-wave spill logs (kSpillLog entries)
@@ -199,10 +198,8 @@ struct Workspace {
     HH_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     int st = dalloc(&counter, 4);
     if (!st) st = dalloc(&stats, 8);
-    if (!st) st = dalloc(&hist, hhk::kOrderBins);
     if (st) return st;
     HH_HIP(hipMemset(stats, 0, 8 * sizeof(unsigned long long)));
-    HH_HIP(hipMemset(hist, 0, hhk::kOrderBins * sizeof(uint32_t)));
     HH_HIP(hipMemset(counter, 0, 4 * sizeof(uint32_t)));
     return HH_OK;
   }
@@ -213,7 +210,6 @@ struct Workspace {
     dfree(status);
     dfree(counter);
     dfree(stats);
-    dfree(hist);
     dfree(seed);
     dfree(order);
     dfree(spill);
@@ -615,7 +611,6 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.hash_limit = cfg.hash_cap / 2 + cfg.hash_cap / 4;
   a.lds_words = cfg.lds_words;
   a.seed = ws.seed;
-  a.hist = ws.hist;
   a.order = ws.order;
   void *args[] = {&a};
   if (before_search) HH_HIP(hipEventRecord(before_search, stream));
@@ -634,7 +629,7 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
     a.phase = 1;
     HH_HIP(hipLaunchKernel(cfg.kern, grid, block, args, lds, stream));
     hipLaunchKernelGGL(hhk::order_queries_kernel, dim3(1), dim3(1024), 0, stream, ws.seed, c.nq,
-                       ws.hist, ws.order, ws.counter);
+                       ws.order, ws.counter);
     HH_HIP(hipGetLastError());
     a.phase = 2;
   }

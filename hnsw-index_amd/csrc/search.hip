@@ -318,12 +318,10 @@ __device__ __forceinline__ bool admit(const SearchArgs &P, const uint32_t *allow
   return true;
 }
 
-// Phase-1 output of one query (all lanes store the same word; lane 0
-// counts the query in its order bin).
-__device__ __forceinline__ void seed_query(const SearchArgs &P, uint32_t qi, int lane,
-                                           uint32_t entry, float best) {
+// Phase-1 output of one query (all lanes store the same word).
+__device__ __forceinline__ void seed_query(const SearchArgs &P, uint32_t qi, uint32_t entry,
+                                           float best) {
   GM(P.seed)[qi] = ((uint64_t)__float_as_uint(best) << 32) | entry;
-  atomicAdd(P.hist + order_bin(best), lane == 0 ? 1u : 0u);
 }
 
 // EXACT: D == 32 * TMAX (branch-free row loads, see load_q).
@@ -383,7 +381,7 @@ hnsw_search_kernel(const SearchArgs P) {
     const int status = (int)rl((uint32_t)G(P.status)[qi], 0);
     if (status != 0 || k == 0 || P.live == 0 || P.top < 0) {
       GM(P.out_count)[qi] = 0;
-      if (P.phase == 1) seed_query(P, qi, lane, kInvalid, -INFINITY);
+      if (P.phase == 1) seed_query(P, qi, kInvalid, -INFINITY);
       continue;
     }
     if (ef < k) ef = k;
@@ -483,7 +481,7 @@ hnsw_search_kernel(const SearchArgs P) {
         }
       }
       if (P.phase == 1) {
-        seed_query(P, qi, lane, cur, best);
+        seed_query(P, qi, cur, best);
         continue;
       }
     }
@@ -715,20 +713,27 @@ hnsw_search_kernel(const SearchArgs P) {
   }
 }
 
-// Longest-first query order for the two-pass launch.  Bins are taken from
-// the largest descent distance down; queries within a bin in any order.
+// Longest-first query order for the two-pass launch: a histogram of the
+// seeds' order bins in LDS, bins taken from the largest descent distance
+// down (queries within a bin in any order); then resets the search kernel's
+// query counter for the second pass.
 __global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *seed, uint64_t nq,
-                                                             uint32_t *hist, uint32_t *order,
-                                                             uint32_t *counter) {
+                                                             uint32_t *order, uint32_t *counter) {
   static_assert(kOrderBins == 4 * 1024, "four bins per thread");
   __shared__ uint32_t off[kOrderBins];
   __shared__ uint32_t part[1024];
   const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 4; i++) off[4 * t + i] = 0u;
+  __syncthreads();
+  for (uint64_t i = t; i < nq; i += 1024)
+    atomicAdd(&off[order_bin(__uint_as_float((uint32_t)(G(seed)[i] >> 32)))], 1u);
+  __syncthreads();
   // thread t owns the descending-order bins 4t..4t+3 (bin kOrderBins-1-r)
   uint32_t c[4], sum = 0;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    c[i] = G(hist)[kOrderBins - 1 - (4 * t + i)];
+    c[i] = off[kOrderBins - 1 - (4 * t + i)];
     sum += c[i];
   }
   part[t] = sum;
@@ -745,10 +750,6 @@ __global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *see
     off[kOrderBins - 1 - (4 * t + i)] = run;
     run += c[i];
   }
-  // every bin was read before the scan's first barrier: clear them, and the
-  // search kernel's query counter, for the next pass
-#pragma unroll
-  for (int i = 0; i < 4; i++) GM(hist)[4 * t + i] = 0u;
   if (t == 0) GM(counter)[0] = 0u;
   __syncthreads();
   for (uint64_t i = t; i < nq; i += 1024) {

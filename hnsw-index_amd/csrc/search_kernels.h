@@ -52,12 +52,11 @@ struct SearchArgs {
   unsigned long long *stats;    // 8 counters
   // Two-pass launches (phase 1 then 2) take long queries first: phase 0
   // runs each query whole; phase 1 only descends the upper layers, writing
-  // seed[qi] = (best distance bits << 32) | layer-0 entry and counting the
-  // distance's order_bin in hist; phase 2 runs layer 0 from seed[], taking
-  // queries in the order of `order` (largest descent distance first).
+  // seed[qi] = (best distance bits << 32) | layer-0 entry; phase 2 runs
+  // layer 0 from seed[], taking queries in the order of `order` (largest
+  // descent distance first).
   uint32_t phase;
   uint64_t *seed;
-  uint32_t *hist;               // kOrderBins counters, zero between launches
   const uint32_t *order;
   uint32_t hash_cap, hash_limit, lds_words;
 };
@@ -85,10 +84,10 @@ constexpr uint32_t kOrderBins = 4096;
 template <int TMAX, bool EXTRA, bool EXACT, int SW>
 __global__ void hnsw_search_kernel(const SearchArgs P);
 // One block of 1024 threads: order[] = the nq queries by descending
-// order_bin of their seed distance; then clears hist and the search
-// kernel's query counter for the next pass.
-__global__ void order_queries_kernel(const uint64_t *seed, uint64_t nq, uint32_t *hist,
-                                     uint32_t *order, uint32_t *counter);
+// order_bin of their seed distance; resets the search kernel's query
+// counter for the next pass.
+__global__ void order_queries_kernel(const uint64_t *seed, uint64_t nq, uint32_t *order,
+                                     uint32_t *counter);
 __global__ void prepare_queries_kernel(const PrepArgs A);
 
 }  // namespace hhk
