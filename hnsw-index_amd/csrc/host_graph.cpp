@@ -383,6 +383,9 @@ struct hh_graph {
   std::vector<float> vectors;
   std::vector<uint64_t> keys;
   std::vector<uint8_t> levels, deleted;
+  // nodes whose link lists changed since hh_graph_take_dirty last cleared
+  // them (set by any thread that writes a block)
+  std::vector<uint8_t> dirty;
   // Links (src/hnsw/graph.rs:26-33): layer 0 = m0+1 words per node; upper
   // blocks of level*(m+1) words per node, located by upper_off.  Words are
   // read with relaxed atomics and counts with acquire, as the crate does.
@@ -421,7 +424,9 @@ struct hh_graph {
     for_each(id, layer, [&](uint32_t x) { out.push_back(x); });
   }
   // Links::set / try_push (src/hnsw/graph.rs:112-135)
+  void mark_dirty(uint32_t id) { __atomic_store_n(&dirty[id], (uint8_t)1, __ATOMIC_RELAXED); }
   void set(uint32_t id, int layer, const uint32_t *ids, size_t cnt) {
+    mark_dirty(id);
     uint32_t *b = block(id, layer);
     for (size_t i = 0; i < cnt; i++) __atomic_store_n(&b[1 + i], ids[i], __ATOMIC_RELAXED);
     __atomic_store_n(&b[0], (uint32_t)cnt, __ATOMIC_RELEASE);
@@ -430,6 +435,7 @@ struct hh_graph {
     uint32_t *b = block(id, layer);
     uint32_t c = __atomic_load_n(&b[0], __ATOMIC_RELAXED);
     if (c >= capacity(layer)) return false;
+    mark_dirty(id);
     __atomic_store_n(&b[1 + c], nb, __ATOMIC_RELAXED);
     __atomic_store_n(&b[0], c + 1, __ATOMIC_RELEASE);
     return true;
@@ -642,6 +648,7 @@ struct hh_graph {
       keys.push_back(ks[i]);
       levels.push_back(lv);
       deleted.push_back(0);
+      dirty.push_back(0);
       key_to_id[ks[i]] = id;
     }
     return first;
@@ -771,6 +778,21 @@ int hh_graph_insert_batch(hh_graph *g, const uint64_t *keys, const float *vector
     return fail(HH_CAPACITY_EXCEEDED, "index capacity exceeded");
   uint32_t first = g->append(keys, prepared.data(), n);
   g->link_parallel(first, threads);
+  return HH_OK;
+}
+
+int hh_graph_take_dirty(hh_graph *g, uint32_t *ids, uint64_t cap, uint64_t *count) {
+  if (!g || !count) return fail(HH_INVALID_ARGUMENT, "null argument");
+  uint64_t c = 0, put = 0;
+  for (size_t i = 0; i < g->dirty.size(); i++) {
+    if (!g->dirty[i]) continue;
+    if (ids && put < cap) {
+      ids[put++] = (uint32_t)i;
+      g->dirty[i] = 0;
+    }
+    c++;
+  }
+  *count = c;
   return HH_OK;
 }
 
@@ -1166,6 +1188,7 @@ int hh_graph_from_bytes(const uint8_t *bytes, uint64_t len, uint64_t max_memory,
   if (n) std::memcpy(g->keys.data(), kcol, (size_t)n * 8);
   g->levels.assign(lcol, lcol + n);
   g->deleted.assign(fcol, fcol + n);
+  g->dirty.assign(n, 0);
   g->vectors.resize((size_t)n * dim);
   if (n) std::memcpy(g->vectors.data(), vcol, (size_t)n * 4 * dim);
   g->layer0.assign((size_t)n * (m0 + 1), 0);
@@ -1217,6 +1240,8 @@ int hh_graph_from_bytes(const uint8_t *bytes, uint64_t len, uint64_t max_memory,
   }
   if (r.pos != r.len)
     return format_error(info, HH_FORMAT_TRAILING_BYTES, "", "trailing bytes after the index");
+  // a freshly loaded graph has no pending link changes
+  std::fill(g->dirty.begin(), g->dirty.end(), (uint8_t)0);
   // ---- key map and entry point (io.rs:522-540)
   g->key_to_id.reserve(n);
   for (uint32_t id = 0; id < n; id++)

@@ -97,18 +97,13 @@ int grid_for(uint64_t work) {
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, 65536));
 }
 
-int upload_rows(const float *host_rows, uint64_t n, uint32_t dim, const hh::RowLayout &lay,
-                float **out) {
-  *out = nullptr;
+// Host rows (natural order) -> n device rows at `rows` in the row layout.
+int upload_rows_into(const float *host_rows, uint64_t n, uint32_t dim, const hh::RowLayout &lay,
+                     float *rows) {
   if (n == 0) return HH_OK;
-  float *tmp = nullptr, *rows = nullptr;
+  float *tmp = nullptr;
   int st = dalloc(&tmp, n * dim);
   if (st) return st;
-  st = dalloc(&rows, n * lay.stride);
-  if (st) {
-    dfree(tmp);
-    return st;
-  }
   hipError_t e = hipMemcpy(tmp, host_rows, n * dim * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(rows, 0, n * lay.stride * sizeof(float));
   if (e == hipSuccess) {
@@ -117,12 +112,53 @@ int upload_rows(const float *host_rows, uint64_t n, uint32_t dim, const hh::RowL
   }
   if (e == hipSuccess) e = hipDeviceSynchronize();
   dfree(tmp);
-  if (e != hipSuccess) {
-    dfree(rows);
+  if (e != hipSuccess)
     return fail(HH_DEVICE_ERROR, std::string("row upload: ") + hipGetErrorString(e));
+  return HH_OK;
+}
+
+int upload_rows(const float *host_rows, uint64_t n, uint32_t dim, const hh::RowLayout &lay,
+                float **out) {
+  *out = nullptr;
+  if (n == 0) return HH_OK;
+  float *rows = nullptr;
+  int st = dalloc(&rows, n * lay.stride);
+  if (!st) st = upload_rows_into(host_rows, n, dim, lay, rows);
+  if (st) {
+    dfree(rows);
+    return st;
   }
   *out = rows;
   return HH_OK;
+}
+
+// Grows a device array holding `used` elements to `cap` elements, keeping
+// its contents.
+template <class T>
+int grow(T **p, uint64_t used, uint64_t cap) {
+  T *q = nullptr;
+  int st = dalloc(&q, std::max<uint64_t>(cap, 1));
+  if (st) return st;
+  if (used && *p) {
+    const hipError_t e = hipMemcpy(q, *p, used * sizeof(T), hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) {
+      dfree(q);
+      return fail(HH_DEVICE_ERROR, std::string("grow: ") + hipGetErrorString(e));
+    }
+  }
+  dfree(*p);
+  *p = q;
+  return HH_OK;
+}
+
+// dst[ids[r] * width + j] = src[r * width + j]
+__global__ void scatter_rows_kernel(const uint32_t *ids, const uint32_t *src, uint32_t *dst,
+                                    uint32_t width, uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count * width;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = i / width;
+    dst[(uint64_t)ids[r] * width + (i - r * width)] = src[i];
+  }
 }
 
 int tmax_for(const hh::RowLayout &lay) {
@@ -161,6 +197,7 @@ struct Workspace {
   uint64_t *cpool = nullptr;  // per-wave candidate pools (kCandPool entries)
   uint32_t *slog = nullptr;   // per-wave spill logs (kSpillLog entries)
   uint64_t spill_slots = 0;
+  uint64_t spill_wcap = 0;   // bitset words per slot the allocation holds
   // brute-force partial lists
   float *part_d = nullptr;
   double *part_d64 = nullptr;
@@ -250,7 +287,9 @@ struct Workspace {
     return HH_OK;
   }
   int ensure_spill(uint64_t slots, uint64_t words) {
-    if (slots <= spill_slots) return HH_OK;
+    if (slots <= spill_slots && words <= spill_wcap) return HH_OK;
+    slots = std::max(slots, spill_slots);
+    words = std::max(words, spill_wcap);
     HH_HIP(hipEventSynchronize(done));
     dfree(spill);
     dfree(cpool);
@@ -264,6 +303,7 @@ struct Workspace {
     }
     HH_HIP(hipMemset(spill, 0, slots * words * sizeof(uint32_t)));
     spill_slots = slots;
+    spill_wcap = words;
     return HH_OK;
   }
   int ensure_part(uint64_t entries, uint64_t lists, bool f64) {
@@ -338,6 +378,9 @@ struct hh_index {
   uint32_t *deleted_bits = nullptr;
   float *sq_offset = nullptr;
   float sq_scale = 1.0f;
+  std::vector<float> sq_offset_host;  // the resident quantizer's offsets
+  // capacities for hh_index_sync (per-node arrays, upper link words)
+  uint64_t cap_n = 0, upper_words = 0, cap_words = 0;
   // Workspaces: one search runs in a workspace at a time; calls on
   // different streams take different workspaces, so their kernels overlap.
   std::mutex pool_mu;
@@ -847,6 +890,7 @@ int hh_index_upload(const hh_graph_desc *d, int device, hh_index **out) {
       HH_HIP(hipMemset(idx->codes, 0, idx->cstride));
       HH_HIP(hipMemcpy(idx->sq_offset, d->sq_offset, d->dim * 4, hipMemcpyHostToDevice));
       idx->sq_scale = d->sq_scale;
+      idx->sq_offset_host.assign(d->sq_offset, d->sq_offset + d->dim);
     }
     *out = idx.release();
     return HH_OK;
@@ -877,6 +921,8 @@ int hh_index_upload(const hh_graph_desc *d, int device, hh_index **out) {
     st = dalloc(&idx->upper_off, n);
     if (!st) st = dalloc(&idx->upper, std::max<uint64_t>(words, 1));
     if (st) return st;
+    idx->upper_words = words;
+    idx->cap_words = std::max<uint64_t>(words, 1);
     HH_HIP(hipMemcpy(idx->upper_off, d->upper_off, n * 8, hipMemcpyHostToDevice));
     if (words) HH_HIP(hipMemcpy(idx->upper, d->upper, words * 4, hipMemcpyHostToDevice));
   }
@@ -902,7 +948,9 @@ int hh_index_upload(const hh_graph_desc *d, int device, hh_index **out) {
     HH_HIP(hipMemcpy(idx->codes, packed.data(), packed.size(), hipMemcpyHostToDevice));
     HH_HIP(hipMemcpy(idx->sq_offset, d->sq_offset, d->dim * 4, hipMemcpyHostToDevice));
     idx->sq_scale = d->sq_scale;
+    idx->sq_offset_host.assign(d->sq_offset, d->sq_offset + d->dim);
   }
+  idx->cap_n = n;
   *out = idx.release();
   return HH_OK;
 }
@@ -927,7 +975,7 @@ int hh_index_enable_bf16(hh_index *idx) {
   if (idx->bf16 || idx->n == 0) return HH_OK;
   HH_HIP(hipSetDevice(idx->device));
   const uint64_t count = idx->n * idx->lay.stride;
-  int st = dalloc(&idx->bf16, count);
+  int st = dalloc(&idx->bf16, std::max(idx->cap_n, idx->n) * idx->lay.stride);
   if (st) return st;
   to_bf16_kernel<<<grid_for(count), 256>>>(idx->rows, idx->bf16, count);
   HH_HIP(hipGetLastError());
@@ -948,7 +996,8 @@ int hh_index_quantize(hh_index *idx, float *offset, float *scale) {
   int8_t *codes = nullptr;
   float *off = nullptr, *off_pos = nullptr;
   uint32_t *mx = nullptr;
-  st = dalloc(&codes, std::max<uint64_t>(n * cstride, cstride));
+  const uint64_t code_rows = std::max<uint64_t>(idx->cap_n, 1);
+  st = dalloc(&codes, code_rows * cstride);
   if (!st) st = dalloc(&off, dim);
   if (!st) st = dalloc(&off_pos, stride);
   if (!st) st = dalloc(&mx, 1);
@@ -963,7 +1012,7 @@ int hh_index_quantize(hh_index *idx, float *offset, float *scale) {
     return st;
   }
   hhk::SqArgs a{idx->rows, n, stride, dim, idx->lay.T, cstride, off, off_pos, mx, codes};
-  const uint64_t code_words = std::max<uint64_t>(n * cstride, cstride) / 4;
+  const uint64_t code_words = code_rows * cstride / 4;
   hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(code_words)), dim3(256), 0, s,
                      reinterpret_cast<uint32_t *>(codes), code_words, 0u);
   hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(1), dim3(256), 0, s, mx, (uint64_t)1, 0u);
@@ -984,7 +1033,9 @@ int hh_index_quantize(hh_index *idx, float *offset, float *scale) {
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(&bits, mx, 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e == hipSuccess && offset && dim) e = hipMemcpy(offset, off, (size_t)dim * 4, hipMemcpyDeviceToHost);
+  std::vector<float> off_host(dim);
+  if (e == hipSuccess && dim)
+    e = hipMemcpy(off_host.data(), off, (size_t)dim * 4, hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     release();
     return fail(HH_DEVICE_ERROR, std::string("device quantizer: ") + hipGetErrorString(e));
@@ -999,10 +1050,142 @@ int hh_index_quantize(hh_index *idx, float *offset, float *scale) {
   idx->sq_offset = off;
   idx->cstride = cstride;
   idx->sq_scale = sc;
+  idx->sq_offset_host = off_host;
+  if (offset && dim) std::memcpy(offset, off_host.data(), (size_t)dim * 4);
   codes = nullptr;
   off = nullptr;
   release();
   if (scale) *scale = sc;
+  return HH_OK;
+}
+
+int hh_index_sync(hh_index *idx, const hh_graph_desc *d, const uint32_t *dirty, uint64_t ndirty) {
+  if (!idx || !d || (ndirty && !dirty)) return fail(HH_INVALID_ARGUMENT, "null argument");
+  if (d->dim != idx->dim || d->metric != idx->metric || d->m != idx->m || d->m0 != idx->m0)
+    return fail(HH_INVALID_ARGUMENT, "graph does not match the resident index: upload it again");
+  if (d->n < idx->n) return fail(HH_INVALID_ARGUMENT, "graph has fewer nodes than the resident index");
+  if (d->n > 0xFFFFFFFEull) return fail(HH_CAPACITY_EXCEEDED, "too many nodes");
+  if (d->n > 0 && (!d->vectors || !d->keys || !d->levels || !d->layer0 || !d->upper_off))
+    return fail(HH_INVALID_ARGUMENT, "null graph array");
+  if (d->top_level >= 0 && d->entry_id >= d->n)
+    return fail(HH_INVALID_ARGUMENT, "entry point out of range");
+  // the quantizer must be the resident one (new rows are encoded with it)
+  const bool codes = idx->codes != nullptr;
+  if (codes != (d->codes != nullptr))
+    return fail(HH_INVALID_ARGUMENT, "quantizer changed: upload the index again");
+  if (codes && (!d->sq_offset || d->sq_scale != idx->sq_scale ||
+                std::memcmp(d->sq_offset, idx->sq_offset_host.data(), (size_t)d->dim * 4) != 0))
+    return fail(HH_INVALID_ARGUMENT, "quantizer changed: upload the index again");
+  const uint64_t n0 = idx->n, n1 = d->n, add = n1 - n0;
+  const uint32_t m0 = d->m0, stride = idx->lay.stride, dim = idx->dim;
+  // links of the new nodes and of the changed old ones, as ELL rows
+  std::vector<uint32_t> ell_new(add * m0, 0xFFFFFFFFu), ids_old, ell_old;
+  bool upper_changed = false;
+  uint64_t words = idx->upper_words;
+  auto ell_row = [&](uint64_t i, uint32_t *row) -> int {
+    const uint32_t *blk = d->layer0 + i * (m0 + 1);
+    const uint32_t c = std::min(blk[0], m0);
+    for (uint32_t s = 0; s < c; s++) {
+      if (blk[1 + s] >= n1) return fail(HH_INVALID_ARGUMENT, "layer-0 neighbour out of range");
+      row[s] = blk[1 + s];
+    }
+    return HH_OK;
+  };
+  for (uint64_t i = n0; i < n1; i++) {
+    int st = ell_row(i, ell_new.data() + (i - n0) * m0);
+    if (st) return st;
+    const uint64_t end = d->upper_off[i] + (uint64_t)d->levels[i] * (d->m + 1);
+    if (d->levels[i]) upper_changed = true;
+    words = std::max(words, end);
+  }
+  for (uint64_t j = 0; j < ndirty; j++) {
+    const uint32_t id = dirty[j];
+    if (id >= n1) return fail(HH_INVALID_ARGUMENT, "dirty node id out of range");
+    if (id >= n0) continue;  // new nodes go over whole
+    ids_old.push_back(id);
+    ell_old.resize(ids_old.size() * m0, 0xFFFFFFFFu);
+    int st = ell_row(id, ell_old.data() + (ids_old.size() - 1) * m0);
+    if (st) return st;
+    if (d->levels[id]) upper_changed = true;
+  }
+  HH_HIP(hipSetDevice(idx->device));
+  int st = drain(idx);
+  if (st) return st;
+  // capacity for the new nodes: half as much again, so a run of small
+  // inserts moves the arrays rarely
+  if (n1 > idx->cap_n) {
+    const uint64_t cap = std::max<uint64_t>({n1, idx->cap_n + idx->cap_n / 2, 1024});
+    st = grow(&idx->rows, n0 * stride, cap * stride);
+    if (!st && idx->bf16) st = grow(&idx->bf16, n0 * stride, cap * stride);
+    if (!st && codes) st = grow(&idx->codes, n0 * idx->cstride, cap * idx->cstride);
+    if (!st) st = grow(&idx->adj0, n0 * m0, cap * m0);
+    if (!st) st = grow(&idx->upper_off, n0, cap);
+    if (!st) st = grow(&idx->keys, n0, cap);
+    if (!st) st = grow(&idx->deleted_bits, (n0 + 31) / 32, (cap + 31) / 32);
+    if (st) return st;
+    idx->cap_n = cap;
+  }
+  if (words > idx->cap_words) {
+    const uint64_t cap = std::max<uint64_t>(words, idx->cap_words + idx->cap_words / 2);
+    st = grow(&idx->upper, idx->upper_words, cap);
+    if (st) return st;
+    idx->cap_words = cap;
+  }
+  if (add) {
+    st = upload_rows_into(d->vectors + n0 * dim, add, dim, idx->lay, idx->rows + n0 * stride);
+    if (st) return st;
+    if (idx->bf16) {
+      to_bf16_kernel<<<grid_for(add * stride), 256>>>(idx->rows + n0 * stride,
+                                                      idx->bf16 + n0 * stride, add * stride);
+      HH_HIP(hipGetLastError());
+    }
+    if (codes) {
+      std::vector<int8_t> packed(add * idx->cstride, 0);
+      for (uint64_t i = 0; i < add; i++)
+        std::memcpy(packed.data() + i * idx->cstride, d->codes + (n0 + i) * dim, dim);
+      HH_HIP(hipMemcpy(idx->codes + n0 * idx->cstride, packed.data(), packed.size(),
+                       hipMemcpyHostToDevice));
+    }
+    HH_HIP(hipMemcpy(idx->adj0 + n0 * m0, ell_new.data(), ell_new.size() * 4,
+                     hipMemcpyHostToDevice));
+    HH_HIP(hipMemcpy(idx->upper_off + n0, d->upper_off + n0, add * 8, hipMemcpyHostToDevice));
+    HH_HIP(hipMemcpy(idx->keys + n0, d->keys + n0, add * 8, hipMemcpyHostToDevice));
+  }
+  if (!ids_old.empty()) {
+    uint32_t *dids = nullptr, *drows = nullptr;
+    st = dalloc(&dids, ids_old.size());
+    if (!st) st = dalloc(&drows, ell_old.size());
+    hipError_t e = hipSuccess;
+    if (!st) e = hipMemcpy(dids, ids_old.data(), ids_old.size() * 4, hipMemcpyHostToDevice);
+    if (!st && e == hipSuccess)
+      e = hipMemcpy(drows, ell_old.data(), ell_old.size() * 4, hipMemcpyHostToDevice);
+    if (!st && e == hipSuccess) {
+      scatter_rows_kernel<<<grid_for(ell_old.size()), 256>>>(dids, drows, idx->adj0, m0,
+                                                             ids_old.size());
+      e = hipGetLastError();
+    }
+    if (!st && e == hipSuccess) e = hipDeviceSynchronize();
+    dfree(dids);
+    dfree(drows);
+    if (st) return st;
+    if (e != hipSuccess) return fail(HH_DEVICE_ERROR, std::string("sync: ") + hipGetErrorString(e));
+  }
+  // upper layers: small (about n/15 blocks of m+1 words); sent whole when
+  // any upper block changed
+  if (upper_changed && words)
+    HH_HIP(hipMemcpy(idx->upper, d->upper, words * 4, hipMemcpyHostToDevice));
+  idx->upper_words = words;
+  {
+    std::vector<uint32_t> bits;
+    pack_deleted(d->deleted, n1, bits);
+    if (!bits.empty())
+      HH_HIP(hipMemcpy(idx->deleted_bits, bits.data(), bits.size() * 4, hipMemcpyHostToDevice));
+  }
+  HH_HIP(hipDeviceSynchronize());
+  idx->n = n1;
+  idx->live = d->live;
+  idx->entry = d->entry_id;
+  idx->top = d->top_level;
   return HH_OK;
 }
 

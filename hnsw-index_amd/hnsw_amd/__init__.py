@@ -235,6 +235,7 @@ class HnswIndex:
         self._device = device
         self._idx = None
         self._dirty = True
+        self._stale = False
         self._bf16 = False
 
     def __del__(self):
@@ -293,6 +294,7 @@ class HnswIndex:
         self._device = device
         self._idx = None
         self._dirty = True
+        self._stale = False
         self._bf16 = False
         return self
 
@@ -342,12 +344,15 @@ class HnswIndex:
         st = lib.hh_graph_insert_batch(self._g, _ptr(keys), _ptr(vectors), keys.size, threads,
                                        ctypes.byref(bad))
         _check(st, key=bad.value)
-        self._dirty = True
+        if self._idx is not None and not self._dirty:
+            self._stale = True  # the resident index takes just the changes
+        else:
+            self._dirty = True
 
     def remove(self, key):
         r = ctypes.c_int(0)
         _check(lib.hh_graph_remove(self._g, key, ctypes.byref(r)))
-        if r.value and self._idx and not self._dirty:
+        if r.value and self._idx and not self._dirty and not self._stale:
             v = self.graph_arrays()
             _check(lib.hh_index_update_deleted(self._idx, _ptr(v["deleted"]), len(self)))
         return bool(r.value)
@@ -429,7 +434,27 @@ class HnswIndex:
             self._idx = h
             self._dirty = False
             self._bf16 = False
+            self._take_dirty()  # the upload carried every link
+            self._stale = False
+        elif self._stale:
+            # inserts and removals since the upload: move only the changes
+            ids = self._take_dirty()
+            d = self._view()
+            st = lib.hh_index_sync(self._idx, ctypes.byref(d), _ptr(ids) if ids.size else None,
+                                   ids.size)
+            self._stale = False
+            if st:
+                self._dirty = True
+                return self.device_index()
         return self._idx
+
+    def _take_dirty(self):
+        """Ids of the nodes whose links changed since the last call."""
+        cnt = ctypes.c_uint64()
+        _check(lib.hh_graph_take_dirty(self._g, None, 0, ctypes.byref(cnt)))
+        ids = np.zeros(max(cnt.value, 1), np.uint32)
+        _check(lib.hh_graph_take_dirty(self._g, _ptr(ids), cnt.value, ctypes.byref(cnt)))
+        return ids[:cnt.value]
 
     def set_query_order(self, longest_first=True):
         """Large batches (1-16 queries per resident wavefront) run in two
@@ -512,6 +537,10 @@ class HnswIndex:
         ost = np.zeros(nq, np.int32)
         _check(lib.hh_search_batch_spec(idx, _ptr(queries), nq, _ptr(specs), kst, _ptr(fl), nf,
                                         _ptr(ok), _ptr(od), _ptr(oc), _ptr(ost)))
+        # slots past each query's count are not written by the library
+        past = np.arange(kst)[None, :] >= oc[:, None]
+        ok[past] = 0
+        od[past] = 0.0
         return ok, od, oc, ost
 
     def search_batch(self, queries, k, ef):

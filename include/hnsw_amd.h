@@ -114,6 +114,12 @@ int hh_graph_insert_batch(hh_graph *g, const uint64_t *keys, const float *vector
                           int threads, uint64_t *bad_key);
 /* HnswIndex::remove (src/hnsw/mod.rs:357-365): *removed = 1 if it was live. */
 int hh_graph_remove(hh_graph *g, uint64_t key, int *removed);
+/* Node ids whose link lists (any layer) were written since the last call
+ * (no crate counterpart: the crate's graph is never mirrored).  *count
+ * receives the number of such nodes; when ids is not NULL, the
+ * min(cap, count) smallest are written to ids in ascending order and
+ * cleared.  Feeds hh_index_sync. */
+int hh_graph_take_dirty(hh_graph *g, uint32_t *ids, uint64_t cap, uint64_t *count);
 /* HnswIndex::enable_quantization (src/hnsw/mod.rs:544-547). */
 int hh_graph_enable_quantization(hh_graph *g);
 int hh_graph_disable_quantization(hh_graph *g);
@@ -217,6 +223,18 @@ int hh_index_enable_bf16(hh_index *idx);
  * and the same rounding of every code.  offset (dim floats) and scale
  * receive the parameters when not NULL.  Replaces a resident quantizer. */
 int hh_index_quantize(hh_index *idx, float *offset, float *scale);
+/* Brings a resident index up to date with the graph it mirrors after
+ * inserts and removals (no crate counterpart: the crate's graph is never
+ * mirrored), moving only what changed: the rows, keys, codes and links of
+ * the nodes added since the upload or last sync, the layer-0 lists of the
+ * old nodes in `dirty` (from hh_graph_take_dirty), the upper-layer links
+ * when any of them changed, the tombstones and the entry point.  Device
+ * arrays grow by half their size when full.  desc must describe the same
+ * graph (dim, metric, m, m0 and the resident quantizer) with at least the
+ * resident number of nodes, else HH_INVALID_ARGUMENT: upload it again.
+ * Searches must not run on the index meanwhile; plans must be re-created. */
+int hh_index_sync(hh_index *idx, const hh_graph_desc *desc, const uint32_t *dirty,
+                  uint64_t ndirty);
 /* Copies the resident codes to host memory as n*dim bytes (the crate's
  * code layout), for hh_graph_set_quantizer.  HH_UNSUPPORTED without codes. */
 int hh_index_download_codes(hh_index *idx, int8_t *codes);
@@ -289,7 +307,8 @@ typedef struct hh_query_spec {
 } hh_query_spec;
 
 /* Mixed batch: query i uses specs[i] (device array); its results start at
- * out_keys + i*k_stride (k_stride >= every spec's k).  ef_max bounds
+ * out_keys + i*k_stride (k_stride >= every spec's k); slots past
+ * out_count[i] are left as they were.  ef_max bounds
  * max(ef, k) over the batch and sizes the launch (0 = 512, the largest
  * supported); a query above it gets HH_UNSUPPORTED.  filters: device array
  * of nfilters bitmaps of ceil(n/32) words each, or NULL. */

@@ -290,3 +290,40 @@ def test_search_needs_a_gpu_and_never_falls_back_to_the_cpu():
     idx = build(b)
     with pytest.raises(H.DeviceError):
         idx.search(q[0], 5, 10)
+
+
+def test_take_dirty_reports_every_changed_link_list():
+    """hh_graph_take_dirty: every node whose layer-0 or upper block changed
+    since the last call is reported (the resident index relies on it)."""
+    import ctypes
+    from hnsw_amd import _lib as L
+    b, _ = O.dataset_gaussian(2600, 1, 16, 19)
+    idx = build(b[:2000], H.Metric.L2, m=8, threads=4)
+    first = idx._take_dirty()
+    assert first.size > 0 and np.all(np.diff(first.astype(np.int64)) > 0)
+    assert idx._take_dirty().size == 0  # cleared
+    v0 = {k: np.array(x, copy=True) for k, x in idx.graph_arrays().items()
+          if isinstance(x, np.ndarray)}
+    idx.insert_batch(np.arange(2000, 2600, dtype=np.uint64), b[2000:], threads=4)
+    dirty = set(idx._take_dirty().tolist())
+    v1 = idx.graph_arrays()
+    m0 = v1["m0"]
+    l0a = v0["layer0"].reshape(2000, m0 + 1)
+    l0b = v1["layer0"].reshape(-1, m0 + 1)[:2000]
+    changed = set(np.nonzero((l0a != l0b).any(axis=1))[0].tolist())
+    m = v1["m"]
+    for node in range(2000):
+        lv = int(v1["levels"][node])
+        if lv:
+            o = int(v1["upper_off"][node])
+            if not np.array_equal(v0["upper"][o:o + lv * (m + 1)], v1["upper"][o:o + lv * (m + 1)]):
+                changed.add(node)
+    assert changed and changed <= dirty
+    assert set(range(2000, 2600)) <= dirty
+    # a partial take returns the smallest ids and clears only those
+    idx.insert_batch(np.arange(5000, 5010, dtype=np.uint64), b[:10] + 0.5, threads=1)
+    cnt = ctypes.c_uint64()
+    ids = np.zeros(3, np.uint32)
+    assert L.lib.hh_graph_take_dirty(idx._g, ids.ctypes.data, 3, ctypes.byref(cnt)) == 0
+    rest = idx._take_dirty()
+    assert cnt.value == 3 + rest.size and ids.max() < rest.min()
