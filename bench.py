@@ -272,14 +272,14 @@ def main():
     idx.search_stats(reset=True)
     run(ef, H.Precision.F32)
     synchronize()
-    st = idx.search_stats()
-    n_dist = st["n_dist"] + st["n_upper"]
-    alg_bytes = n_dist * dim * 4 + st["n_rerank"] * dim * 4
-    adj_bytes = st["n_expand"] * 32 * 4
+    stats = idx.search_stats()
+    n_dist = stats["n_dist"] + stats["n_upper"]
+    alg_bytes = n_dist * dim * 4 + stats["n_rerank"] * dim * 4
+    adj_bytes = stats["n_expand"] * 32 * 4
     kern_ms = float(np.median(kms))
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     log(f"search kernel {kern_ms:.3f} ms/launch; {n_dist / nq:.0f} distance evaluations/query "
-        f"({st['n_upper'] / nq:.0f} in the upper layers), {st['n_expand'] / nq:.1f} layer-0 "
+        f"({stats['n_upper'] / nq:.0f} in the upper layers), {stats['n_expand'] / nq:.1f} layer-0 "
         f"expansions/query; {achieved:.0f} GB/s of gathered rows")
 
     variants, serve, cpu = {}, None, None
@@ -364,6 +364,44 @@ def main():
     except Exception as e:  # the baseline is reported, never required
         log(f"cpu baseline unavailable: {e}")
 
+    # keeping the resident index in step with the graph: 10,000 inserts (1%)
+    # and 1,000 removals, then the incremental sync against a full upload
+    mutation = None
+    try:
+        extra = H.synthetic.Dataset.uniform_clusters(10_000, 1, dim, 1000, 0.1, 0.12, 43,
+                                                     threads=threads).base()
+        idx.device_index()  # nothing pending
+        t0 = time.perf_counter()
+        idx.insert_batch(np.arange(n, n + 10_000, dtype=np.uint64), extra, threads=threads)
+        for key in range(0, 1_000_000, 1000):
+            idx.remove(key)
+        host_s = time.perf_counter() - t0
+        ids = idx._take_dirty()
+        d = idx._view()
+        synchronize()
+        t0 = time.perf_counter()
+        rc = H.lib.hh_index_sync(idx._idx, ctypes.byref(d), ids.ctypes.data if ids.size else None,
+                                 ids.size)
+        sync_rows_s = time.perf_counter() - t0
+        if rc:
+            raise RuntimeError(L.last_error())
+        idx._stale = False
+        pending = int(ids.size)
+        t0 = time.perf_counter()
+        h2 = ctypes.c_void_p()
+        H._check(H.lib.hh_index_upload(ctypes.byref(d), 0, ctypes.byref(h2)))
+        synchronize()
+        full_s = time.perf_counter() - t0
+        H.lib.hh_index_free(h2)
+        mutation = {"inserted": 10_000, "removed": 1000, "host_insert_s": round(host_s, 3),
+                    "changed_link_lists": pending, "sync_s": round(sync_rows_s, 4),
+                    "full_upload_s": round(full_s, 3)}
+        log(f"mutation: 10,000 inserts + 1,000 removals on the host in {host_s:.2f} s "
+            f"({pending} link lists changed); sync {sync_rows_s * 1e3:.1f} ms vs full upload "
+            f"{full_s * 1e3:.0f} ms")
+    except Exception as e:
+        log(f"mutation measurement unavailable: {e}")
+
     emit(args, locals())
 
 
@@ -419,9 +457,10 @@ def emit(args, v):
         "step_ms_p99": round(float(np.percentile(step_ms, 99)), 3),
         "search_kernel_ms": round(kern_ms, 4),
         "two_stream_qps": round(v["pipe_qps"]),
+        "index_sync": v.get("mutation"),
         "dist_evals_per_query": round(n_dist / nq, 1),
-        "upper_evals_per_query": round(v["st"]["n_upper"] / nq, 1),
-        "expansions_per_query": round(v["st"]["n_expand"] / nq, 1),
+        "upper_evals_per_query": round(v["stats"]["n_upper"] / nq, 1),
+        "expansions_per_query": round(v["stats"]["n_expand"] / nq, 1),
         "gathered_bytes_per_launch": alg_bytes,
         "adjacency_bytes_per_launch": adj_bytes,
         "sweep_f32": sweep,

@@ -188,6 +188,7 @@ struct Workspace {
   int8_t *qcode = nullptr;
   int32_t *status = nullptr;
   uint64_t cap_q = 0;
+  uint32_t q_stride = 0, q_cstride = 0;  // row and code strides cap_q was sized for
   uint32_t *counter = nullptr;
   unsigned long long *stats = nullptr;
   // longest-first query order of the two-pass launch
@@ -265,8 +266,14 @@ struct Workspace {
     if (done) (void)hipEventDestroy(done);
     done = nullptr;
   }
+  // Per-query buffers for nq queries of `stride` floats and `cstride` code
+  // bytes.  The code stride changes when a quantizer is trained on a
+  // resident index, so it is part of the capacity.
   int ensure_queries(uint64_t nq, uint32_t stride, uint32_t cstride) {
-    if (nq <= cap_q) return HH_OK;
+    if (nq <= cap_q && stride <= q_stride && cstride <= q_cstride) return HH_OK;
+    nq = std::max(nq, cap_q);
+    stride = std::max(stride, q_stride);
+    cstride = std::max(cstride, q_cstride);
     HH_HIP(hipEventSynchronize(done));
     dfree(qprep);
     dfree(qcode);
@@ -280,10 +287,12 @@ struct Workspace {
     if (!st) st = dalloc(&seed, cap);
     if (!st) st = dalloc(&order, cap);
     if (st) {
-      cap_q = 0;
+      cap_q = q_stride = q_cstride = 0;
       return st;
     }
     cap_q = cap;
+    q_stride = stride;
+    q_cstride = std::max<uint32_t>(cstride, 1);
     return HH_OK;
   }
   int ensure_spill(uint64_t slots, uint64_t words) {

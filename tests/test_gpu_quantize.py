@@ -166,3 +166,19 @@ def test_full_size_training_matches_oracle(metric):
     sample = rng.choice(n, 2000, replace=False)
     assert np.array_equal(codes[sample], O.sq_encode_rows(ooff, oscale, x[sample]))
     assert codes.min() >= -127 and codes.max() <= 127
+
+
+def test_int8_search_on_an_index_that_searched_before_training():
+    """The query workspace made by earlier f32 searches holds no code rows;
+    training on the resident index must size it for the query codes."""
+    b, q = O.dataset_gaussian(3000, 600, 64, 61)
+    idx = build(b, H.Metric.L2, threads=4)
+    idx.search_arrays(q, 10, 64)                 # workspace sized without codes
+    idx.search_arrays(q[:100], 10, 64)
+    idx.enable_quantization(on_device=True)
+    g = oracle_graph(idx)
+    for n in (600, 100):
+        print(f"case nq={n}", flush=True)
+        gk, gd, gc = idx.search_arrays(q[:n], 10, 64, precision=H.Precision.INT8)
+        ok, od, oc, _ = g.search_batch(q[:n], 10, 64, quantized=True)
+        assert np.array_equal(gk, ok) and same_bits(gd, od) and np.array_equal(gc, oc)
