@@ -402,6 +402,24 @@ def main():
     except Exception as e:
         log(f"mutation measurement unavailable: {e}")
 
+    # the same graph built with INSERT's layer-0 searches on the GPU
+    device_build = None
+    try:
+        t0 = time.perf_counter()
+        di = H.HnswIndex(dim, H.Metric.L2, H.HnswParams(16))
+        di.insert_batch(keys, data.base(), threads=threads, device=True)
+        dev_s = time.perf_counter() - t0
+        dk, dd, dc = di.search_arrays(data.queries(), k, ef)
+        dr = H.lib.hh_recall_at_k(nq, tk.ctypes.data, tc.ctypes.data, k, dk.ctypes.data,
+                                  dc.ctypes.data, k, k)
+        device_build = {"seconds": round(dev_s, 2), "host_build_s": round(build_s, 2),
+                        "threads": threads, "recall_at_10": round(dr, 4), "ef": ef}
+        log(f"device-assisted build: {dev_s:.1f} s (host build {build_s:.1f} s), recall@{k} "
+            f"at ef={ef} {dr:.4f} (host-built graph {rec:.4f})")
+        del di
+    except Exception as e:
+        log(f"device-assisted build unavailable: {e}")
+
     emit(args, locals())
 
 
@@ -458,6 +476,7 @@ def emit(args, v):
         "search_kernel_ms": round(kern_ms, 4),
         "two_stream_qps": round(v["pipe_qps"]),
         "index_sync": v.get("mutation"),
+        "device_assisted_build": v.get("device_build"),
         "dist_evals_per_query": round(n_dist / nq, 1),
         "upper_evals_per_query": round(v["stats"]["n_upper"] / nq, 1),
         "expansions_per_query": round(v["stats"]["n_expand"] / nq, 1),

@@ -624,6 +624,18 @@ struct hh_graph {
     if (promo.owns_lock()) entry.store(((uint64_t)level << 32) | id, std::memory_order_release);
   }
 
+  // INSERT (src/hnsw/build.rs:41-96) for a level-0 node whose layer-0
+  // candidates were found elsewhere (the device search from the greedy
+  // descent's end point, ef = max(ef_construction, m)): the selection and
+  // linking of the same function.
+  void link_from(uint32_t id, std::vector<Cand> &found) {
+    std::sort(found.begin(), found.end(), cand_less);
+    const float *q = vec(id);
+    auto sel = select(id, q, found, p.m, 0);
+    install_own(id, sel, 0);
+    for (uint32_t nb : sel) connect(nb, id, 0);
+  }
+
   // append_nodes + push_nodes (src/hnsw/mod.rs:737-767)
   uint32_t append(const uint64_t *ks, const float *prepared, uint64_t cnt) {
     uint32_t first = (uint32_t)n();
@@ -778,6 +790,74 @@ int hh_graph_insert_batch(hh_graph *g, const uint64_t *keys, const float *vector
     return fail(HH_CAPACITY_EXCEEDED, "index capacity exceeded");
   uint32_t first = g->append(keys, prepared.data(), n);
   g->link_parallel(first, threads);
+  return HH_OK;
+}
+
+int hh_graph_append(hh_graph *g, const uint64_t *keys, const float *vectors, uint64_t n,
+                    uint64_t *first, uint64_t *bad_key) {
+  if (!g || !first || (n && (!keys || !vectors))) return fail(HH_INVALID_ARGUMENT, "null argument");
+  std::vector<float> prepared(n * g->dim);
+  for (uint64_t i = 0; i < n; i++) {
+    int st = prepare_vector(g->metric, g->dim, vectors + i * g->dim, prepared.data() + i * g->dim);
+    if (st) return st;
+  }
+  std::unordered_set<uint64_t> seen;
+  seen.reserve(n);
+  for (uint64_t i = 0; i < n; i++) {
+    if (g->key_to_id.count(keys[i]) || !seen.insert(keys[i]).second) {
+      if (bad_key) *bad_key = keys[i];
+      return fail(HH_DUPLICATE_KEY, "key " + std::to_string(keys[i]) + " is already present");
+    }
+  }
+  if (kMaxNodes - g->n() < n) return fail(HH_CAPACITY_EXCEEDED, "index capacity exceeded");
+  *first = g->append(keys, prepared.data(), n);
+  return HH_OK;
+}
+
+int hh_graph_link_batch(hh_graph *g, uint64_t first, uint64_t count, const uint32_t *cand_ids,
+                        const float *cand_d, const uint32_t *cand_counts, uint32_t cand_stride,
+                        int threads) {
+  if (!g) return fail(HH_INVALID_ARGUMENT, "null argument");
+  if (first > g->n() || count > g->n() - first)
+    return fail(HH_INVALID_ARGUMENT, "nodes outside the graph");
+  if (cand_ids && (!cand_d || !cand_counts || cand_stride == 0))
+    return fail(HH_INVALID_ARGUMENT, "null candidate array");
+  const uint32_t begin = (uint32_t)first, end = (uint32_t)(first + count);
+  // candidates apply to level-0 nodes of a graph that already has an entry
+  const bool have_entry = g->entry.load() != kNoEntry;
+  auto one = [&](uint32_t id, Scratch &s) {
+    const uint64_t r = id - begin;
+    if (cand_ids && have_entry && g->levels[id] == 0) {
+      std::vector<Cand> found;
+      const uint32_t c = std::min(cand_counts[r], cand_stride);
+      found.reserve(c);
+      for (uint32_t j = 0; j < c; j++) {
+        const uint32_t nb = cand_ids[r * cand_stride + j];
+        if (nb >= g->n() || nb == id) continue;
+        found.push_back(Cand{cand_d[r * cand_stride + j], nb});
+      }
+      g->link_from(id, found);
+    } else {
+      g->link(id, s);
+    }
+  };
+  if (threads <= 1) {
+    Scratch s;
+    for (uint32_t id = begin; id < end; id++) one(id, s);
+    return HH_OK;
+  }
+  std::atomic<uint32_t> cursor{begin};
+  auto work = [&]() {
+    Scratch s;
+    for (;;) {
+      const uint32_t st = cursor.fetch_add(8);
+      if (st >= end) return;
+      for (uint32_t id = st; id < std::min(end, st + 8); id++) one(id, s);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; t++) pool.emplace_back(work);
+  for (auto &t : pool) t.join();
   return HH_OK;
 }
 

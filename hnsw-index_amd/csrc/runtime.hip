@@ -528,6 +528,11 @@ struct SearchCall {
   float *out_dist;
   uint32_t *out_count;
   int32_t *status;
+  // construction candidates (hh_index_build_candidates): the queries are
+  // the resident rows [rows_first, rows_first + nq), results are node ids,
+  // and tombstoned nodes are candidates like any other
+  bool build = false;
+  uint64_t rows_first = 0;
 };
 
 // Largest max(ef, k) a launch must serve: the call's own for a uniform
@@ -610,7 +615,18 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   pa.has_bf16 = idx->bf16 != nullptr;
   pa.status = c.status ? c.status : ws.status;
   pa.counter = ws.counter;
-  hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(prep_grid(c.nq)), dim3(256), 0, stream, pa);
+  if (c.build) {
+    // stored rows are already prepared and in the row layout
+    HH_HIP(hipMemcpyAsync(ws.qprep, idx->rows + c.rows_first * idx->lay.stride,
+                          c.nq * idx->lay.stride * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(c.nq)), dim3(256), 0, stream,
+                       reinterpret_cast<uint32_t *>(pa.status), c.nq, 0u);
+    hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(1), dim3(64), 0, stream, ws.counter,
+                       (uint64_t)1, 0u);
+  } else {
+    hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(prep_grid(c.nq)), dim3(256), 0, stream,
+                       pa);
+  }
   HH_HIP(hipGetLastError());
 
   hhk::SearchArgs a{};
@@ -622,7 +638,8 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.upper = idx->upper;
   a.keys = idx->keys;
   // the tombstone bitmap is only read when something is deleted
-  a.deleted_bits = idx->live < idx->n ? idx->deleted_bits : nullptr;
+  a.deleted_bits = idx->live < idx->n && !c.build ? idx->deleted_bits : nullptr;
+  a.ids_out = c.build ? 1u : 0u;
   a.n = (uint32_t)idx->n;
   a.m = idx->m;
   a.m0 = idx->m0;
@@ -724,6 +741,8 @@ void release_ws(hh_index *idx, Workspace *w) {
   idx->pool_cv.notify_one();
 }
 
+int launch_call(hh_index *idx, const SearchCall &c, hipStream_t stream);
+
 // Shared launch path of the uniform and the per-query-spec searches.
 int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
                   uint32_t precision, const uint32_t *allow, const hh_query_spec *specs,
@@ -733,6 +752,11 @@ int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, 
   if (nq == 0) return HH_OK;
   const SearchCall c{queries, nq, k, ef, precision, allow, specs, k_stride, filters, nfilters,
                      out_keys, out_dist, out_count, status};
+  return launch_call(idx, c, stream);
+}
+
+int launch_call(hh_index *idx, const SearchCall &c, hipStream_t stream) {
+  const uint64_t nq = c.nq;
   int st = check_call(idx, c);
   if (st) return st;
   HH_HIP(hipSetDevice(idx->device));
@@ -1195,6 +1219,50 @@ int hh_index_sync(hh_index *idx, const hh_graph_desc *d, const uint32_t *dirty, 
   idx->live = d->live;
   idx->entry = d->entry_id;
   idx->top = d->top_level;
+  return HH_OK;
+}
+
+int hh_index_build_candidates(hh_index *idx, uint64_t first, uint64_t count, uint32_t ef,
+                              uint32_t *ids, float *dists, uint32_t *counts) {
+  if (!idx || (count && (!ids || !dists || !counts))) return fail(HH_INVALID_ARGUMENT, "null argument");
+  if (first > idx->n || count > idx->n - first)
+    return fail(HH_INVALID_ARGUMENT, "rows outside the resident index");
+  if (ef == 0 || ef > 512) return fail(HH_UNSUPPORTED, "ef must be in 1..512");
+  if (count == 0) return HH_OK;
+  HH_HIP(hipSetDevice(idx->device));
+  uint64_t *dk = nullptr;
+  float *dd = nullptr;
+  uint32_t *dc = nullptr;
+  int32_t *ds = nullptr;
+  int st = dalloc(&dk, count * ef);
+  if (!st) st = dalloc(&dd, count * ef);
+  if (!st) st = dalloc(&dc, count);
+  if (!st) st = dalloc(&ds, count);
+  auto cleanup = [&] {
+    dfree(dk);
+    dfree(dd);
+    dfree(dc);
+    dfree(ds);
+  };
+  if (st) {
+    cleanup();
+    return st;
+  }
+  SearchCall c{idx->rows, count, ef, ef, 0, nullptr, nullptr, ef, nullptr, 0, dk, dd, dc, ds};
+  c.build = true;
+  c.rows_first = first;
+  hipStream_t s = idx->stream;
+  st = launch_call(idx, c, s);
+  std::vector<uint64_t> hk(count * ef);
+  hipError_t e = hipSuccess;
+  if (!st) e = hipMemcpyAsync(hk.data(), dk, count * ef * 8, hipMemcpyDeviceToHost, s);
+  if (!st && e == hipSuccess) e = hipMemcpyAsync(dists, dd, count * ef * 4, hipMemcpyDeviceToHost, s);
+  if (!st && e == hipSuccess) e = hipMemcpyAsync(counts, dc, count * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  cleanup();
+  if (st) return st;
+  if (e != hipSuccess) return fail(HH_DEVICE_ERROR, std::string("candidates: ") + hipGetErrorString(e));
+  for (uint64_t i = 0; i < count * ef; i++) ids[i] = (uint32_t)hk[i];
   return HH_OK;
 }
 

@@ -672,7 +672,7 @@ hnsw_search_kernel(const SearchArgs P) {
       st.rr += wsize;
     }
     for (int idx = lane; idx < wsize; idx += 64) {
-      const uint64_t key = G(P.keys)[res_id[idx]];
+      const uint64_t key = P.ids_out ? (uint64_t)res_id[idx] : G(P.keys)[res_id[idx]];
       res_key[2 * idx] = (uint32_t)key;
       res_key[2 * idx + 1] = (uint32_t)(key >> 32);
     }

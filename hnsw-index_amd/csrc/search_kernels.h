@@ -58,6 +58,8 @@ struct SearchArgs {
   uint32_t phase;
   uint64_t *seed;
   const uint32_t *order;
+  // construction candidates: node ids instead of keys in out_keys
+  uint32_t ids_out;
   uint32_t hash_cap, hash_limit, lds_words;
 };
 

@@ -331,7 +331,11 @@ class HnswIndex:
     def insert(self, key, vector):
         self.insert_batch([key], np.asarray(vector, np.float32).reshape(1, -1), threads=1)
 
-    def insert_batch(self, keys, vectors, threads=None):
+    def insert_batch(self, keys, vectors, threads=None, device=False):
+        """HnswIndex::insert_batch (src/hnsw/mod.rs:290-349).  With device,
+        the layer-0 searches of INSERT run on the GPU (see _link_on_device);
+        the graph is of the same kind as a parallel host build, not equal to
+        it."""
         keys = np.ascontiguousarray(keys, np.uint64)
         vectors = _f32(vectors)
         if vectors.size != keys.size * self._dim:
@@ -341,13 +345,53 @@ class HnswIndex:
         if threads is None:
             threads = min(16, os.cpu_count() or 1)
         bad = ctypes.c_uint64(0)
+        if device:
+            first = ctypes.c_uint64()
+            _check(lib.hh_graph_append(self._g, _ptr(keys), _ptr(vectors), keys.size,
+                                       ctypes.byref(first), ctypes.byref(bad)), key=bad.value)
+            self._mark_changed()
+            self._link_on_device(first.value, keys.size, threads)
+            return
         st = lib.hh_graph_insert_batch(self._g, _ptr(keys), _ptr(vectors), keys.size, threads,
                                        ctypes.byref(bad))
         _check(st, key=bad.value)
+        self._mark_changed()
+
+    def _mark_changed(self):
         if self._idx is not None and not self._dirty:
             self._stale = True  # the resident index takes just the changes
         else:
             self._dirty = True
+
+    def _link_on_device(self, first, count, threads, host_prefix=4096, fraction=8):
+        """Links appended nodes [first, first + count) in rounds.  While the
+        graph is small (below host_prefix linked nodes) the host INSERT links
+        them; after that each round takes up to 1/fraction of the linked
+        graph: the resident index is synced (the round's rows arrive with no
+        links, so no search can reach them), the GPU searches each row as
+        INSERT does at layer 0 (hh_index_build_candidates), and the host runs
+        the selection and linking on those candidates (hh_graph_link_batch;
+        nodes of level >= 1 take the full host INSERT)."""
+        ef = max(int(self._params._p.ef_construction), int(self._params._p.m))
+        done = 0
+        while done < count:
+            linked = first + done
+            start = first + done
+            if linked < host_prefix:
+                take = min(count - done, host_prefix - linked)
+                _check(lib.hh_graph_link_batch(self._g, start, take, None, None, None, 0, threads))
+            else:
+                take = min(count - done, max(256, linked // fraction))
+                h = self.device_index()
+                ids = np.zeros((take, ef), np.uint32)
+                d = np.zeros((take, ef), np.float32)
+                c = np.zeros(take, np.uint32)
+                _check(lib.hh_index_build_candidates(h, start, take, ef, _ptr(ids), _ptr(d),
+                                                     _ptr(c)))
+                _check(lib.hh_graph_link_batch(self._g, start, take, _ptr(ids), _ptr(d), _ptr(c),
+                                               ef, threads))
+            self._mark_changed()
+            done += take
 
     def remove(self, key):
         r = ctypes.c_int(0)

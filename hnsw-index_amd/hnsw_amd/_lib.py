@@ -84,6 +84,9 @@ _sig("hh_index_quantize", INT, P, P, ctypes.POINTER(ctypes.c_float))
 _sig("hh_index_set_query_order", INT, P, INT)
 _sig("hh_index_sync", INT, P, ctypes.POINTER(GraphDesc), P, U64)
 _sig("hh_graph_take_dirty", INT, P, P, U64, ctypes.POINTER(U64))
+_sig("hh_graph_append", INT, P, P, P, U64, ctypes.POINTER(U64), ctypes.POINTER(U64))
+_sig("hh_graph_link_batch", INT, P, U64, U64, P, P, P, U32, INT)
+_sig("hh_index_build_candidates", INT, P, U64, U64, U32, P, P, P)
 _sig("hh_index_download_codes", INT, P, P)
 _sig("hh_index_stats", INT, P, ctypes.POINTER(SearchStats))
 _sig("hh_index_reset_stats", INT, P)

@@ -114,6 +114,23 @@ int hh_graph_insert_batch(hh_graph *g, const uint64_t *keys, const float *vector
                           int threads, uint64_t *bad_key);
 /* HnswIndex::remove (src/hnsw/mod.rs:357-365): *removed = 1 if it was live. */
 int hh_graph_remove(hh_graph *g, uint64_t key, int *removed);
+/* Construction in two steps, for linking with candidates found on the
+ * device (no crate counterpart; together they are HnswIndex::insert_batch,
+ * src/hnsw/mod.rs:290-349).  hh_graph_append validates and stores the
+ * vectors as insert_batch does (same errors) without linking them; *first
+ * receives the first new node id. */
+int hh_graph_append(hh_graph *g, const uint64_t *keys, const float *vectors, uint64_t n,
+                    uint64_t *first, uint64_t *bad_key);
+/* Links nodes [first, first + count) (INSERT, src/hnsw/build.rs:41-96) on
+ * `threads` threads.  A level-0 node of a graph that has an entry point
+ * takes its layer-0 candidates from row (id - first) of cand_ids / cand_d
+ * (cand_counts[r] entries, stride cand_stride; hh_index_build_candidates
+ * output) and runs the crate's selection and linking on them; every other
+ * node (and every node when cand_ids is NULL) is linked by the full host
+ * INSERT. */
+int hh_graph_link_batch(hh_graph *g, uint64_t first, uint64_t count, const uint32_t *cand_ids,
+                        const float *cand_d, const uint32_t *cand_counts, uint32_t cand_stride,
+                        int threads);
 /* Node ids whose link lists (any layer) were written since the last call
  * (no crate counterpart: the crate's graph is never mirrored).  *count
  * receives the number of such nodes; when ids is not NULL, the
@@ -235,6 +252,16 @@ int hh_index_quantize(hh_index *idx, float *offset, float *scale);
  * Searches must not run on the index meanwhile; plans must be re-created. */
 int hh_index_sync(hh_index *idx, const hh_graph_desc *desc, const uint32_t *dirty,
                   uint64_t ndirty);
+/* Layer-0 construction candidates of the resident rows [first, first +
+ * count) (no crate counterpart): each row is searched as its own query,
+ * exactly as INSERT searches for a level-0 node (greedy descent from the
+ * entry point, then search_layer with `ef`, src/hnsw/build.rs:66-85), with
+ * tombstoned nodes as candidates.  ids / dists: count x ef, ascending
+ * (distance, id); counts: entries per row.  ef in 1..512.  The rows must
+ * have no links yet (appended, then hh_index_sync'd), so a row never finds
+ * itself. */
+int hh_index_build_candidates(hh_index *idx, uint64_t first, uint64_t count, uint32_t ef,
+                              uint32_t *ids, float *dists, uint32_t *counts);
 /* Copies the resident codes to host memory as n*dim bytes (the crate's
  * code layout), for hh_graph_set_quantizer.  HH_UNSUPPORTED without codes. */
 int hh_index_download_codes(hh_index *idx, int8_t *codes);

@@ -327,3 +327,30 @@ def test_take_dirty_reports_every_changed_link_list():
     assert L.lib.hh_graph_take_dirty(idx._g, ids.ctypes.data, 3, ctypes.byref(cnt)) == 0
     rest = idx._take_dirty()
     assert cnt.value == 3 + rest.size and ids.max() < rest.min()
+
+
+def test_append_then_link_equals_insert_batch():
+    """hh_graph_append + hh_graph_link_batch without candidates is the
+    host INSERT: sequentially it builds the same graph as insert_batch."""
+    import ctypes
+    from hnsw_amd import _lib as L
+    b, _ = O.dataset_gaussian(1500, 1, 20, 23)
+    a = build(b, H.Metric.L2, m=8, threads=1)
+    c = H.HnswIndex(20, H.Metric.L2, H.HnswParams(8))
+    keys = np.arange(1500, dtype=np.uint64)
+    first, bad = ctypes.c_uint64(), ctypes.c_uint64()
+    assert L.lib.hh_graph_append(c._g, keys.ctypes.data, b.ctypes.data, 1500, ctypes.byref(first),
+                                 ctypes.byref(bad)) == 0 and first.value == 0
+    assert L.lib.hh_graph_link_batch(c._g, 0, 700, None, None, None, 0, 1) == 0
+    assert L.lib.hh_graph_link_batch(c._g, 700, 800, None, None, None, 0, 1) == 0
+    va, vc = a.graph_arrays(), c.graph_arrays()
+    for f in ("levels", "layer0", "upper_off", "upper", "keys"):
+        assert np.array_equal(va[f], vc[f]), f
+    assert va["entry_id"] == vc["entry_id"] and va["top"] == vc["top"]
+    # append validates like insert_batch: nothing is stored on error
+    dup = np.array([5, 9999], np.uint64)
+    assert L.lib.hh_graph_append(c._g, dup.ctypes.data, b[:2].ctypes.data, 2, ctypes.byref(first),
+                                 ctypes.byref(bad)) == H.DuplicateKey.status
+    assert bad.value == 5 and c.node_count() == 1500
+    assert L.lib.hh_graph_link_batch(c._g, 1400, 200, None, None, None, 0, 1) == \
+        H.InvalidArgument.status
