@@ -363,7 +363,7 @@ class HnswIndex:
         else:
             self._dirty = True
 
-    def _link_on_device(self, first, count, threads, host_prefix=4096, fraction=8):
+    def _link_on_device(self, first, count, threads, host_prefix=4096, fraction=32):
         """Links appended nodes [first, first + count) in rounds.  While the
         graph is small (below host_prefix linked nodes) the host INSERT links
         them; after that each round takes up to 1/fraction of the linked
