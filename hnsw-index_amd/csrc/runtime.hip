@@ -689,14 +689,30 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   // long queries that start last.  Two passes start them first: the upper-
   // layer descent of every query, an order by descending descent distance
   // (which tracks the layer-0 work), then layer 0 in that order.  Results
-  // do not depend on the order.  At ef <= 64 the extra descent pass costs
-  // about what the order saves (measured at 1M x 128, 10k queries), so
-  // only larger ef buckets take it.
+  // do not depend on the order.
   const uint64_t waves = cfg.blocks * (uint64_t)cfg.wpb;
+  // The descent pass runs as descent_kernel (four queries per wavefront)
+  // where it applies: exact-layout f32 rows, m <= 16, one precision.
+  const int tmax = tmax_for(idx->lay);
+  const bool fast_descent = idx->lay.U == 0 && idx->lay.R == 0 && (int)idx->lay.T == tmax &&
+                            tmax <= 8 && idx->m <= 16 && !c.specs && c.precision == 0;
+  // Measured at 1M x 128, 10k queries: with the descent kernel the order
+  // pays from ef = 32 (ef=64 +29%); with the search kernel as the descent
+  // pass, only above ef = 64.
+  const uint32_t efb = spec_ef_bound(c);
   if (idx->longest_first && idx->top >= 1 && c.nq > waves && c.nq <= 16 * waves &&
-      spec_ef_bound(c) > 64) {
+      ((fast_descent && efb > 16) || efb > 64)) {
     a.phase = 1;
-    HH_HIP(hipLaunchKernel(cfg.kern, grid, block, args, lds, stream));
+    if (fast_descent) {
+      const dim3 dgrid((unsigned)((c.nq + 15) / 16));
+      if (tmax == 4)
+        hipLaunchKernelGGL(hhk::descent_kernel<4>, dgrid, dim3(256), 0, stream, a);
+      else
+        hipLaunchKernelGGL(hhk::descent_kernel<8>, dgrid, dim3(256), 0, stream, a);
+      HH_HIP(hipGetLastError());
+    } else {
+      HH_HIP(hipLaunchKernel(cfg.kern, grid, block, args, lds, stream));
+    }
     hipLaunchKernelGGL(hhk::order_queries_kernel, dim3(1), dim3(1024), 0, stream, ws.seed, c.nq,
                        ws.order, ws.counter);
     HH_HIP(hipGetLastError());

@@ -713,6 +713,127 @@ hnsw_search_kernel(const SearchArgs P) {
   }
 }
 
+// ------------------------------------------------------ descent pass
+// The first pass of a two-pass launch when rows have the exact layout
+// (D = 32 * TMAX) and m <= 16: four queries per wavefront, one per 16-lane
+// quarter, one neighbour row per lane.  A lane computes a whole distance
+// with the AVX2 kernel's 32 accumulators (value 32t + j into accumulator j,
+// one fused multiply-add per element in t order, x86.rs:47-69 / 96-120)
+// and its lane combination s[l] = (a_l + a_l+8) + (a_l+16 + a_l+24),
+// h = ((s0+s4)+(s1+s5))+((s2+s6)+(s3+s7)) (x86.rs:21-27), so every
+// distance has the bits of the search kernel's quarter reduction.  The
+// greedy steps are those of search_graph / greedy_closest
+// (src/hnsw/search.rs:49-68,146-161): the first list position holding the
+// smallest distance, taken when strictly below the best.
+template <int TMAX, bool L2>
+__device__ __forceinline__ float lane_distance(const float *q, gptr<float> row) {
+  float acc[32];
+#pragma unroll
+  for (int j = 0; j < 32; j++) acc[j] = 0.0f;
+#pragma unroll
+  for (int b = 0; b < TMAX / 4; b++) {
+#pragma unroll
+    for (int j = 0; j < 32; j++) {
+      const float4 x = *(gptr<float4>)(row + b * 128 + 4 * j);
+      const float4 y = *reinterpret_cast<const float4 *>(q + b * 128 + 4 * j);
+      if (L2) {
+        float d = y.x - x.x;
+        acc[j] = __builtin_fmaf(d, d, acc[j]);
+        d = y.y - x.y;
+        acc[j] = __builtin_fmaf(d, d, acc[j]);
+        d = y.z - x.z;
+        acc[j] = __builtin_fmaf(d, d, acc[j]);
+        d = y.w - x.w;
+        acc[j] = __builtin_fmaf(d, d, acc[j]);
+      } else {
+        acc[j] = __builtin_fmaf(y.x, x.x, acc[j]);
+        acc[j] = __builtin_fmaf(y.y, x.y, acc[j]);
+        acc[j] = __builtin_fmaf(y.z, x.z, acc[j]);
+        acc[j] = __builtin_fmaf(y.w, x.w, acc[j]);
+      }
+    }
+  }
+  float sl[8];
+#pragma unroll
+  for (int l = 0; l < 8; l++)
+    sl[l] = __fadd_rn(__fadd_rn(acc[l], acc[l + 8]), __fadd_rn(acc[l + 16], acc[l + 24]));
+  return __fadd_rn(__fadd_rn(__fadd_rn(sl[0], sl[4]), __fadd_rn(sl[1], sl[5])),
+                   __fadd_rn(__fadd_rn(sl[2], sl[6]), __fadd_rn(sl[3], sl[7])));
+}
+
+template <int TMAX>
+__global__ void __launch_bounds__(256) descent_kernel(const SearchArgs P) {
+  static_assert(TMAX % 4 == 0, "exact layout");
+  constexpr int kStride = 32 * TMAX;
+  __shared__ float qs[16][kStride];
+  const int lane = lane_id();
+  const int i16 = lane & 15;
+  const int qslot = (int)(threadIdx.x >> 4);  // quarter within the block
+  const uint64_t qi = (uint64_t)blockIdx.x * 16 + qslot;
+  unsigned long long up = 0;
+  if (qi < P.nq) {
+    uint32_t k = P.k;
+    if (P.specs) k = G(P.specs)[qi].k;
+    const int status = G(P.status)[qi];
+    if (status != 0 || k == 0 || P.live == 0 || P.top < 0) {
+      GM(P.seed)[qi] = ((uint64_t)__float_as_uint(-INFINITY) << 32) | kInvalid;
+    } else {
+      // the query (prepared, row layout) into this quarter's LDS slot
+      gptr<float4> src = (gptr<float4>)(P.qprep + qi * P.stride);
+      float4 *dst = reinterpret_cast<float4 *>(qs[qslot]);
+      for (int c = i16; c < kStride / 4; c += 16) dst[c] = src[c];
+      wave_sync();
+      const float *q = qs[qslot];
+      const bool l2 = P.metric == 0;
+      auto dist = [&](uint32_t id) -> float {
+        gptr<float> row = G(P.rows) + (size_t)id * P.stride;
+        const float s = l2 ? lane_distance<TMAX, true>(q, row) : lane_distance<TMAX, false>(q, row);
+        return finish_metric(s, P.metric);
+      };
+      uint32_t cur = P.entry;
+      uint64_t cur_off = G(P.upper_off)[cur];
+      float best = dist(cur);
+      up++;
+      for (int layer = P.top; layer >= 1; layer--) {
+        for (;;) {
+          gptr<uint32_t> blk = G(P.upper) + cur_off + (size_t)(layer - 1) * (P.m + 1);
+          uint32_t cnt = blk[0];
+          if (cnt > P.m) cnt = P.m;
+          const bool valid = (uint32_t)i16 < cnt;
+          const uint32_t nid = valid ? blk[1 + i16] : kInvalid;
+          const uint64_t noff = valid ? (uint64_t)G(P.upper_off)[nid] : 0;
+          float md = valid ? dist(nid) : 0.0f;
+          int mi = valid ? i16 : 16;
+#pragma unroll
+          for (int m = 1; m < 16; m <<= 1) {
+            const float od = __shfl_xor(md, m, 16);
+            const int oi = __shfl_xor(mi, m, 16);
+            const bool take = oi < 16 && (mi == 16 || od < md || (!(md < od) && oi < mi));
+            if (take) {
+              md = od;
+              mi = oi;
+            }
+          }
+          up += cnt;
+          const uint32_t bn = (uint32_t)__shfl((int)nid, mi & 15, 16);
+          const uint32_t bo_lo = (uint32_t)__shfl((int)(uint32_t)noff, mi & 15, 16);
+          const uint32_t bo_hi = (uint32_t)__shfl((int)(uint32_t)(noff >> 32), mi & 15, 16);
+          if (mi < 16 && md < best) {
+            best = md;
+            cur = bn;
+            cur_off = ((uint64_t)bo_hi << 32) | bo_lo;
+          } else {
+            break;
+          }
+        }
+      }
+      GM(P.seed)[qi] = ((uint64_t)__float_as_uint(best) << 32) | cur;
+    }
+  }
+  // one count per quarter: lane 0 of each quarter adds its query's total
+  if (i16 == 0 && up && P.stats) atomicAdd(&P.stats[2], up);
+}
+
 // Longest-first query order for the two-pass launch: a histogram of the
 // seeds' order bins in LDS, bins taken from the largest descent distance
 // down (queries within a bin in any order); then resets the search kernel's
@@ -842,6 +963,9 @@ __global__ void __launch_bounds__(256) prepare_queries_kernel(const PrepArgs A) 
        qi += (uint64_t)gridDim.x * 4)
     prepare_one(A, qi, lane);
 }
+
+template __global__ void descent_kernel<4>(const SearchArgs);
+template __global__ void descent_kernel<8>(const SearchArgs);
 
 #define HH_SEARCH_INST(TM, EX, XA)                                                  \
   template __global__ void hnsw_search_kernel<TM, EX, XA, 1>(const SearchArgs);     \

@@ -85,6 +85,11 @@ constexpr uint32_t kOrderBins = 4096;
 
 template <int TMAX, bool EXTRA, bool EXACT, int SW>
 __global__ void hnsw_search_kernel(const SearchArgs P);
+// First pass of a two-pass launch for exact-layout rows (D = 32 * TMAX,
+// TMAX 4 or 8) and m <= 16: 16 queries per 256-thread block (grid
+// ceil(nq / 16)), each descent's seed written as by phase 1.
+template <int TMAX>
+__global__ void descent_kernel(const SearchArgs P);
 // One block of 1024 threads: order[] = the nq queries by descending
 // order_bin of their seed distance; resets the search kernel's query
 // counter for the next pass.

@@ -266,12 +266,13 @@ int hh_index_build_candidates(hh_index *idx, uint64_t first, uint64_t count, uin
  * code layout), for hh_graph_set_quantizer.  HH_UNSUPPORTED without codes. */
 int hh_index_download_codes(hh_index *idx, int8_t *codes);
 /* Query order of large batches (no counterpart in the crate; results never
- * depend on it).  longest_first != 0 (the default): a batch with ef (or a
- * mixed batch's ef bound) above 64 and more than one and at most 16 queries
- * per resident wavefront runs in two passes, the upper-layer descent of
- * every query, then layer 0 with the queries of the largest descent
- * distance first, so that long queries do not start last; 0: one pass in
- * query order. */
+ * depend on it).  longest_first != 0 (the default): a batch of more than
+ * one and at most 16 queries per resident wavefront runs in two passes,
+ * the upper-layer descent of every query, then layer 0 with the queries of
+ * the largest descent distance first, so that long queries do not start
+ * last.  It applies from ef 32 for uniform f32 batches on rows of 128 or
+ * 256 dimensions with m <= 16 (the descent then runs four queries per
+ * wavefront), and above ef 64 otherwise.  0: one pass in query order. */
 int hh_index_set_query_order(hh_index *idx, int longest_first);
 
 /* Search statistics accumulated by the kernels since the last reset. */

@@ -378,3 +378,33 @@ def test_longest_first_order_returns_what_one_pass_returns():
         assert np.array_equal(a.view(np.uint8), b_.view(np.uint8))
     assert out[True][3][7] == 4 and out[True][2][7] == 0
     assert (out[True][2][ks == 0] == 0).all()
+
+
+@pytest.mark.parametrize("metric", [H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine])
+@pytest.mark.parametrize("dim", [128, 256])
+def test_descent_kernel_two_pass_matches_one_pass(metric, dim):
+    """Exact-layout rows with m <= 16 take the descent kernel (four queries
+    per wavefront, whole distances per lane in the AVX2 order) as the first
+    pass at every ef: the results and every counter equal the one-pass
+    launch's, and the oracle's."""
+    print(f"case metric={metric} dim={dim}", flush=True)
+    b, q = O.dataset_gaussian(4000, 12000, dim, 83 + dim)
+    idx = build(b, metric, m=16, threads=8)
+    assert idx.graph_arrays()["top"] >= 1
+    runs = {}
+    for longest_first in (False, True):
+        idx.set_query_order(longest_first)
+        for ef in (16, 64):
+            idx.search_stats(reset=True)
+            res = idx.search_arrays(q, 10, ef)
+            runs[longest_first, ef] = res, idx.search_stats()
+    for ef in (16, 64):
+        (k1, d1, c1), s1 = runs[False, ef]
+        (k2, d2, c2), s2 = runs[True, ef]
+        assert np.array_equal(c1, c2) and np.array_equal(k1, k2) and same_bits(d1, d2)
+        for f in ("queries", "n_dist", "n_upper", "n_expand", "n_overflow"):
+            assert s1[f] == s2[f], (ef, f)
+    g = oracle_graph(idx)
+    ok, od, oc, cnt = g.search_batch(q[:300], 10, 64)
+    (k2, d2, c2), s2 = runs[True, 64]
+    assert np.array_equal(k2[:300], ok) and same_bits(d2[:300], od)
