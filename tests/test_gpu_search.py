@@ -394,11 +394,11 @@ def test_descent_kernel_two_pass_matches_one_pass(metric, dim):
     runs = {}
     for longest_first in (False, True):
         idx.set_query_order(longest_first)
-        for ef in (16, 64):
+        for ef in (32, 64):
             idx.search_stats(reset=True)
             res = idx.search_arrays(q, 10, ef)
             runs[longest_first, ef] = res, idx.search_stats()
-    for ef in (16, 64):
+    for ef in (32, 64):
         (k1, d1, c1), s1 = runs[False, ef]
         (k2, d2, c2), s2 = runs[True, ef]
         assert np.array_equal(c1, c2) and np.array_equal(k1, k2) and same_bits(d1, d2)
