@@ -695,7 +695,8 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   // where it applies: exact-layout f32 rows, m <= 16, one precision.
   const int tmax = tmax_for(idx->lay);
   const bool fast_descent = idx->lay.U == 0 && idx->lay.R == 0 && (int)idx->lay.T == tmax &&
-                            tmax <= 8 && idx->m <= 16 && !c.specs && c.precision == 0;
+                            tmax <= 8 && idx->m <= 16 && !c.specs && !c.build &&
+                            (c.precision != 1 || idx->cstride == 32u * (uint32_t)tmax);
   // Measured at 1M x 128, 10k queries: with the descent kernel the order
   // pays from ef = 32 (ef=64 +29%); with the search kernel as the descent
   // pass, only above ef = 64.

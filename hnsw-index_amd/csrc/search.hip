@@ -725,8 +725,8 @@ hnsw_search_kernel(const SearchArgs P) {
 // greedy steps are those of search_graph / greedy_closest
 // (src/hnsw/search.rs:49-68,146-161): the first list position holding the
 // smallest distance, taken when strictly below the best.
-template <int TMAX, bool L2>
-__device__ __forceinline__ float lane_distance(const float *q, gptr<float> row) {
+template <int TMAX, bool L2, class E>
+__device__ __forceinline__ float lane_distance(const float *q, gptr<E> row) {
   float acc[32];
 #pragma unroll
   for (int j = 0; j < 32; j++) acc[j] = 0.0f;
@@ -734,7 +734,14 @@ __device__ __forceinline__ float lane_distance(const float *q, gptr<float> row) 
   for (int b = 0; b < TMAX / 4; b++) {
 #pragma unroll
     for (int j = 0; j < 32; j++) {
-      const float4 x = *(gptr<float4>)(row + b * 128 + 4 * j);
+      float4 x;
+      if constexpr (sizeof(E) == 4) {
+        x = *(gptr<float4>)(row + b * 128 + 4 * j);
+      } else {
+        const uint2 h = *(gptr<uint2>)(row + b * 128 + 4 * j);
+        x = make_float4(__uint_as_float(h.x << 16), __uint_as_float(h.x & 0xFFFF0000u),
+                        __uint_as_float(h.y << 16), __uint_as_float(h.y & 0xFFFF0000u));
+      }
       const float4 y = *reinterpret_cast<const float4 *>(q + b * 128 + 4 * j);
       if (L2) {
         float d = y.x - x.x;
@@ -761,11 +768,61 @@ __device__ __forceinline__ float lane_distance(const float *q, gptr<float> row) 
                    __fadd_rn(__fadd_rn(sl[2], sl[6]), __fadd_rn(sl[3], sl[7])));
 }
 
+// int8 codes: the exact integer sums of the crate's quantized distance
+// (src/kernels/int8.rs:13-48) in any order, rescaled as the search kernel
+// does (src/quantization.rs:85-92).
+template <int TMAX>
+__device__ __forceinline__ float lane_distance_i8(const SearchArgs &P, const uint2 *qc,
+                                                  gptr<uint2> row) {
+  constexpr int kWords = 32 * TMAX / 8;  // uint2 words of codes (cstride = D)
+  int acc = 0;
+#pragma unroll 8
+  for (int c = 0; c < kWords; c++) {
+    const uint2 a = qc[c], b = row[c];
+    if (P.metric == 0) {
+      acc += __builtin_amdgcn_sdot4((int)a.x, (int)a.x, 0, false) -
+             2 * __builtin_amdgcn_sdot4((int)a.x, (int)b.x, 0, false) +
+             __builtin_amdgcn_sdot4((int)b.x, (int)b.x, 0, false);
+      acc += __builtin_amdgcn_sdot4((int)a.y, (int)a.y, 0, false) -
+             2 * __builtin_amdgcn_sdot4((int)a.y, (int)b.y, 0, false) +
+             __builtin_amdgcn_sdot4((int)b.y, (int)b.y, 0, false);
+    } else {
+      acc += __builtin_amdgcn_sdot4((int)a.x, (int)b.x, 0, false);
+      acc += __builtin_amdgcn_sdot4((int)a.y, (int)b.y, 0, false);
+    }
+  }
+  const float s2 = __fmul_rn(P.sq_scale, P.sq_scale);
+  if (P.metric == 0) return __fmul_rn(__uint2float_rn((uint32_t)acc), s2);
+  if (P.metric == 1) return __fmul_rn(-__int2float_rn(acc), s2);
+  return __fsub_rn(1.0f, __fmul_rn(__int2float_rn(acc), s2));
+}
+
+// The traversal distance of the batch's precision (0 f32, 1 int8 codes,
+// 2 bf16 rows), with the bits the search kernel computes.
+template <int TMAX>
+__device__ __forceinline__ float descent_distance(const SearchArgs &P, const float *q,
+                                                  const uint2 *qc, uint32_t id) {
+  if (P.precision == 1)
+    return lane_distance_i8<TMAX>(P, qc, (gptr<uint2>)(P.codes + (size_t)id * P.cstride));
+  float s;
+  if (P.precision == 2) {
+    gptr<uint16_t> row = G(P.bf16) + (size_t)id * P.stride;
+    s = P.metric == 0 ? lane_distance<TMAX, true, uint16_t>(q, row)
+                      : lane_distance<TMAX, false, uint16_t>(q, row);
+  } else {
+    gptr<float> row = G(P.rows) + (size_t)id * P.stride;
+    s = P.metric == 0 ? lane_distance<TMAX, true, float>(q, row)
+                      : lane_distance<TMAX, false, float>(q, row);
+  }
+  return finish_metric(s, P.metric);
+}
+
 template <int TMAX>
 __global__ void __launch_bounds__(256) descent_kernel(const SearchArgs P) {
   static_assert(TMAX % 4 == 0, "exact layout");
   constexpr int kStride = 32 * TMAX;
   __shared__ float qs[16][kStride];
+  __shared__ uint2 qcs[16][kStride / 8];
   const int lane = lane_id();
   const int i16 = lane & 15;
   const int qslot = (int)(threadIdx.x >> 4);  // quarter within the block
@@ -782,17 +839,16 @@ __global__ void __launch_bounds__(256) descent_kernel(const SearchArgs P) {
       gptr<float4> src = (gptr<float4>)(P.qprep + qi * P.stride);
       float4 *dst = reinterpret_cast<float4 *>(qs[qslot]);
       for (int c = i16; c < kStride / 4; c += 16) dst[c] = src[c];
+      if (P.precision == 1) {
+        gptr<uint2> csrc = (gptr<uint2>)(P.qcode + qi * P.cstride);
+        for (int c = i16; c < kStride / 8; c += 16) qcs[qslot][c] = csrc[c];
+      }
       wave_sync();
       const float *q = qs[qslot];
-      const bool l2 = P.metric == 0;
-      auto dist = [&](uint32_t id) -> float {
-        gptr<float> row = G(P.rows) + (size_t)id * P.stride;
-        const float s = l2 ? lane_distance<TMAX, true>(q, row) : lane_distance<TMAX, false>(q, row);
-        return finish_metric(s, P.metric);
-      };
+      const uint2 *qc = qcs[qslot];
       uint32_t cur = P.entry;
       uint64_t cur_off = G(P.upper_off)[cur];
-      float best = dist(cur);
+      float best = descent_distance<TMAX>(P, q, qc, cur);
       up++;
       for (int layer = P.top; layer >= 1; layer--) {
         for (;;) {
@@ -802,7 +858,7 @@ __global__ void __launch_bounds__(256) descent_kernel(const SearchArgs P) {
           const bool valid = (uint32_t)i16 < cnt;
           const uint32_t nid = valid ? blk[1 + i16] : kInvalid;
           const uint64_t noff = valid ? (uint64_t)G(P.upper_off)[nid] : 0;
-          float md = valid ? dist(nid) : 0.0f;
+          float md = valid ? descent_distance<TMAX>(P, q, qc, nid) : 0.0f;
           int mi = valid ? i16 : 16;
 #pragma unroll
           for (int m = 1; m < 16; m <<= 1) {

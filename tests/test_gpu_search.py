@@ -408,3 +408,29 @@ def test_descent_kernel_two_pass_matches_one_pass(metric, dim):
     ok, od, oc, cnt = g.search_batch(q[:300], 10, 64)
     (k2, d2, c2), s2 = runs[True, 64]
     assert np.array_equal(k2[:300], ok) and same_bits(d2[:300], od)
+
+
+@pytest.mark.parametrize("metric", [H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine])
+def test_descent_kernel_two_pass_bf16_and_int8(metric):
+    """The descent kernel's bf16 and int8 distances have the search
+    kernel's bits: two-pass and one-pass launches agree in those
+    precisions too (results after the exact re-rank, and counters)."""
+    print(f"case metric={metric}", flush=True)
+    b, q = O.dataset_gaussian(4000, 12000, 128, 89)
+    idx = build(b, metric, m=16, threads=8)
+    idx.enable_quantization(on_device=True)
+    idx.enable_bf16()
+    for prec in (H.Precision.BF16, H.Precision.INT8):
+        runs = {}
+        for longest_first in (False, True):
+            idx.set_query_order(longest_first)
+            idx.search_stats(reset=True)
+            runs[longest_first] = idx.search_arrays(q, 10, 64, precision=prec), idx.search_stats()
+        (k1, d1, c1), s1 = runs[False]
+        (k2, d2, c2), s2 = runs[True]
+        assert np.array_equal(c1, c2) and np.array_equal(k1, k2) and same_bits(d1, d2), prec
+        for f in ("queries", "n_dist", "n_upper", "n_expand", "n_rerank"):
+            assert s1[f] == s2[f], (prec, f)
+    g = oracle_graph(idx)
+    ok, od, oc, _ = g.search_batch(q[:300], 10, 64, quantized=True)
+    assert np.array_equal(k2[:300], ok) and same_bits(d2[:300], od)
