@@ -36,6 +36,7 @@ uint32_t index_dim(const hh_index *idx);
 uint64_t index_nodes(const hh_index *idx);
 // Wavefronts the search kernel keeps resident on the whole GPU at ef <= 64.
 uint32_t index_resident_waves(hh_index *idx);
+int index_reserve(hh_index *idx, uint64_t nq, uint32_t count);
 
 // Row layout in HBM.  A dimension splits as D = 32*T + 8*U + R (the block
 // structure of the crate's AVX2 kernel, src/kernels/x86.rs:47-77).  The

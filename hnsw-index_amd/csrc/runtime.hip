@@ -813,6 +813,27 @@ uint32_t index_resident_waves(hh_index *idx) {
   if (!tmax_for(idx->lay) || search_config(idx, c, cfg)) return 1024;
   return (uint32_t)std::max<uint64_t>(1, cfg.blocks * cfg.wpb);
 }
+
+// Sizes `count` of the index's workspaces for launches of up to nq queries
+// at the largest resident wave count, so that no launch of a serving loop
+// reallocates (a device free synchronises the whole device).
+int index_reserve(hh_index *idx, uint64_t nq, uint32_t count) {
+  if (!tmax_for(idx->lay)) return HH_OK;
+  HH_HIP(hipSetDevice(idx->device));
+  const uint64_t slots = index_resident_waves(idx);
+  const uint64_t words = std::max<uint64_t>(1, (idx->n + 31) / 32);
+  std::vector<Workspace *> held;
+  int st = HH_OK;
+  for (uint32_t i = 0; i < count && !st; i++) {
+    Workspace *w = acquire_ws(idx, st);
+    if (!w) break;
+    held.push_back(w);
+    st = w->ensure_queries(nq, idx->lay.stride, idx->cstride);
+    if (!st) st = w->ensure_spill(slots, words);
+  }
+  for (Workspace *w : held) release_ws(idx, w);
+  return st;
+}
 }  // namespace hh
 
 namespace {

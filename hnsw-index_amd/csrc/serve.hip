@@ -252,6 +252,12 @@ int hh_serve_open(hh_index *idx, const hh_serve_config *cfg, hh_server **out) {
     s->slots.push_back(std::move(sl));
     if (st) return st;
   }
+  {
+    // every workspace a launch can take (the index keeps up to four) sized
+    // now, not on the first large batch
+    int st = hh::index_reserve(idx, s->max_batch, 4);
+    if (st) return st;
+  }
   hh_server *raw = s.get();
   raw->worker = std::thread([raw] { raw->run(); });
   *out = s.release();
