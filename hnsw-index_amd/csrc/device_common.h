@@ -294,88 +294,6 @@ __device__ __forceinline__ float finish_metric(float sum, uint32_t metric) {
   return __fsub_rn(1.0f, sum);
 }
 
-// ------------------------------------------------------ sorted lists
-// A sorted list of up to 64*S (distance, id) pairs spread over the wave:
-// element i lives in lane i / S, slot i % S.  size is wave-uniform.
-template <int S>
-struct SList {
-  float d[S];
-  uint32_t id[S];
-};
-
-template <int S>
-__device__ __forceinline__ int slist_rank(const SList<S> &L, int size, float nd, uint32_t nid) {
-  const int lane = lane_id();
-  int p = 0;
-#pragma unroll
-  for (int s = 0; s < S; s++) {
-    bool lt = (lane * S + s) < size && cless(L.d[s], L.id[s], nd, nid);
-    p += popc64(__ballot(lt));
-  }
-  return p;
-}
-
-// Inserts at position p, shifting the tail right; the element at index
-// 64*S - 1 falls off.
-template <int S>
-__device__ __forceinline__ void slist_insert_at(SList<S> &L, int p, float nd, uint32_t nid) {
-  const int lane = lane_id();
-  const float upd = __shfl_up(L.d[S - 1], 1, 64);
-  const uint32_t upi = (uint32_t)__shfl_up((int)L.id[S - 1], 1, 64);
-#pragma unroll
-  for (int s = S - 1; s >= 0; s--) {
-    const int idx = lane * S + s;
-    const float pd = s > 0 ? L.d[s > 0 ? s - 1 : 0] : upd;
-    const uint32_t pi = s > 0 ? L.id[s > 0 ? s - 1 : 0] : upi;
-    if (idx > p) {
-      L.d[s] = pd;
-      L.id[s] = pi;
-    } else if (idx == p) {
-      L.d[s] = nd;
-      L.id[s] = nid;
-    }
-  }
-}
-
-template <int S>
-__device__ __forceinline__ void slist_pop_front(SList<S> &L) {
-  const float dn = __shfl_down(L.d[0], 1, 64);
-  const uint32_t in = (uint32_t)__shfl_down((int)L.id[0], 1, 64);
-#pragma unroll
-  for (int s = 0; s < S; s++) {
-    L.d[s] = s < S - 1 ? L.d[s < S - 1 ? s + 1 : 0] : dn;
-    L.id[s] = s < S - 1 ? L.id[s < S - 1 ? s + 1 : 0] : in;
-  }
-}
-
-template <int S>
-__device__ __forceinline__ void slist_get(const SList<S> &L, int i, float &d, uint32_t &id) {
-  const int ln = i / S, sl = i % S;
-  float vd = L.d[0];
-  uint32_t vi = L.id[0];
-#pragma unroll
-  for (int s = 1; s < S; s++)
-    if (sl == s) {
-      vd = L.d[s];
-      vi = L.id[s];
-    }
-  d = rlf(vd, ln);
-  id = rl(vi, ln);
-}
-
-// Number of leading elements whose distance is not IEEE-greater than w.
-template <int S>
-__device__ __forceinline__ int slist_count_not_above(const SList<S> &L, int size, float w) {
-  const int lane = lane_id();
-  int c = 0;
-#pragma unroll
-  for (int s = 0; s < S; s++) {
-    bool keep = (lane * S + s) < size && !(L.d[s] > w);
-    c += popc64(__ballot(keep));
-  }
-  return c;
-}
-
 // ------------------------------------------------------- visited set
 // Open-addressing hash of node ids in LDS, exact (no false positives or
 // negatives), any capacity (Lemire range reduction), linear probing.

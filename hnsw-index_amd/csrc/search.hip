@@ -38,21 +38,6 @@ __device__ __forceinline__ float ckey_d(uint64_t k) {
   return __int_as_float(t ^ (int32_t)(((uint32_t)(t >> 31)) >> 1));
 }
 
-// Appends candidate x to the wave's pool (all lanes store the same word).
-// `dead`: the candidate can no longer be expanded (results full and its
-// distance above the worst result, src/hnsw/search.rs:98-100): dropped.
-__device__ __forceinline__ void pool_push(gmut<uint64_t> pool, bool has_pool, uint32_t cap,
-                                          bool dead, uint64_t x, uint32_t &pn, uint64_t &pmin,
-                                          unsigned long long &ovf, unsigned long long &moved) {
-  if (dead) return;
-  const bool room = has_pool && pn < cap;
-  if (room) pool[pn] = x;
-  ovf += room ? 0 : 1;
-  moved += room ? 1 : 0;
-  pmin = room && x < pmin ? x : pmin;
-  pn += room ? 1 : 0;
-}
-
 // ---------------------------------------------------------------- lists
 // C (candidates) and W (results) are sorted arrays of candidate words
 // (`ckey`) in the wave's LDS.  A neighbour batch is merged in one pass: the
