@@ -164,7 +164,7 @@ __global__ void scatter_rows_kernel(const uint32_t *ids, const uint32_t *src, ui
 int tmax_for(const hh::RowLayout &lay) {
   if (lay.T <= 4) return 4;
   if (lay.T <= 8) return 8;
-  if (lay.T <= 32) return 32;
+  if (lay.T <= 32 && lay.dim <= 1024) return 32;
   return 0;
 }
 // LDS words per wavefront of the search kernel: the visited table, the
@@ -590,6 +590,39 @@ int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
   return HH_OK;
 }
 
+// The descent pass as descent_kernel: rows of T 4-blocks take the TMAX
+// instance that holds them, the exact layouts (D = 32 * TMAX) the unrolled
+// one, m <= 16 sixteen lanes per query and m <= 32 thirty-two.
+int launch_descent(const hh_index *idx, uint64_t nq, const hhk::SearchArgs &a,
+                   hipStream_t stream) {
+  using K = void (*)(const hhk::SearchArgs);
+  const int tmax = tmax_for(idx->lay);
+  const bool exact = idx->lay.U == 0 && idx->lay.R == 0 && (int)idx->lay.T == tmax &&
+                     idx->cstride == 32u * (uint32_t)tmax;
+  const int wide = idx->m > 16;
+  const int ti = tmax == 4 ? 0 : tmax == 8 ? 1 : 2;
+  static const K table[3][2][2] = {
+      {{hhk::descent_kernel<4, 16, false>, hhk::descent_kernel<4, 16, true>},
+       {hhk::descent_kernel<4, 32, false>, hhk::descent_kernel<4, 32, true>}},
+      {{hhk::descent_kernel<8, 16, false>, hhk::descent_kernel<8, 16, true>},
+       {hhk::descent_kernel<8, 32, false>, hhk::descent_kernel<8, 32, true>}},
+      {{hhk::descent_kernel<32, 16, false>, hhk::descent_kernel<32, 16, true>},
+       {hhk::descent_kernel<32, 32, false>, hhk::descent_kernel<32, 32, true>}}};
+  const int threads = tmax <= 8 ? hhk::descent_threads<8>() : hhk::descent_threads<32>();
+  const int stride_cap = tmax == 4 ? hhk::descent_stride<4>()
+                         : tmax == 8 ? hhk::descent_stride<8>() : hhk::descent_stride<32>();
+  const int code_cap = tmax == 4 ? hhk::descent_code_words<4>()
+                       : tmax == 8 ? hhk::descent_code_words<8>() : hhk::descent_code_words<32>();
+  if (!tmax || idx->m > 32 || idx->lay.stride > (uint32_t)stride_cap ||
+      idx->cstride / 8 > (uint32_t)code_cap)
+    return fail(HH_UNSUPPORTED, "descent kernel: layout outside its slot sizes");
+  const uint32_t per_block = (uint32_t)threads / (wide ? 32u : 16u);
+  const dim3 grid((unsigned)((nq + per_block - 1) / per_block)), block((unsigned)threads);
+  hipLaunchKernelGGL(table[ti][wide][exact], grid, block, 0, stream, a);
+  HH_HIP(hipGetLastError());
+  return HH_OK;
+}
+
 // Enqueues the prepare and search kernels of one call on `stream`, using
 // workspace `ws` (whose buffers the caller has sized).  No synchronisation.
 int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const SearchConfig &cfg,
@@ -691,12 +724,9 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   // (which tracks the layer-0 work), then layer 0 in that order.  Results
   // do not depend on the order.
   const uint64_t waves = cfg.blocks * (uint64_t)cfg.wpb;
-  // The descent pass runs as descent_kernel (four queries per wavefront)
-  // where it applies: exact-layout f32 rows, m <= 16, one precision.
-  const int tmax = tmax_for(idx->lay);
-  const bool fast_descent = idx->lay.U == 0 && idx->lay.R == 0 && (int)idx->lay.T == tmax &&
-                            tmax <= 8 && idx->m <= 16 && !c.specs && !c.build &&
-                            (c.precision != 1 || idx->cstride == 32u * (uint32_t)tmax);
+  // The descent pass runs as descent_kernel (one query per 16 or 32 lanes)
+  // where it applies: m <= 32 and one precision for the batch.
+  const bool fast_descent = idx->m <= 32 && !c.specs && !c.build;
   // Measured at 1M x 128, 10k queries: with the descent kernel the order
   // pays from ef = 32 (ef=64 +29%); with the search kernel as the descent
   // pass, only above ef = 64.
@@ -705,12 +735,7 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
       ((fast_descent && efb > 16) || efb > 64)) {
     a.phase = 1;
     if (fast_descent) {
-      const dim3 dgrid((unsigned)((c.nq + 15) / 16));
-      if (tmax == 4)
-        hipLaunchKernelGGL(hhk::descent_kernel<4>, dgrid, dim3(256), 0, stream, a);
-      else
-        hipLaunchKernelGGL(hhk::descent_kernel<8>, dgrid, dim3(256), 0, stream, a);
-      HH_HIP(hipGetLastError());
+      if (int st = launch_descent(idx, c.nq, a, stream)) return st;
     } else {
       HH_HIP(hipLaunchKernel(cfg.kern, grid, block, args, lds, stream));
     }

@@ -236,19 +236,24 @@ __device__ __forceinline__ void gather_rows(const SearchArgs &P, const QLanes<TM
 // Distances from the query to rows ids[0..cnt) (ids in LDS), written to
 // out[0..cnt) (LDS), with traversal precision `prec` (0 exact f32,
 // 1 int8 codes, 2 bf16 rows).  All 64 lanes must call it.
-template <int TMAX>
-constexpr int code_chunks() { return (TMAX * 32 + 127) / 128; }
+// 128-byte code chunks a lane group holds: cstride / 128 for the exact
+// layouts, else the most any dim of that TMAX needs (dim <= 32 * TMAX + 31,
+// and dim <= 1024).
+template <int TMAX, bool EXACT>
+constexpr int code_chunks() {
+  return EXACT ? TMAX / 4 : ((TMAX * 32 + 31 + 127) / 128 < 8 ? (TMAX * 32 + 31 + 127) / 128 : 8);
+}
 
 template <int TMAX, bool EXTRA, bool EXACT>
 __device__ __forceinline__ void gather_dists(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
-                                             const uint2 (&qc)[code_chunks<TMAX>()], int prec,
+                                             const uint2 (&qc)[code_chunks<TMAX, EXACT>()], int prec,
                                              const uint32_t *ids, int cnt, float *out) {
   const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
   if (prec == 1) {
     // int8 codes of the crate's quantizer: exact integer sums
     // (src/kernels/int8.rs:13-48), rescaled as src/quantization.rs:85-92.
     // Four rows per quarter per round, loads first.
-    constexpr int CC = code_chunks<TMAX>();
+    constexpr int CC = code_chunks<TMAX, EXACT>();
     const int chunks = (int)(P.cstride >> 7);
     const float s2 = __fmul_rn(P.sq_scale, P.sq_scale);
     for (int base = 0; base < cnt; base += 16) {
@@ -385,14 +390,14 @@ hnsw_search_kernel(const SearchArgs P) {
     q.e[0] = q.r = 0.0f;
     if constexpr (EXTRA) q.e[1] = q.e[2] = 0.0f;
     load_q<TMAX, EXTRA, float>(P.qprep + (size_t)qi * P.stride, shp, i16, q);
-    uint2 qc[code_chunks<TMAX>()];
+    uint2 qc[code_chunks<TMAX, EXACT>()];
 #pragma unroll
-    for (int c = 0; c < code_chunks<TMAX>(); c++) qc[c] = make_uint2(0, 0);
+    for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++) qc[c] = make_uint2(0, 0);
     if (prec == 1) {
       gptr<uint2> cq = (gptr<uint2>)(P.qcode + (size_t)qi * P.cstride);
       const int chunks = (int)(P.cstride >> 7);
 #pragma unroll
-      for (int c = 0; c < code_chunks<TMAX>(); c++)
+      for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++)
         if (c < chunks) qc[c] = cq[c * 16 + i16];
     }
 
@@ -710,6 +715,49 @@ hnsw_search_kernel(const SearchArgs P) {
 // greedy steps are those of search_graph / greedy_closest
 // (src/hnsw/search.rs:49-68,146-161): the first list position holding the
 // smallest distance, taken when strictly below the best.
+// Accumulator j's four fused multiply-adds of one full 4-block (values
+// 32t + j, t in 4b..4b+3, at j*4 + t%4), in t order.
+template <bool L2, class E>
+__device__ __forceinline__ float acc4(float acc, const float *q, gptr<E> blk, int j) {
+  float4 x;
+  if constexpr (sizeof(E) == 4) {
+    x = *(gptr<float4>)(blk + 4 * j);
+  } else {
+    const uint2 h = *(gptr<uint2>)(blk + 4 * j);
+    x = make_float4(__uint_as_float(h.x << 16), __uint_as_float(h.x & 0xFFFF0000u),
+                    __uint_as_float(h.y << 16), __uint_as_float(h.y & 0xFFFF0000u));
+  }
+  const float4 y = *reinterpret_cast<const float4 *>(q + 4 * j);
+  if (L2) {
+    float d = y.x - x.x;
+    acc = __builtin_fmaf(d, d, acc);
+    d = y.y - x.y;
+    acc = __builtin_fmaf(d, d, acc);
+    d = y.z - x.z;
+    acc = __builtin_fmaf(d, d, acc);
+    d = y.w - x.w;
+    return __builtin_fmaf(d, d, acc);
+  }
+  acc = __builtin_fmaf(y.x, x.x, acc);
+  acc = __builtin_fmaf(y.y, x.y, acc);
+  acc = __builtin_fmaf(y.z, x.z, acc);
+  return __builtin_fmaf(y.w, x.w, acc);
+}
+
+// The crate's combination of the 32 accumulators: s[l] = (a_l + a_{l+8}) +
+// (a_{l+16} + a_{l+24}), h = ((s0+s4)+(s1+s5))+((s2+s6)+(s3+s7))
+// (src/kernels/x86.rs:21-27, 70-73).
+#define HH_COMBINE32(acc, out)                                                                 \
+  do {                                                                                         \
+    float sl_[8];                                                                              \
+    _Pragma("unroll") for (int l_ = 0; l_ < 8; l_++) sl_[l_] =                                 \
+        __fadd_rn(__fadd_rn(acc[l_], acc[l_ + 8]), __fadd_rn(acc[l_ + 16], acc[l_ + 24]));    \
+    out = __fadd_rn(__fadd_rn(__fadd_rn(sl_[0], sl_[4]), __fadd_rn(sl_[1], sl_[5])),           \
+                    __fadd_rn(__fadd_rn(sl_[2], sl_[6]), __fadd_rn(sl_[3], sl_[7])));          \
+  } while (0)
+
+// One lane's whole distance sum of an exact-layout row (D = 32 * TMAX) in
+// the crate's order, the same bits as the quarter kernel's.
 template <int TMAX, bool L2, class E>
 __device__ __forceinline__ float lane_distance(const float *q, gptr<E> row) {
   float acc[32];
@@ -718,51 +766,91 @@ __device__ __forceinline__ float lane_distance(const float *q, gptr<E> row) {
 #pragma unroll
   for (int b = 0; b < TMAX / 4; b++) {
 #pragma unroll
-    for (int j = 0; j < 32; j++) {
-      float4 x;
-      if constexpr (sizeof(E) == 4) {
-        x = *(gptr<float4>)(row + b * 128 + 4 * j);
-      } else {
-        const uint2 h = *(gptr<uint2>)(row + b * 128 + 4 * j);
-        x = make_float4(__uint_as_float(h.x << 16), __uint_as_float(h.x & 0xFFFF0000u),
-                        __uint_as_float(h.y << 16), __uint_as_float(h.y & 0xFFFF0000u));
-      }
-      const float4 y = *reinterpret_cast<const float4 *>(q + b * 128 + 4 * j);
+    for (int j = 0; j < 32; j++) acc[j] = acc4<L2, E>(acc[j], q + b * 128, row + b * 128, j);
+  }
+  float h;
+  HH_COMBINE32(acc, h);
+  return h;
+}
+
+// The last, partial block of TW < 4 values per accumulator (value 32t + j
+// at j*TW + t%4).
+template <int TW, bool L2, class E>
+__device__ __forceinline__ void acc_partial(float (&acc)[32], const float *q, gptr<E> blk) {
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+#pragma unroll
+    for (int tt = 0; tt < TW; tt++) {
+      const float x = to_f<E>(blk[j * TW + tt]), y = q[j * TW + tt];
       if (L2) {
-        float d = y.x - x.x;
-        acc[j] = __builtin_fmaf(d, d, acc[j]);
-        d = y.y - x.y;
-        acc[j] = __builtin_fmaf(d, d, acc[j]);
-        d = y.z - x.z;
-        acc[j] = __builtin_fmaf(d, d, acc[j]);
-        d = y.w - x.w;
+        const float d = y - x;
         acc[j] = __builtin_fmaf(d, d, acc[j]);
       } else {
-        acc[j] = __builtin_fmaf(y.x, x.x, acc[j]);
-        acc[j] = __builtin_fmaf(y.y, x.y, acc[j]);
-        acc[j] = __builtin_fmaf(y.z, x.z, acc[j]);
-        acc[j] = __builtin_fmaf(y.w, x.w, acc[j]);
+        acc[j] = __builtin_fmaf(y, x, acc[j]);
       }
     }
   }
-  float sl[8];
+}
+
+// Any layout: the full 4-blocks, the last partial block (value 32t + j at
+// j*tw + t%4, tw = T mod 4), the 8-wide remainder into accumulators 0..7,
+// then the unfused tail products added after the combination, in order
+// (x86.rs:70-77, 121-129) -- the quarter kernel's order, element by element.
+template <bool L2, class E>
+__device__ __forceinline__ float lane_distance_any(const float *q, gptr<E> row, const Shape &s) {
+  float acc[32];
 #pragma unroll
-  for (int l = 0; l < 8; l++)
-    sl[l] = __fadd_rn(__fadd_rn(acc[l], acc[l + 8]), __fadd_rn(acc[l + 16], acc[l + 24]));
-  return __fadd_rn(__fadd_rn(__fadd_rn(sl[0], sl[4]), __fadd_rn(sl[1], sl[5])),
-                   __fadd_rn(__fadd_rn(sl[2], sl[6]), __fadd_rn(sl[3], sl[7])));
+  for (int j = 0; j < 32; j++) acc[j] = 0.0f;
+  const int full = s.T >> 2;
+  for (int b = 0; b < full; b++) {
+#pragma unroll
+    for (int j = 0; j < 32; j++) acc[j] = acc4<L2, E>(acc[j], q + b * 128, row + b * 128, j);
+  }
+  const int tw = s.T & 3;
+  if (tw == 1) acc_partial<1, L2, E>(acc, q + full * 128, row + full * 128);
+  if (tw == 2) acc_partial<2, L2, E>(acc, q + full * 128, row + full * 128);
+  if (tw == 3) acc_partial<3, L2, E>(acc, q + full * 128, row + full * 128);
+  const int rest = 32 * s.T;
+  for (int u = 0; u < s.U; u++) {
+#pragma unroll
+    for (int l = 0; l < 8; l++) {
+      const float x = to_f<E>(row[rest + 8 * u + l]), y = q[rest + 8 * u + l];
+      if (L2) {
+        const float d = y - x;
+        acc[l] = __builtin_fmaf(d, d, acc[l]);
+      } else {
+        acc[l] = __builtin_fmaf(y, x, acc[l]);
+      }
+    }
+  }
+  float h;
+  HH_COMBINE32(acc, h);
+  for (int r = 0; r < s.R; r++) {
+    const int pos = rest + 8 * s.U + r;
+    const float x = to_f<E>(row[pos]), y = q[pos];
+    if (L2) {
+      const float d = y - x;
+      h = __fadd_rn(h, __fmul_rn(d, d));
+    } else {
+      h = __fadd_rn(h, __fmul_rn(y, x));
+    }
+  }
+  return h;
 }
 
 // int8 codes: the exact integer sums of the crate's quantized distance
-// (src/kernels/int8.rs:13-48) in any order, rescaled as the search kernel
-// does (src/quantization.rs:85-92).
-template <int TMAX>
+// (src/kernels/int8.rs:13-48) in any order over the code row (zero padded to
+// cstride on both sides), rescaled as the search kernel does
+// (src/quantization.rs:85-92).
+// WORDS: the code row's uint2 count when known at compile time (exact
+// layouts: cstride = D), else 0 and cstride / 8 is read at run time.
+template <int WORDS>
 __device__ __forceinline__ float lane_distance_i8(const SearchArgs &P, const uint2 *qc,
                                                   gptr<uint2> row) {
-  constexpr int kWords = 32 * TMAX / 8;  // uint2 words of codes (cstride = D)
+  const int words = WORDS ? WORDS : (int)(P.cstride >> 3);
   int acc = 0;
 #pragma unroll 8
-  for (int c = 0; c < kWords; c++) {
+  for (int c = 0; c < words; c++) {
     const uint2 a = qc[c], b = row[c];
     if (P.metric == 0) {
       acc += __builtin_amdgcn_sdot4((int)a.x, (int)a.x, 0, false) -
@@ -784,34 +872,45 @@ __device__ __forceinline__ float lane_distance_i8(const SearchArgs &P, const uin
 
 // The traversal distance of the batch's precision (0 f32, 1 int8 codes,
 // 2 bf16 rows), with the bits the search kernel computes.
-template <int TMAX>
-__device__ __forceinline__ float descent_distance(const SearchArgs &P, const float *q,
-                                                  const uint2 *qc, uint32_t id) {
+template <int TMAX, bool EXACT>
+__device__ __forceinline__ float descent_distance(const SearchArgs &P, const Shape &shp,
+                                                  const float *q, const uint2 *qc, uint32_t id) {
   if (P.precision == 1)
-    return lane_distance_i8<TMAX>(P, qc, (gptr<uint2>)(P.codes + (size_t)id * P.cstride));
+    return lane_distance_i8<EXACT ? 4 * TMAX : 0>(
+        P, qc, (gptr<uint2>)(P.codes + (size_t)id * P.cstride));
   float s;
   if (P.precision == 2) {
     gptr<uint16_t> row = G(P.bf16) + (size_t)id * P.stride;
-    s = P.metric == 0 ? lane_distance<TMAX, true, uint16_t>(q, row)
-                      : lane_distance<TMAX, false, uint16_t>(q, row);
+    if constexpr (EXACT)
+      s = P.metric == 0 ? lane_distance<TMAX, true, uint16_t>(q, row)
+                        : lane_distance<TMAX, false, uint16_t>(q, row);
+    else
+      s = P.metric == 0 ? lane_distance_any<true, uint16_t>(q, row, shp)
+                        : lane_distance_any<false, uint16_t>(q, row, shp);
   } else {
     gptr<float> row = G(P.rows) + (size_t)id * P.stride;
-    s = P.metric == 0 ? lane_distance<TMAX, true, float>(q, row)
-                      : lane_distance<TMAX, false, float>(q, row);
+    if constexpr (EXACT)
+      s = P.metric == 0 ? lane_distance<TMAX, true, float>(q, row)
+                        : lane_distance<TMAX, false, float>(q, row);
+    else
+      s = P.metric == 0 ? lane_distance_any<true, float>(q, row, shp)
+                        : lane_distance_any<false, float>(q, row, shp);
   }
   return finish_metric(s, P.metric);
 }
 
-template <int TMAX>
-__global__ void __launch_bounds__(256) descent_kernel(const SearchArgs P) {
-  static_assert(TMAX % 4 == 0, "exact layout");
-  constexpr int kStride = 32 * TMAX;
-  __shared__ float qs[16][kStride];
-  __shared__ uint2 qcs[16][kStride / 8];
-  const int lane = lane_id();
-  const int i16 = lane & 15;
-  const int qslot = (int)(threadIdx.x >> 4);  // quarter within the block
-  const uint64_t qi = (uint64_t)blockIdx.x * 16 + qslot;
+template <int TMAX, int LPQ, bool EXACT>
+__global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const SearchArgs P) {
+  static_assert(!EXACT || TMAX % 4 == 0, "exact layout");
+  constexpr int kStride = descent_stride<TMAX>();
+  constexpr int kCodeWords = descent_code_words<TMAX>();
+  constexpr int kQueries = descent_threads<TMAX>() / LPQ;
+  __shared__ __align__(16) float qs[kQueries][kStride];
+  __shared__ __align__(16) uint2 qcs[kQueries][kCodeWords];
+  const int il = lane_id() & (LPQ - 1);
+  const int qslot = (int)(threadIdx.x / LPQ);
+  const uint64_t qi = (uint64_t)blockIdx.x * kQueries + qslot;
+  const Shape shp{P.T, P.U, P.R, P.stride};
   unsigned long long up = 0;
   if (qi < P.nq) {
     uint32_t k = P.k;
@@ -820,46 +919,47 @@ __global__ void __launch_bounds__(256) descent_kernel(const SearchArgs P) {
     if (status != 0 || k == 0 || P.live == 0 || P.top < 0) {
       GM(P.seed)[qi] = ((uint64_t)__float_as_uint(-INFINITY) << 32) | kInvalid;
     } else {
-      // the query (prepared, row layout) into this quarter's LDS slot
+      // the query (prepared, row layout) and its codes into this slot
       gptr<float4> src = (gptr<float4>)(P.qprep + qi * P.stride);
       float4 *dst = reinterpret_cast<float4 *>(qs[qslot]);
-      for (int c = i16; c < kStride / 4; c += 16) dst[c] = src[c];
+      for (int c = il; c < (int)(P.stride / 4); c += LPQ) dst[c] = src[c];
       if (P.precision == 1) {
         gptr<uint2> csrc = (gptr<uint2>)(P.qcode + qi * P.cstride);
-        for (int c = i16; c < kStride / 8; c += 16) qcs[qslot][c] = csrc[c];
+        for (int c = il; c < (int)(P.cstride / 8); c += LPQ) qcs[qslot][c] = csrc[c];
       }
       wave_sync();
       const float *q = qs[qslot];
       const uint2 *qc = qcs[qslot];
       uint32_t cur = P.entry;
       uint64_t cur_off = G(P.upper_off)[cur];
-      float best = descent_distance<TMAX>(P, q, qc, cur);
+      float best = descent_distance<TMAX, EXACT>(P, shp, q, qc, cur);
       up++;
       for (int layer = P.top; layer >= 1; layer--) {
         for (;;) {
           gptr<uint32_t> blk = G(P.upper) + cur_off + (size_t)(layer - 1) * (P.m + 1);
           uint32_t cnt = blk[0];
           if (cnt > P.m) cnt = P.m;
-          const bool valid = (uint32_t)i16 < cnt;
-          const uint32_t nid = valid ? blk[1 + i16] : kInvalid;
+          const bool valid = (uint32_t)il < cnt;
+          const uint32_t nid = valid ? blk[1 + il] : kInvalid;
           const uint64_t noff = valid ? (uint64_t)G(P.upper_off)[nid] : 0;
-          float md = valid ? descent_distance<TMAX>(P, q, qc, nid) : 0.0f;
-          int mi = valid ? i16 : 16;
+          float md = valid ? descent_distance<TMAX, EXACT>(P, shp, q, qc, nid) : 0.0f;
+          int mi = valid ? il : LPQ;
 #pragma unroll
-          for (int m = 1; m < 16; m <<= 1) {
-            const float od = __shfl_xor(md, m, 16);
-            const int oi = __shfl_xor(mi, m, 16);
-            const bool take = oi < 16 && (mi == 16 || od < md || (!(md < od) && oi < mi));
+          for (int m = 1; m < LPQ; m <<= 1) {
+            const float od = __shfl_xor(md, m, LPQ);
+            const int oi = __shfl_xor(mi, m, LPQ);
+            const bool take = oi < LPQ && (mi == LPQ || od < md || (!(md < od) && oi < mi));
             if (take) {
               md = od;
               mi = oi;
             }
           }
           up += cnt;
-          const uint32_t bn = (uint32_t)__shfl((int)nid, mi & 15, 16);
-          const uint32_t bo_lo = (uint32_t)__shfl((int)(uint32_t)noff, mi & 15, 16);
-          const uint32_t bo_hi = (uint32_t)__shfl((int)(uint32_t)(noff >> 32), mi & 15, 16);
-          if (mi < 16 && md < best) {
+          const int src_lane = mi & (LPQ - 1);
+          const uint32_t bn = (uint32_t)__shfl((int)nid, src_lane, LPQ);
+          const uint32_t bo_lo = (uint32_t)__shfl((int)(uint32_t)noff, src_lane, LPQ);
+          const uint32_t bo_hi = (uint32_t)__shfl((int)(uint32_t)(noff >> 32), src_lane, LPQ);
+          if (mi < LPQ && md < best) {
             best = md;
             cur = bn;
             cur_off = ((uint64_t)bo_hi << 32) | bo_lo;
@@ -871,8 +971,8 @@ __global__ void __launch_bounds__(256) descent_kernel(const SearchArgs P) {
       GM(P.seed)[qi] = ((uint64_t)__float_as_uint(best) << 32) | cur;
     }
   }
-  // one count per quarter: lane 0 of each quarter adds its query's total
-  if (i16 == 0 && up && P.stats) atomicAdd(&P.stats[2], up);
+  // one count per query: its first lane adds the query's total
+  if (il == 0 && up && P.stats) atomicAdd(&P.stats[2], up);
 }
 
 // Longest-first query order for the two-pass launch: a histogram of the
@@ -1005,8 +1105,15 @@ __global__ void __launch_bounds__(256) prepare_queries_kernel(const PrepArgs A) 
     prepare_one(A, qi, lane);
 }
 
-template __global__ void descent_kernel<4>(const SearchArgs);
-template __global__ void descent_kernel<8>(const SearchArgs);
+#define HH_DESCENT_INST(TM, XA)                                             \
+  template __global__ void descent_kernel<TM, 16, XA>(const SearchArgs);    \
+  template __global__ void descent_kernel<TM, 32, XA>(const SearchArgs);
+HH_DESCENT_INST(4, true)
+HH_DESCENT_INST(8, true)
+HH_DESCENT_INST(32, true)
+HH_DESCENT_INST(4, false)
+HH_DESCENT_INST(8, false)
+HH_DESCENT_INST(32, false)
 
 #define HH_SEARCH_INST(TM, EX, XA)                                                  \
   template __global__ void hnsw_search_kernel<TM, EX, XA, 1>(const SearchArgs);     \

@@ -85,10 +85,19 @@ constexpr uint32_t kOrderBins = 4096;
 
 template <int TMAX, bool EXTRA, bool EXACT, int SW>
 __global__ void hnsw_search_kernel(const SearchArgs P);
-// First pass of a two-pass launch for exact-layout rows (D = 32 * TMAX,
-// TMAX 4 or 8) and m <= 16: 16 queries per 256-thread block (grid
-// ceil(nq / 16)), each descent's seed written as by phase 1.
+// First pass of a two-pass launch for m <= LPQ (16 or 32 lanes per query):
+// descent_threads / LPQ queries per block (grid ceil(nq / that)), each
+// descent's seed written as by phase 1.  EXACT: D = 32 * TMAX.
+// Per query slot it holds the prepared query (row stride <= 32 * TMAX + 32
+// floats for any dim of that TMAX) and its codes (cstride <= that rounded
+// up to 128 bytes).
 template <int TMAX>
+constexpr int descent_threads() { return TMAX <= 8 ? 256 : 64; }
+template <int TMAX>
+constexpr int descent_stride() { return 32 * TMAX + 32; }
+template <int TMAX>
+constexpr int descent_code_words() { return (32 * TMAX + 32 + 127) / 128 * 16; }
+template <int TMAX, int LPQ, bool EXACT>
 __global__ void descent_kernel(const SearchArgs P);
 // One block of 1024 threads: order[] = the nq queries by descending
 // order_bin of their seed distance; resets the search kernel's query

@@ -67,6 +67,22 @@ def test_quantized_search_matches_oracle():
         assert st["n_rerank"] == cnt.n_rerank
 
 
+@pytest.mark.parametrize("dim", [130, 159, 200, 270, 287, 520, 1000])
+def test_int8_search_reads_every_code_chunk(dim):
+    """int8 traversal sums the whole code row, every 128-byte chunk of it:
+    dims whose codes run one chunk past 32 * (dim // 32) rounded up (129-159,
+    257-287) and the wide rows equal the oracle's quantized search
+    (regression: the 129-159 and 257-287 ranges once summed one chunk short)."""
+    print(f"case dim={dim}", flush=True)
+    for metric in (H.Metric.L2, H.Metric.Cosine):
+        b, q = O.dataset_gaussian(1500, 40, dim, 3 + dim)
+        idx = build(b, metric, threads=8)
+        idx.enable_quantization(on_device=True)
+        (gk, gd, gc), (ok, od, oc), st, cnt = _compare(idx, q, 10, 64, quantized=True)
+        assert np.array_equal(gk, ok) and same_bits(gd, od) and np.array_equal(gc, oc), metric
+        assert st["n_dist"] + st["n_upper"] == cnt.n_dist and st["n_rerank"] == cnt.n_rerank
+
+
 def test_filter_matches_oracle_and_is_respected():
     """src/hnsw/tests.rs:293-313"""
     b, q = O.dataset_gaussian(2000, 10, 8, 8)
@@ -408,6 +424,40 @@ def test_descent_kernel_two_pass_matches_one_pass(metric, dim):
     ok, od, oc, cnt = g.search_batch(q[:300], 10, 64)
     (k2, d2, c2), s2 = runs[True, 64]
     assert np.array_equal(k2[:300], ok) and same_bits(d2[:300], od)
+
+
+@pytest.mark.parametrize("dim,m,metric", [
+    (96, 24, H.Metric.L2), (100, 32, H.Metric.InnerProduct), (130, 16, H.Metric.Cosine),
+    (300, 8, H.Metric.L2), (960, 32, H.Metric.Cosine), (1024, 24, H.Metric.L2)])
+def test_descent_kernel_any_layout_and_m32(dim, m, metric):
+    """The descent kernel also serves rows with a partial last 4-block, an
+    8-wide remainder and a tail (dim 96, 100, 130, 300, 960) and m up to 32
+    (32 lanes per query): f32, bf16 and int8 two-pass launches equal the
+    one-pass launch (results and counters) and the oracle."""
+    print(f"case dim={dim} m={m} metric={metric}", flush=True)
+    b, q = O.dataset_gaussian(3000, 9000, dim, 7 + dim + m)
+    idx = build(b, metric, m=m, threads=8)
+    assert idx.graph_arrays()["top"] >= 1
+    idx.enable_quantization(on_device=True)
+    idx.enable_bf16()
+    for prec in (H.Precision.F32, H.Precision.BF16, H.Precision.INT8):
+        runs = {}
+        for longest_first in (False, True):
+            idx.set_query_order(longest_first)
+            idx.search_stats(reset=True)
+            runs[longest_first] = idx.search_arrays(q, 10, 48, precision=prec), idx.search_stats()
+        (k1, d1, c1), s1 = runs[False]
+        (k2, d2, c2), s2 = runs[True]
+        assert np.array_equal(c1, c2) and np.array_equal(k1, k2) and same_bits(d1, d2), prec
+        for f in ("queries", "n_dist", "n_upper", "n_expand", "n_rerank"):
+            assert s1[f] == s2[f], (prec, f)
+        if prec == H.Precision.F32:
+            f32 = k2, d2
+    g = oracle_graph(idx)
+    ok, od, oc, _ = g.search_batch(q[:200], 10, 48)
+    assert np.array_equal(f32[0][:200], ok) and same_bits(f32[1][:200], od)
+    ok, od, oc, _ = g.search_batch(q[:200], 10, 48, quantized=True)
+    assert np.array_equal(k2[:200], ok) and same_bits(d2[:200], od)
 
 
 @pytest.mark.parametrize("metric", [H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine])
