@@ -180,6 +180,13 @@ int slots_for(uint32_t ef) {
   return 0;
 }
 
+// Index of the list-size bucket of ef (ef <= 64, 128, 256, 512), the unit
+// the launch shape and order are tuned in; -1 above 512.
+int ef_bucket(uint32_t ef) {
+  const int sw = slots_for(ef);
+  return sw == 1 ? 0 : sw == 2 ? 1 : sw == 4 ? 2 : sw == 8 ? 3 : -1;
+}
+
 // Per-index workspace; one search runs in it at a time (ordered by `done`).
 struct Workspace {
   std::mutex mu;
@@ -407,6 +414,9 @@ struct hh_index {
   hipEvent_t t0 = nullptr, t1 = nullptr;
   // two-pass launches that start the longest queries first
   bool longest_first = true;
+  // per ef bucket, the order hh_index_autotune measured faster: 0 = the
+  // rule of thumb, 1 = one pass, 2 = two passes
+  int tuned_order[4] = {0, 0, 0, 0};
 
   ~hh_index() {
     (void)hipSetDevice(device);
@@ -725,14 +735,21 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   // do not depend on the order.
   const uint64_t waves = cfg.blocks * (uint64_t)cfg.wpb;
   // The descent pass runs as descent_kernel (one query per 16 or 32 lanes)
-  // where it applies: m <= 32 and one precision for the batch.
-  const bool fast_descent = idx->m <= 32 && !c.specs && !c.build;
-  // Measured at 1M x 128, 10k queries: with the descent kernel the order
-  // pays from ef = 32 (ef=64 +29%); with the search kernel as the descent
-  // pass, only above ef = 64.
+  // where it applies: m <= 32, search batches (not construction).
+  const bool fast_descent = idx->m <= 32 && !c.build;
+  // Rule of thumb, measured at 10k queries: on 1M x 128 clustered L2 rows
+  // (m = 16) the order pays from ef = 32 (ef=64 +29%); on 200k x 960
+  // cosine (m = 32) and 500k x 256 inner-product (m = 24) rows it is
+  // within -3%..+2% of one pass at ef 64 and 256.  hh_index_autotune
+  // replaces the rule by a measurement.
   const uint32_t efb = spec_ef_bound(c);
-  if (idx->longest_first && idx->top >= 1 && c.nq > waves && c.nq <= 16 * waves &&
-      ((fast_descent && efb > 16) || efb > 64)) {
+  const int tmax = tmax_for(idx->lay);
+  const bool narrow = idx->m <= 16 && idx->lay.U == 0 && idx->lay.R == 0 &&
+                      (int)idx->lay.T == tmax && tmax <= 8;
+  const int eb = ef_bucket(std::max<uint32_t>(efb, 1));
+  const int tuned = eb >= 0 ? idx->tuned_order[eb] : 0;
+  const bool two_pass = tuned ? tuned == 2 : ((fast_descent && narrow && efb > 16) || efb > 64);
+  if (idx->longest_first && idx->top >= 1 && c.nq > waves && c.nq <= 16 * waves && two_pass) {
     a.phase = 1;
     if (fast_descent) {
       if (int st = launch_descent(idx, c.nq, a, stream)) return st;
@@ -1404,24 +1421,31 @@ int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t
   int best_wpb = 0;
   const uint32_t caps[] = {1024, 1536, 1792, 2048, 3072, 4096, 6144, 8192, 12288, 16384};
   int st = HH_OK;
+  // best of three timed launches after a warm-up, in the current shape and order
+  auto time_launch = [&](float &ms_best) {
+    ms_best = 1e30f;
+    for (int rep = 0; rep < 4 && !st; rep++) {
+      if (hipEventRecord(e0, s) != hipSuccess) st = HH_DEVICE_ERROR;
+      if (!st)
+        st = launch_search(idx, queries, nq, k, ef, precision, nullptr, nullptr, k, nullptr, 0,
+                           out_keys, out_dist, out_count, status, s);
+      if (!st && hipEventRecord(e1, s) != hipSuccess) st = HH_DEVICE_ERROR;
+      float ms = 0;
+      if (!st && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = HH_DEVICE_ERROR;
+      if (rep > 0) ms_best = std::min(ms_best, ms);  // first run warms up
+    }
+  };
+  const int saved_order = idx->tuned_order[bucket];
+  idx->tuned_order[bucket] = 0;
   for (uint32_t cap : caps) {
     if (cap < 256u * (uint32_t)sw || cap * 4 < efmax * 16) continue;
     for (int w : {1, 2, 4}) {
       if ((uint64_t)lds_words_for(cap, sw) * 4 * (uint32_t)w > 65536) continue;
       idx->tuned_hash_cap[bucket] = cap;
       idx->tuned_wpb[bucket] = w;
-      float ms_best = 1e30f;
-      for (int rep = 0; rep < 3 && !st; rep++) {
-        if (hipEventRecord(e0, s) != hipSuccess) st = HH_DEVICE_ERROR;
-        if (!st)
-          st = launch_search(idx, queries, nq, k, ef, precision, nullptr, nullptr, k, nullptr, 0,
-                             out_keys, out_dist, out_count, status, s);
-        if (!st && hipEventRecord(e1, s) != hipSuccess) st = HH_DEVICE_ERROR;
-        float ms = 0;
-        if (!st && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-          st = HH_DEVICE_ERROR;
-        if (rep > 0) ms_best = std::min(ms_best, ms);  // first run warms up
-      }
+      float ms_best;
+      time_launch(ms_best);
       if (st) break;
       if (ms_best < best_ms) {
         best_ms = ms_best;
@@ -1431,10 +1455,23 @@ int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t
     }
     if (st) break;
   }
+  // then one pass against two in the chosen shape (when the order is on)
+  int best_order = 0;
+  if (!st && idx->longest_first) {
+    idx->tuned_hash_cap[bucket] = best_cap;
+    idx->tuned_wpb[bucket] = best_wpb;
+    float ms1 = 0, ms2 = 0;
+    idx->tuned_order[bucket] = 1;
+    time_launch(ms1);
+    idx->tuned_order[bucket] = 2;
+    if (!st) time_launch(ms2);
+    best_order = ms2 < ms1 ? 2 : 1;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   idx->tuned_hash_cap[bucket] = st ? 0 : best_cap;
   idx->tuned_wpb[bucket] = st ? 0 : best_wpb;
+  idx->tuned_order[bucket] = st ? saved_order : best_order;
   if (st) return st == HH_DEVICE_ERROR ? fail(st, "autotune launch failed") : st;
   if (chosen_hash_cap) *chosen_hash_cap = best_cap;
   if (chosen_wpb) *chosen_wpb = best_wpb;
@@ -1449,6 +1486,7 @@ int hh_index_set_launch_shape(hh_index *idx, uint32_t ef, uint32_t hash_cap, int
   if (hash_cap == 0) {
     idx->tuned_hash_cap[bucket] = 0;
     idx->tuned_wpb[bucket] = 0;
+    idx->tuned_order[bucket] = 0;
     return HH_OK;
   }
   if (wpb != 1 && wpb != 2 && wpb != 4)

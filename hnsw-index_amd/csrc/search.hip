@@ -873,13 +873,14 @@ __device__ __forceinline__ float lane_distance_i8(const SearchArgs &P, const uin
 // The traversal distance of the batch's precision (0 f32, 1 int8 codes,
 // 2 bf16 rows), with the bits the search kernel computes.
 template <int TMAX, bool EXACT>
-__device__ __forceinline__ float descent_distance(const SearchArgs &P, const Shape &shp,
-                                                  const float *q, const uint2 *qc, uint32_t id) {
-  if (P.precision == 1)
+__device__ __forceinline__ float descent_distance(const SearchArgs &P, uint32_t prec,
+                                                  const Shape &shp, const float *q,
+                                                  const uint2 *qc, uint32_t id) {
+  if (prec == 1)
     return lane_distance_i8<EXACT ? 4 * TMAX : 0>(
         P, qc, (gptr<uint2>)(P.codes + (size_t)id * P.cstride));
   float s;
-  if (P.precision == 2) {
+  if (prec == 2) {
     gptr<uint16_t> row = G(P.bf16) + (size_t)id * P.stride;
     if constexpr (EXACT)
       s = P.metric == 0 ? lane_distance<TMAX, true, uint16_t>(q, row)
@@ -913,8 +914,12 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
   const Shape shp{P.T, P.U, P.R, P.stride};
   unsigned long long up = 0;
   if (qi < P.nq) {
-    uint32_t k = P.k;
-    if (P.specs) k = G(P.specs)[qi].k;
+    uint32_t k = P.k, prec = P.precision;
+    if (P.specs) {
+      const hh_query_spec sp = G(P.specs)[qi];
+      k = sp.k;
+      prec = sp.precision;
+    }
     const int status = G(P.status)[qi];
     if (status != 0 || k == 0 || P.live == 0 || P.top < 0) {
       GM(P.seed)[qi] = ((uint64_t)__float_as_uint(-INFINITY) << 32) | kInvalid;
@@ -923,7 +928,7 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
       gptr<float4> src = (gptr<float4>)(P.qprep + qi * P.stride);
       float4 *dst = reinterpret_cast<float4 *>(qs[qslot]);
       for (int c = il; c < (int)(P.stride / 4); c += LPQ) dst[c] = src[c];
-      if (P.precision == 1) {
+      if (prec == 1) {
         gptr<uint2> csrc = (gptr<uint2>)(P.qcode + qi * P.cstride);
         for (int c = il; c < (int)(P.cstride / 8); c += LPQ) qcs[qslot][c] = csrc[c];
       }
@@ -932,7 +937,7 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
       const uint2 *qc = qcs[qslot];
       uint32_t cur = P.entry;
       uint64_t cur_off = G(P.upper_off)[cur];
-      float best = descent_distance<TMAX, EXACT>(P, shp, q, qc, cur);
+      float best = descent_distance<TMAX, EXACT>(P, prec, shp, q, qc, cur);
       up++;
       for (int layer = P.top; layer >= 1; layer--) {
         for (;;) {
@@ -942,7 +947,7 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
           const bool valid = (uint32_t)il < cnt;
           const uint32_t nid = valid ? blk[1 + il] : kInvalid;
           const uint64_t noff = valid ? (uint64_t)G(P.upper_off)[nid] : 0;
-          float md = valid ? descent_distance<TMAX, EXACT>(P, shp, q, qc, nid) : 0.0f;
+          float md = valid ? descent_distance<TMAX, EXACT>(P, prec, shp, q, qc, nid) : 0.0f;
           int mi = valid ? il : LPQ;
 #pragma unroll
           for (int m = 1; m < LPQ; m <<= 1) {

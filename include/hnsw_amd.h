@@ -270,9 +270,12 @@ int hh_index_download_codes(hh_index *idx, int8_t *codes);
  * one and at most 16 queries per resident wavefront runs in two passes,
  * the upper-layer descent of every query, then layer 0 with the queries of
  * the largest descent distance first, so that long queries do not start
- * last.  It applies from ef 32 for uniform f32 batches on rows of 128 or
- * 256 dimensions with m <= 16 (the descent then runs four queries per
- * wavefront), and above ef 64 otherwise.  0: one pass in query order. */
+ * last.  By default it applies from ef 32 to batches of one precision on
+ * rows of 128 or 256 dimensions with m <= 16, and above ef 64 otherwise;
+ * hh_index_autotune replaces that rule, per ef bucket, by whichever of one
+ * and two passes it measured faster.  The descent pass of a batch of one
+ * precision with m <= 32 runs 4 (m <= 16) or 2 queries per wavefront.
+ * 0: one pass in query order. */
 int hh_index_set_query_order(hh_index *idx, int longest_first);
 
 /* Search statistics accumulated by the kernels since the last reset. */
@@ -292,8 +295,9 @@ int hh_index_reset_stats(hh_index *idx);
  * (arguments as hh_search_batch_device) under each candidate visited-set
  * capacity (LDS per wavefront, hence wavefronts per CU) and wavefronts per
  * block, and keeps the fastest for this ef bucket (ef <= 64, 128, 256,
- * 512) on this index.  Synchronous.  Results of the timed runs are left in
- * the output buffers. */
+ * 512) on this index; with the longest-first order on, it then times one
+ * pass against two in that shape and keeps the faster order for the bucket.
+ * Synchronous.  Results of the timed runs are left in the output buffers. */
 int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, uint32_t ef,
                       uint32_t precision, uint64_t *out_keys, float *out_dist,
                       uint32_t *out_count, int32_t *status, void *stream,
@@ -301,7 +305,8 @@ int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t
 /* Sets the launch shape of an ef bucket directly (for example one found by
  * hh_index_autotune in an earlier process): visited-set capacity in
  * entries (>= 256 per 64 of ef, LDS per block <= 64 KiB) and waves per
- * block (1, 2 or 4).  hash_cap 0 restores the default choice. */
+ * block (1, 2 or 4).  hash_cap 0 restores the default choice (and the
+ * default query-order rule of the bucket). */
 int hh_index_set_launch_shape(hh_index *idx, uint32_t ef, uint32_t hash_cap, int wpb);
 /* Times the search kernel of every following launch with a pair of HIP
  * events on the launch stream (no reference counterpart: measurement). */

@@ -396,6 +396,42 @@ def test_longest_first_order_returns_what_one_pass_returns():
     assert (out[True][2][ks == 0] == 0).all()
 
 
+def test_autotune_keeps_results_in_every_shape_and_order():
+    """hh_index_autotune times the launch shapes, then one pass against two,
+    and keeps the fastest for the ef bucket: the tuned index returns what the
+    untuned one did, in plain and in spec batches, and a zero capacity
+    restores the defaults."""
+    import ctypes
+    from hnsw_amd import _lib as L
+    from hnsw_amd.device import Buffer
+    b, q = O.dataset_gaussian(4000, 20000, 24, 31)
+    idx = build(b, H.Metric.L2, m=8, threads=8)
+    nq, k, ef = q.shape[0], 10, 96
+    rng = np.random.default_rng(2)
+    ks, efs = rng.choice([1, 10], nq), rng.integers(10, 200, nq)
+    ref = idx.search_arrays(q, k, ef)
+    ref_mixed = idx.search_mixed(q, ks, efs)
+    h = idx.device_index()
+    qd = Buffer.from_array(np.ascontiguousarray(q, np.float32))
+    bk, bd, bc, bs = Buffer(nq * k * 8), Buffer(nq * k * 4), Buffer(nq * 4), Buffer(nq * 4)
+    cap, wpb = ctypes.c_uint32(), ctypes.c_int()
+    print("case autotune", flush=True)
+    assert L.lib.hh_index_autotune(h, qd.ptr, nq, k, ef, 0, bk.ptr, bd.ptr, bc.ptr, bs.ptr, None,
+                                   ctypes.byref(cap), ctypes.byref(wpb)) == 0, L.last_error()
+    assert cap.value >= 512 and wpb.value in (1, 2, 4)
+    tk = bk.download((nq, k), np.uint64)
+    assert np.array_equal(tk, ref[0])
+    got = idx.search_arrays(q, k, ef)
+    assert np.array_equal(got[0], ref[0]) and same_bits(got[1], ref[1])
+    assert np.array_equal(got[2], ref[2])
+    got_mixed = idx.search_mixed(q, ks, efs)
+    for a, b_ in zip(ref_mixed, got_mixed):
+        assert np.array_equal(a.view(np.uint8), b_.view(np.uint8))
+    assert L.lib.hh_index_set_launch_shape(h, ef, 0, 0) == 0
+    got = idx.search_arrays(q, k, ef)
+    assert np.array_equal(got[0], ref[0]) and same_bits(got[1], ref[1])
+
+
 @pytest.mark.parametrize("metric", [H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine])
 @pytest.mark.parametrize("dim", [128, 256])
 def test_descent_kernel_two_pass_matches_one_pass(metric, dim):
