@@ -93,6 +93,10 @@ def main():
     ap.add_argument("--launch-shape", default="",
                     help="CAP,WPB: visited-set capacity and waves per block instead of the "
                          "autotuner's choice")
+    ap.add_argument("--config", choices=["c3", "c4", "c5"], default="",
+                    help="measure one of the other workloads instead of the headline: c3 "
+                         "1M x 960 cosine M=32 in f32/int8/bf16, c4 2M x 256 inner product "
+                         "M=24 filtered mixed batches, c5 10M x 96 L2 in bf16/int8 + serving")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each query is "
@@ -108,6 +112,8 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     threads = args.threads or min(16, os.cpu_count() or 1)
+    if args.config:
+        return run_config(args, threads)
     efs = [int(x) for x in args.efs.split(",")]
     n, nq, dim, k = args.n, args.nq, args.dim, args.k
 
@@ -421,6 +427,242 @@ def main():
         log(f"device-assisted build unavailable: {e}")
 
     emit(args, locals())
+
+
+def build_index(H, data, metric, m, threads):
+    n = data.base_len()
+    idx = H.HnswIndex(data.dim(), metric, H.HnswParams(m))
+    keys = np.arange(n, dtype=np.uint64)
+    t0 = time.time()
+    chunk = 200_000
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        idx.insert_batch(keys[s:e], data.base()[s:e], threads=threads)
+        log(f"graph build: {e}/{n} linked ({time.time() - t0:.1f} s, {threads} threads)")
+    return idx, keys, time.time() - t0
+
+
+class DeviceBatch:
+    """Device copies of a query set and result buffers for k_stride results."""
+
+    def __init__(self, H, queries, k_stride):
+        from hnsw_amd.device import Buffer, Stream
+        self.nq, self.k_stride = queries.shape[0], k_stride
+        self.stream = Stream()
+        self.q = Buffer.from_array(np.ascontiguousarray(queries, np.float32))
+        nq = self.nq
+        self.ok, self.od = Buffer(nq * k_stride * 8), Buffer(nq * k_stride * 4)
+        self.oc, self.st = Buffer(nq * 4), Buffer(nq * 4)
+
+    def results(self, k=0):
+        """Keys, distances and counts of the last launch; a uniform launch
+        packs its results at stride k, a mixed one at k_stride."""
+        k = k or self.k_stride
+        self.stream.synchronize()
+        return (self.ok.download((self.nq, k), np.uint64),
+                self.od.download((self.nq, k), np.float32),
+                self.oc.download(self.nq, np.uint32))
+
+
+def qps_of(launch, nq, reps=5):
+    from hnsw_amd.device import synchronize
+    launch()
+    synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        launch()
+    synchronize()
+    return nq * reps / (time.perf_counter() - t)
+
+
+def run_config(args, threads):
+    """The other workloads of the query path, each printed as one JSON line:
+    recall against the exact GPU brute force (spot-checked against FP64) and
+    throughput with queries resident in HBM."""
+    import hnsw_amd as H
+    from hnsw_amd import _lib as L
+    from hnsw_amd.device import Buffer
+    out = {"metric": f"query-path measurements, workload {args.config}", "n_gpus": 1,
+           "data": "synthetic", "higher_is_better": True, "unit": "queries/s"}
+
+    def uniform_search(idx, batch, k, ef, prec):
+        h = idx.device_index()
+
+        def launch():
+            if H.lib.hh_search_batch_device(h, batch.q.ptr, batch.nq, k, ef, int(prec), None,
+                                            batch.ok.ptr, batch.od.ptr, batch.oc.ptr,
+                                            batch.st.ptr, batch.stream.handle):
+                raise RuntimeError(L.last_error())
+        return launch
+
+    def truth_of(data, metric, k):
+        bf = H.BruteForceIndex(data.dim(), metric)
+        bf.insert_batch(np.arange(data.base_len(), dtype=np.uint64), data.base())
+        t = time.time()
+        tk, td, tc = bf.search_arrays(data.queries(), k)
+        log(f"ground truth: {data.query_len()} queries, k={k}, in {time.time() - t:.2f} s")
+        nf = min(200, data.query_len())
+        fk, fd, fc = bf.search_arrays(data.queries()[:nf], 10, f64=True)
+        agree = float(np.mean([len(set(tk[i, :10].tolist()) & set(fk[i].tolist())) / 10
+                               for i in range(nf)]))
+        return bf, tk, tc, fk, fd, agree
+
+    if args.config in ("c3", "c5"):
+        if args.config == "c3":
+            n, nq, dim, m, metric = args.n, 10_000, 960, 32, H.Metric.Cosine
+            log(f"generating {n} x {dim} U(0,1) rows, L2-normalised, + {nq} queries")
+            data = H.synthetic.Dataset.unit_uniform(n, nq, dim, 42, threads=threads)
+            ks, efs = (10, 100), (64, 256)
+        else:
+            n = 10_000_000 if args.n == 1_000_000 else args.n
+            nq, dim, m, metric = 10_000, 96, 16, H.Metric.L2
+            log(f"generating {n} x {dim} rows in 8 clusters (sigma 0.3) + {nq} queries")
+            data = H.synthetic.Dataset.gauss_clusters(n, nq, dim, 8, 0.3, 42, threads=threads)
+            ks, efs = (10,), (64,)
+        idx, keys, build_s = build_index(H, data, metric, m, threads)
+        bf, tk, tc, fk, fd, agree = truth_of(data, metric, max(ks))
+        del bf
+        idx.device_index()
+        t = time.time()
+        idx.enable_quantization(on_device=True)
+        quant_s = time.time() - t
+        idx.enable_bf16()
+        batch = DeviceBatch(H, data.queries(), max(ks))
+        rows = []
+        for prec in (H.Precision.F32, H.Precision.INT8, H.Precision.BF16):
+            for k in ks:
+                for ef in efs:
+                    launch = uniform_search(idx, batch, k, ef, prec)
+                    launch()
+                    gk, gd, gc = batch.results(k)
+                    rec = H.lib.hh_recall_at_k(nq, tk.ctypes.data, tc.ctypes.data, tk.shape[1],
+                                               gk.ctypes.data, gc.ctypes.data, k, k)
+                    # returned distances against the FP64 oracle's, same keys
+                    err = 0.0
+                    for i in range(fk.shape[0]):
+                        want = dict(zip(fk[i].tolist(), fd[i].tolist()))
+                        for j in range(min(int(gc[i]), 10)):
+                            if int(gk[i, j]) in want:
+                                err = max(err, abs(float(gd[i, j]) - want[int(gk[i, j])]))
+                    q = qps_of(launch, nq)
+                    rows.append({"precision": prec.name.lower(), "k": k, "ef": ef,
+                                 "recall": round(rec, 4), "qps": round(q),
+                                 "max_abs_dist_err_vs_f64": float(f"{err:.3g}")})
+                    log(f"{prec.name.lower()} k={k} ef={ef}: recall@{k}={rec:.4f} "
+                        f"qps={q:,.0f} max |d - d_f64| {err:.3g}")
+        out.update({"value": max(r["qps"] for r in rows if r["precision"] == "f32"),
+                    "dtype": "f32 / int8 / bf16", "results": rows,
+                    "config": {"workload": f"hnsw search_batch, {n} x {dim} "
+                                           f"{'cosine' if metric == H.Metric.Cosine else 'L2'}, "
+                                           f"M={m}, efC=200, {nq} queries",
+                               "graph_build_s": round(build_s, 1),
+                               "quantizer_train_encode_s": round(quant_s, 3),
+                               "truth_vs_f64_agreement": round(agree, 5)}})
+        if args.config == "c5":
+            serve = []
+            qs = data.queries()
+            for rate in (50_000.0, 200_000.0, 500_000.0):
+                for prec in (H.Precision.BF16, H.Precision.INT8):
+                    nreq = 50_000
+                    rng = np.random.default_rng(7)
+                    arrivals = np.cumsum(rng.exponential(1e6 / rate, nreq))
+                    with H.Server(idx, k_max=10, max_inflight=2) as srv:
+                        t = time.perf_counter()
+                        lat, sk, sd, sc, sst = srv.replay(
+                            qs[np.arange(nreq) % nq], np.full(nreq, 10), np.full(nreq, 64),
+                            arrivals, precisions=np.full(nreq, int(prec)))
+                        wall = time.perf_counter() - t
+                    rec = H.lib.hh_recall_at_k(nq, tk.ctypes.data, tc.ctypes.data, tk.shape[1],
+                                               np.ascontiguousarray(sk[:nq]).ctypes.data,
+                                               np.ascontiguousarray(sc[:nq]).ctypes.data, 10, 10)
+                    row = {"offered_qps": int(rate), "precision": prec.name.lower(),
+                           "achieved_qps": round(nreq / wall),
+                           "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                           "recall_at_10": round(rec, 4)}
+                    serve.append(row)
+                    log(f"serving: {row}")
+            out["serving"] = serve
+        print(json.dumps(out), flush=True)
+        return
+
+    # c4: filtered mixed batches
+    n = 2_000_000 if args.n == 1_000_000 else args.n
+    nq, dim, m, cats = 50_000, 256, 24, 20
+    log(f"generating {n} x {dim} N(0,1) rows with lognormal(0,0.5) norms, {cats} categories")
+    data = H.synthetic.Dataset.lognormal(n, nq, dim, 0.5, cats, 42, threads=threads)
+    idx, keys, build_s = build_index(H, data, H.Metric.InnerProduct, m, threads)
+    h = idx.device_index()
+    rng = np.random.default_rng(11)
+    nfilt = 64
+    masks, pass_rates = [], []
+    for f in range(nfilt):
+        chosen = rng.choice(cats, rng.integers(1, 4), replace=False)
+        mask = np.isin(data.categories, chosen)
+        masks.append(mask)
+        pass_rates.append(float(mask.mean()))
+    words = (n + 31) // 32
+    table = np.zeros((nfilt, words), np.uint32)
+    for f, mask in enumerate(masks):
+        ids = np.nonzero(mask)[0]
+        np.bitwise_or.at(table[f], ids >> 5, (np.uint32(1) << (ids & 31).astype(np.uint32)))
+    kq = rng.choice([1, 10, 100], nq).astype(np.uint32)
+    efq = np.maximum(rng.integers(32, 513, nq), kq).astype(np.uint32)
+    fq = rng.integers(0, nfilt, nq).astype(np.uint32)
+    specs = np.stack([kq, efq, np.zeros(nq, np.uint32), fq], axis=1).astype(np.uint32)
+    batch = DeviceBatch(H, data.queries(), 100)
+    s_dev, f_dev = Buffer.from_array(specs), Buffer.from_array(table)
+
+    def launch():
+        if H.lib.hh_search_batch_spec_device(h, batch.q.ptr, nq, s_dev.ptr, 100, 512, f_dev.ptr,
+                                             nfilt, batch.ok.ptr, batch.od.ptr, batch.oc.ptr,
+                                             batch.st.ptr, batch.stream.handle):
+            raise RuntimeError(L.last_error())
+    launch()
+    gk, gd, gc = batch.results()
+    q_direct = qps_of(launch, nq, 3)
+    plan = ctypes.c_void_p()
+    if H.lib.hh_search_plan_create(h, batch.q.ptr, nq, 100, 512, 0, None, s_dev.ptr, 100,
+                                   f_dev.ptr, nfilt, batch.ok.ptr, batch.od.ptr, batch.oc.ptr,
+                                   batch.st.ptr, ctypes.byref(plan)):
+        raise RuntimeError(L.last_error())
+
+    def replay():
+        if H.lib.hh_search_plan_launch(plan, batch.stream.handle):
+            raise RuntimeError(L.last_error())
+    q_plan = qps_of(replay, nq, 3)
+    pk, pd, pc = batch.results()
+    H.lib.hh_search_plan_free(plan)
+    same = bool(np.array_equal(pk, gk) and np.array_equal(pc, gc))
+    log(f"filtered mixed batch: {q_direct:,.0f} QPS direct, {q_plan:,.0f} QPS graph replay "
+        f"(identical: {same})")
+    # recall against the filtered exact brute force, query by query at its own k
+    bf = H.BruteForceIndex(dim, H.Metric.InnerProduct)
+    bf.insert_batch(keys, data.base())
+    hits = expected = 0
+    t = time.time()
+    for f in range(nfilt):
+        sel = np.nonzero(fq == f)[0]
+        if sel.size == 0:
+            continue
+        tk, td, tc = bf.search_arrays(data.queries()[sel], 100, filter=masks[f])
+        for r, i in enumerate(sel):
+            kk = int(kq[i])
+            tn, fn = min(int(tc[r]), kk), min(int(gc[i]), kk)
+            expected += tn
+            hits += len(set(tk[r, :tn].tolist()) & set(gk[i, :fn].tolist()))
+    rec = hits / max(expected, 1)
+    log(f"recall at each query's k vs filtered brute force: {rec:.4f} ({time.time() - t:.1f} s)")
+    out.update({"value": round(q_plan), "dtype": "f32",
+                "results": {"direct_launch_qps": round(q_direct), "graph_replay_qps": round(q_plan),
+                            "recall_at_k": round(rec, 4), "replay_identical": same,
+                            "filter_pass_rate_range": [round(min(pass_rates), 3),
+                                                       round(max(pass_rates), 3)]},
+                "config": {"workload": f"hnsw mixed filtered batch, {n} x {dim} inner product, "
+                                       f"M={m}, efC=200, {nq} queries, k in {{1,10,100}}, "
+                                       "ef in [32,512], 64 bitmaps of 1-3 of 20 categories",
+                           "graph_build_s": round(build_s, 1)}})
+    print(json.dumps(out), flush=True)
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "search_c2_1m128_traffic.md")

@@ -191,4 +191,10 @@ __global__ void __launch_bounds__(256) sq_encode_kernel(SqArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(64) pack_codes_kernel(const int8_t *src, uint32_t cstride,
+                                                        uint32_t dim, uint64_t rows, int8_t *dst) {
+  for (uint64_t r = blockIdx.x; r < rows; r += gridDim.x)
+    for (uint32_t c = threadIdx.x; c < dim; c += 64) dst[r * dim + c] = src[r * cstride + c];
+}
+
 }  // namespace hhk

@@ -23,5 +23,9 @@ __global__ void sq_mean_kernel(SqArgs a);
 // any grid of 256-thread blocks
 __global__ void sq_max_kernel(SqArgs a);
 __global__ void sq_encode_kernel(SqArgs a);
+// Packs rows x dim code bytes from rows of cstride bytes into dst (rows x
+// dim, dense): one 64-thread block per row, any grid.
+__global__ void pack_codes_kernel(const int8_t *src, uint32_t cstride, uint32_t dim,
+                                  uint64_t rows, int8_t *dst);
 
 }  // namespace hhk

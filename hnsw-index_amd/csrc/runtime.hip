@@ -1351,8 +1351,27 @@ int hh_index_download_codes(hh_index *idx, int8_t *codes) {
   if (!idx->codes) return fail(HH_UNSUPPORTED, "no quantizer codes resident");
   if (idx->n == 0) return HH_OK;
   HH_HIP(hipSetDevice(idx->device));
-  HH_HIP(hipMemcpy2D(codes, idx->dim, idx->codes, idx->cstride, idx->dim, idx->n,
-                     hipMemcpyDeviceToHost));
+  if (int st = drain(idx)) return st;
+  // rows are cstride bytes apart on the device: pack chunks of them densely
+  // there (one kernel), then copy each chunk in one piece
+  const uint64_t chunk = std::min<uint64_t>(idx->n, std::max<uint64_t>(1, (64ull << 20) / idx->dim));
+  int8_t *stage = nullptr;
+  HH_HIP(hipMalloc((void **)&stage, chunk * idx->dim));
+  hipError_t e = hipSuccess;
+  for (uint64_t r0 = 0; r0 < idx->n && e == hipSuccess; r0 += chunk) {
+    const uint64_t rows = std::min(chunk, idx->n - r0);
+    hipLaunchKernelGGL(hhk::pack_codes_kernel, dim3((unsigned)std::min<uint64_t>(rows, 65536)),
+                       dim3(64), 0, idx->stream, idx->codes + r0 * idx->cstride, idx->cstride,
+                       idx->dim, rows, stage);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(codes + r0 * idx->dim, stage, rows * idx->dim, hipMemcpyDeviceToHost,
+                         idx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(idx->stream);
+  }
+  dfree(stage);
+  if (e != hipSuccess)
+    return fail(HH_DEVICE_ERROR, std::string("code download: ") + hipGetErrorString(e));
   return HH_OK;
 }
 

@@ -704,17 +704,19 @@ hnsw_search_kernel(const SearchArgs P) {
 }
 
 // ------------------------------------------------------ descent pass
-// The first pass of a two-pass launch when rows have the exact layout
-// (D = 32 * TMAX) and m <= 16: four queries per wavefront, one per 16-lane
-// quarter, one neighbour row per lane.  A lane computes a whole distance
-// with the AVX2 kernel's 32 accumulators (value 32t + j into accumulator j,
-// one fused multiply-add per element in t order, x86.rs:47-69 / 96-120)
-// and its lane combination s[l] = (a_l + a_l+8) + (a_l+16 + a_l+24),
-// h = ((s0+s4)+(s1+s5))+((s2+s6)+(s3+s7)) (x86.rs:21-27), so every
-// distance has the bits of the search kernel's quarter reduction.  The
-// greedy steps are those of search_graph / greedy_closest
+// The first pass of a two-pass launch when m <= 32: one query per 16 lanes
+// (m <= 16, four per wavefront) or 32 lanes (two per wavefront), one
+// neighbour row per lane.  A lane computes a whole distance with the AVX2
+// kernel's 32 accumulators (value 32t + j into accumulator j, one fused
+// multiply-add per element in t order, the 8-wide remainder into
+// accumulators 0..7, x86.rs:47-77 / 96-129) and its lane combination
+// s[l] = (a_l + a_l+8) + (a_l+16 + a_l+24),
+// h = ((s0+s4)+(s1+s5))+((s2+s6)+(s3+s7)) (x86.rs:21-27), then the tail
+// products, so every distance has the bits of the search kernel's quarter
+// reduction.  The greedy steps are those of search_graph / greedy_closest
 // (src/hnsw/search.rs:49-68,146-161): the first list position holding the
 // smallest distance, taken when strictly below the best.
+
 // Accumulator j's four fused multiply-adds of one full 4-block (values
 // 32t + j, t in 4b..4b+3, at j*4 + t%4), in t order.
 template <bool L2, class E>
