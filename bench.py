@@ -533,8 +533,10 @@ def run_config(args, threads):
             for k in ks:
                 for ef in efs:
                     launch = uniform_search(idx, batch, k, ef, prec)
+                    idx.search_stats(reset=True)
                     launch()
                     gk, gd, gc = batch.results(k)
+                    sts = idx.search_stats()
                     rec = H.lib.hh_recall_at_k(nq, tk.ctypes.data, tc.ctypes.data, tk.shape[1],
                                                gk.ctypes.data, gc.ctypes.data, k, k)
                     # returned distances against the FP64 oracle's, same keys
@@ -547,7 +549,10 @@ def run_config(args, threads):
                     q = qps_of(launch, nq)
                     rows.append({"precision": prec.name.lower(), "k": k, "ef": ef,
                                  "recall": round(rec, 4), "qps": round(q),
-                                 "max_abs_dist_err_vs_f64": float(f"{err:.3g}")})
+                                 "max_abs_dist_err_vs_f64": float(f"{err:.3g}"),
+                                 "dist_evals_per_query": round(
+                                     (sts["n_dist"] + sts["n_upper"]) / nq, 1),
+                                 "expansions_per_query": round(sts["n_expand"] / nq, 1)})
                     log(f"{prec.name.lower()} k={k} ef={ef}: recall@{k}={rec:.4f} "
                         f"qps={q:,.0f} max |d - d_f64| {err:.3g}")
         out.update({"value": max(r["qps"] for r in rows if r["precision"] == "f32"),
@@ -618,9 +623,15 @@ def run_config(args, threads):
                                              nfilt, batch.ok.ptr, batch.od.ptr, batch.oc.ptr,
                                              batch.st.ptr, batch.stream.handle):
             raise RuntimeError(L.last_error())
+    idx.search_stats(reset=True)
     launch()
     gk, gd, gc = batch.results()
+    sts = idx.search_stats()
     q_direct = qps_of(launch, nq, 3)
+    per_q = {f: round(sts[f] / nq, 1) for f in ("n_dist", "n_upper", "n_expand", "n_overflow",
+                                                  "n_visited_spill", "n_cand_spill")}
+    log(f"per query: {per_q}; gathered rows at the direct rate "
+        f"{(sts['n_dist'] + sts['n_upper']) * dim * 4 * q_direct / nq / 1e9:.0f} GB/s")
     plan = ctypes.c_void_p()
     if H.lib.hh_search_plan_create(h, batch.q.ptr, nq, 100, 512, 0, None, s_dev.ptr, 100,
                                    f_dev.ptr, nfilt, batch.ok.ptr, batch.od.ptr, batch.oc.ptr,
@@ -656,6 +667,7 @@ def run_config(args, threads):
     out.update({"value": round(q_plan), "dtype": "f32",
                 "results": {"direct_launch_qps": round(q_direct), "graph_replay_qps": round(q_plan),
                             "recall_at_k": round(rec, 4), "replay_identical": same,
+                            "per_query": per_q,
                             "filter_pass_rate_range": [round(min(pass_rates), 3),
                                                        round(max(pass_rates), 3)]},
                 "config": {"workload": f"hnsw mixed filtered batch, {n} x {dim} inner product, "

@@ -79,10 +79,12 @@ __global__ void to_bf16_kernel(const float *src, uint16_t *dst, uint64_t count) 
   }
 }
 
-// Candidates a query may hold beyond its register list (8 bytes each, per
-// resident wavefront).  Only a query that visits more than this many
-// not-yet-expanded nodes drops candidates (counted in n_overflow).
-constexpr uint32_t kCandPool = 8192;
+// Candidates a query may hold beyond its LDS list (8 bytes each, 512 KiB
+// per resident wavefront).  Only a query with more live, not-yet-expanded
+// candidates than this drops some (counted in n_overflow).  A selective
+// filter keeps about ef / pass-rate of them live until its results fill:
+// 10,240 at ef 512 and 5%, and 8,192 entries overflowed there.
+constexpr uint32_t kCandPool = 65536;
 // Nodes a query may mark in its HBM visited bitset (after leaving the LDS
 // table) and still clear word by word at its end; beyond, the whole bitset.
 constexpr uint32_t kSpillLog = 4096;

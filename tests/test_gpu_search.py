@@ -83,6 +83,29 @@ def test_int8_search_reads_every_code_chunk(dim):
         assert st["n_dist"] + st["n_upper"] == cnt.n_dist and st["n_rerank"] == cnt.n_rerank
 
 
+def test_one_percent_filter_at_ef_512_keeps_every_candidate():
+    """A 1% filter at ef=512 keeps about ef / pass-rate (tens of thousands)
+    candidates live until its results fill, far past the LDS list: the
+    per-wave HBM pool holds them all (n_overflow 0), so keys, distances and
+    expansions equal the oracle's unbounded heap (regression: an 8,192-entry
+    pool dropped candidates on such queries)."""
+    n, dim = 60000, 16
+    d = H.synthetic.Dataset.lognormal(n, 4, dim, 0.5, 100, 17)
+    b, q, cat = d.base(), d.queries(), d.categories
+    idx = build(b, H.Metric.InnerProduct, m=8, threads=8)
+    g = oracle_graph(idx)
+    allow = cat == 7
+    assert 0.005 < allow.mean() < 0.02
+    idx.search_stats(reset=True)
+    gk, gd, gc = idx.search_arrays(q, 100, 512, filter=allow)
+    st = idx.search_stats()
+    ok, od, oc, cnt = g.search_batch(q, 100, 512, allow=allow.astype(np.uint8))
+    print(f"case 1% filter: {st}", flush=True)
+    assert st["n_overflow"] == 0 and st["n_cand_spill"] > 8192, st
+    assert st["n_expand"] == cnt.n_expand
+    assert np.array_equal(gc, oc) and np.array_equal(gk, ok) and same_bits(gd, od)
+
+
 def test_filter_matches_oracle_and_is_respected():
     """src/hnsw/tests.rs:293-313"""
     b, q = O.dataset_gaussian(2000, 10, 8, 8)
