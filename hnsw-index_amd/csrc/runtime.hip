@@ -609,8 +609,10 @@ int launch_descent(const hh_index *idx, uint64_t nq, const hhk::SearchArgs &a,
                    hipStream_t stream) {
   using K = void (*)(const hhk::SearchArgs);
   const int tmax = tmax_for(idx->lay);
+  // (cstride is 0 until a quantizer exists; with one it is dim rounded up to
+  // 128, which the exact instances take as a compile-time length)
   const bool exact = idx->lay.U == 0 && idx->lay.R == 0 && (int)idx->lay.T == tmax &&
-                     idx->cstride == 32u * (uint32_t)tmax;
+                     (idx->cstride == 0 || idx->cstride == 32u * (uint32_t)tmax);
   const int wide = idx->m > 16;
   const int ti = tmax == 4 ? 0 : tmax == 8 ? 1 : 2;
   static const K table[3][2][2] = {
