@@ -191,6 +191,9 @@ __device__ __forceinline__ void gather_round(const SearchArgs &P, const QLanes<T
   const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
   const Shape s{P.T, P.U, P.R, P.stride};
   constexpr int B = rows_per_quarter<TMAX>();
+  // groups of four rows (one per quarter) this round needs: a group wholly
+  // past cnt is skipped (wave-uniform), so at most three slots are wasted
+  const int groups = (cnt - base + 3) >> 2;
   {
     uint32_t rid[B];
 #pragma unroll
@@ -207,10 +210,11 @@ __device__ __forceinline__ void gather_round(const SearchArgs &P, const QLanes<T
         x[b].e[0] = x[b].r = 0.0f;
         if constexpr (EXTRA) x[b].e[1] = x[b].e[2] = 0.0f;
       }
-      load_q<TMAX, EXTRA, E, EXACT>(rows + (size_t)rid[b] * P.stride, s, i, x[b]);
+      if (b == 0 || b < groups) load_q<TMAX, EXTRA, E, EXACT>(rows + (size_t)rid[b] * P.stride, s, i, x[b]);
     }
 #pragma unroll
     for (int b = 0; b < B; b++) {
+      if (b > 0 && b >= groups) break;
       const int row = base + 4 * b + quarter;
       float lo, hi, tail;
       if (P.metric == 0) quarter_acc<TMAX, EXTRA, true, EXACT>(q, x[b], s, i, lo, hi, tail);
@@ -257,6 +261,7 @@ __device__ __forceinline__ void gather_dists(const SearchArgs &P, const QLanes<T
     const int chunks = (int)(P.cstride >> 7);
     const float s2 = __fmul_rn(P.sq_scale, P.sq_scale);
     for (int base = 0; base < cnt; base += 16) {
+      const int groups = (cnt - base + 3) >> 2;  // four-row groups needed (wave-uniform)
       uint2 cb[4][CC];
 #pragma unroll
       for (int b = 0; b < 4; b++) {
@@ -264,10 +269,12 @@ __device__ __forceinline__ void gather_dists(const SearchArgs &P, const QLanes<T
         gptr<uint2> cr = (gptr<uint2>)(P.codes + (size_t)ids[row < cnt ? row : cnt - 1] * P.cstride);
 #pragma unroll
         for (int c = 0; c < CC; c++)
-          cb[b][c] = (EXACT || c < chunks) ? (uint2)cr[c * 16 + i] : make_uint2(0u, 0u);
+          cb[b][c] = (b == 0 || b < groups) && (EXACT || c < chunks) ? (uint2)cr[c * 16 + i]
+                                                                   : make_uint2(0u, 0u);
       }
 #pragma unroll
       for (int b = 0; b < 4; b++) {
+        if (b > 0 && b >= groups) break;
         const int row = base + 4 * b + quarter;
         int acc = 0;
 #pragma unroll
@@ -577,6 +584,26 @@ hnsw_search_kernel(const SearchArgs P) {
         // at the batch start can never pass.)
         const uint64_t sm = __ballot(lane < nn && (wsize < ef || myd < worst));
         const bool surv = (sm >> lane) & 1;
+        if (last && pf_ok && sm) {
+          // the smallest surviving neighbour (accepted below unless ties
+          // fill W first), if it beats C's front, is the next candidate:
+          // fetch its adjacency row during the acceptance and the merges
+          // (a hint only: the smallest distance by row DPP and the four
+          // row minima, its lowest lane on ties)
+          float dm = surv ? myd : INFINITY;
+          dm = fminf(dm, dpp<kRor8>(dm));
+          dm = fminf(dm, dpp<kRor4>(dm));
+          dm = fminf(dm, dpp<kXor1>(dm));
+          dm = fminf(dm, dpp<kXor2>(dm));
+          dm = fminf(fminf(rlf(dm, 0), rlf(dm, 16)), fminf(rlf(dm, 32), rlf(dm, 48)));
+          const uint64_t at = __ballot(surv && myd == dm);
+          const int jl = __ffsll((unsigned long long)at) - 1;
+          const uint64_t key = at ? ckey(dm, rl(myid, jl)) : ~0ull;
+          if (at && (csize == 0 || key < Ck[head])) {
+            pf2_id = (uint32_t)key;
+            pf2_adj = lane < (int)P.m0 ? adj0[(size_t)pf2_id * P.m0 + lane] : kInvalid;
+          }
+        }
         uint64_t we[SW];
         list_load<SW>(Wk, wsize, we);
         uint32_t below = 0;
@@ -592,21 +619,6 @@ hnsw_search_kernel(const SearchArgs P) {
         const bool acc = surv && below < ef;
         const uint64_t amask = __ballot(acc);
         const uint64_t akey = acc ? ckey(myd, myid) : ~0ull;
-        if (last && pf_ok && amask) {
-          // the smallest accepted neighbour, if it beats C's front, is the
-          // next candidate: fetch its adjacency row during the merges
-          uint64_t key = akey;
-#pragma unroll
-          for (int m = 1; m < 64; m <<= 1) {
-            const uint64_t o = ((uint64_t)(uint32_t)__shfl_xor((int)(key >> 32), m, 64) << 32) |
-                               (uint32_t)__shfl_xor((int)(uint32_t)key, m, 64);
-            key = o < key ? o : key;
-          }
-          if (csize == 0 || key < Ck[head]) {
-            pf2_id = (uint32_t)key;
-            pf2_adj = lane < (int)P.m0 ? adj0[(size_t)pf2_id * P.m0 + lane] : kInvalid;
-          }
-        }
         if (amask) {
           uint64_t ce[2 * SW];
           list_load<2 * SW>(Ck + head, csize, ce);
