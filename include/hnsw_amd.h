@@ -223,7 +223,8 @@ typedef struct hh_index hh_index;
 /* Copies the graph into HBM as: ELL layer 0 (n x m0 ids, unused slots
  * 0xFFFFFFFF), upper blocks, permuted f32 rows, keys, tombstone bits and,
  * when present, int8 codes.  Replaces the crate's in-memory HnswIndex for
- * the query path.  device: HIP ordinal. */
+ * the query path.  device: HIP ordinal.  The search kernels take dims up
+ * to 1024; a search of a wider index returns HH_UNSUPPORTED. */
 int hh_index_upload(const hh_graph_desc *desc, int device, hh_index **out);
 void hh_index_free(hh_index *idx);
 /* Re-uploads the tombstone flags (after HnswIndex::remove). */

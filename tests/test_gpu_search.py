@@ -106,6 +106,22 @@ def test_one_percent_filter_at_ef_512_keeps_every_candidate():
     assert np.array_equal(gc, oc) and np.array_equal(gk, ok) and same_bits(gd, od)
 
 
+def test_search_above_1024_dims_is_unsupported_not_wrong():
+    """Rows of 1025-1055 dims fit the widest row layout but not the code
+    chunks of the kernels; the search refuses them (HH_UNSUPPORTED) instead
+    of returning partial sums, and 1024 dims search normally."""
+    rng = np.random.default_rng(1)
+    for dim, ok in ((1024, True), (1030, False)):
+        b = rng.standard_normal((40, dim)).astype(np.float32)
+        idx = build(b)
+        if ok:
+            gk, gd, gc = idx.search_arrays(b[:3], 5, 16)
+            assert (gc == 5).all() and np.array_equal(gk[:, 0], np.arange(3))
+        else:
+            with pytest.raises(H.Unsupported):
+                idx.search_arrays(b[:3], 5, 16)
+
+
 def test_filter_matches_oracle_and_is_respected():
     """src/hnsw/tests.rs:293-313"""
     b, q = O.dataset_gaussian(2000, 10, 8, 8)
