@@ -606,17 +606,37 @@ hnsw_search_kernel(const SearchArgs P) {
         }
         uint64_t we[SW];
         list_load<SW>(Wk, wsize, we);
-        uint32_t below = 0;
-        for (uint64_t m = sm; m; m &= m - 1) {
-          const int j = __ffsll((unsigned long long)m) - 1;
-          const float dj = rlf(myd, j);
-          uint32_t cw = 0;  // results at or below d_j
+        // Shortcut: every survivor passes when even the largest survivor
+        // distance has fewer than ef entries at or below it counting W's and
+        // every admitted survivor (then no neighbour's count can reach ef).
+        const uint32_t nadm = (uint32_t)popc64(sm & am);
+        bool all_pass = wsize + nadm < ef;
+        if (!all_pass && sm) {
+          float dx = surv ? myd : -INFINITY;
+          dx = fmaxf(dx, dpp<kRor8>(dx));
+          dx = fmaxf(dx, dpp<kRor4>(dx));
+          dx = fmaxf(dx, dpp<kXor1>(dx));
+          dx = fmaxf(dx, dpp<kXor2>(dx));
+          dx = fmaxf(fmaxf(rlf(dx, 0), rlf(dx, 16)), fmaxf(rlf(dx, 32), rlf(dx, 48)));
+          uint32_t cw = 0;
 #pragma unroll
           for (int c = 0; c < SW; c++)
-            cw += (uint32_t)popc64(__ballot(c * 64 + lane < (int)wsize && !(ckey_d(we[c]) > dj)));
-          below += (lane == j ? cw : 0u) + (((am >> j) & 1) && j < lane && !(dj > myd) ? 1u : 0u);
+            cw += (uint32_t)popc64(__ballot(c * 64 + lane < (int)wsize && !(ckey_d(we[c]) > dx)));
+          all_pass = cw + nadm < ef;
         }
-        const bool acc = surv && below < ef;
+        uint32_t below = 0;
+        if (!all_pass) {
+          for (uint64_t m = sm; m; m &= m - 1) {
+            const int j = __ffsll((unsigned long long)m) - 1;
+            const float dj = rlf(myd, j);
+            uint32_t cw = 0;  // results at or below d_j
+#pragma unroll
+            for (int c = 0; c < SW; c++)
+              cw += (uint32_t)popc64(__ballot(c * 64 + lane < (int)wsize && !(ckey_d(we[c]) > dj)));
+            below += (lane == j ? cw : 0u) + (((am >> j) & 1) && j < lane && !(dj > myd) ? 1u : 0u);
+          }
+        }
+        const bool acc = surv && (all_pass || below < ef);
         const uint64_t amask = __ballot(acc);
         const uint64_t akey = acc ? ckey(myd, myid) : ~0ull;
         if (amask) {
