@@ -64,6 +64,7 @@ __device__ __forceinline__ uint32_t merge_ranks(const uint64_t (&e)[CPL], uint32
     uint32_t r = 0;
 #pragma unroll
     for (int c = 0; c < CPL; c++) {
+      if (c > 0 && c * 64 >= (int)size) break;  // chunks past the list: nothing to rank
       const bool valid = c * 64 + lane < (int)size;
       r += (uint32_t)popc64(__ballot(valid && e[c] < kj));
       ep[c] += kj < e[c] ? 1u : 0u;
@@ -89,6 +90,7 @@ __device__ __forceinline__ uint64_t w_merge(uint64_t *W, const uint64_t (&e)[WPL
   if (la) last = rl64(akey, __ffsll((unsigned long long)la) - 1);
 #pragma unroll
   for (int c = 0; c < WPL; c++) {
+    if (c > 0 && c * 64 >= (int)size) break;  // chunks past W hold nothing
     const bool valid = c * 64 + lane < (int)size;
     const uint64_t lb = __ballot(valid && ep[c] == nsize - 1);
     if (lb) last = rl64(e[c], __ffsll((unsigned long long)lb) - 1);
@@ -126,6 +128,7 @@ __device__ __forceinline__ void c_merge(uint64_t *C, const uint64_t (&e)[CPL], u
   uint32_t live = 0;  // kept entries that are not dead
 #pragma unroll
   for (int c = 0; c < CPL; c++) {
+    if (c > 0 && c * 64 >= (int)size) break;  // chunks past C hold nothing
     const bool valid = c * 64 + lane < (int)size;
     const uint64_t b = __ballot(valid && ep[c] == keep);
     if (b) first_out = rl64(e[c], __ffsll((unsigned long long)b) - 1);
@@ -174,7 +177,8 @@ __device__ __forceinline__ void list_load(const uint64_t *L, uint32_t size, uint
 #pragma unroll
   for (int c = 0; c < CPL; c++) {
     const uint32_t i = c * 64 + lane;
-    e[c] = i < size ? L[i] : ~0ull;
+    e[c] = ~0ull;
+    if (c == 0 || c * 64 < (int)size) e[c] = i < size ? L[i] : ~0ull;
   }
 }
 
