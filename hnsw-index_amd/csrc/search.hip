@@ -51,9 +51,11 @@ __device__ __forceinline__ float ckey_d(uint64_t k) {
 // list elements e[0..CPL) (valid where chunk*64 + lane < size), plus the
 // number of batch elements ordering before it; and, for every list
 // element, its index plus the number of batch elements ordering before it.
-template <int CPL>
-__device__ __forceinline__ uint32_t merge_ranks(const uint64_t (&e)[CPL], uint32_t (&ep)[CPL],
-                                                uint32_t size, uint64_t amask, uint64_t akey) {
+// N: the chunks that can hold list elements (chunks past the list rank
+// nothing), a compile-time count so the per-element loop stays straight.
+template <int CPL, int N>
+__device__ __forceinline__ uint32_t merge_ranks_n(const uint64_t (&e)[CPL], uint32_t (&ep)[CPL],
+                                                  uint32_t size, uint64_t amask, uint64_t akey) {
   const int lane = lane_id();
   uint32_t ap = 0;
 #pragma unroll
@@ -63,8 +65,7 @@ __device__ __forceinline__ uint32_t merge_ranks(const uint64_t (&e)[CPL], uint32
     const uint64_t kj = rl64(akey, j);
     uint32_t r = 0;
 #pragma unroll
-    for (int c = 0; c < CPL; c++) {
-      if (c > 0 && c * 64 >= (int)size) break;  // chunks past the list: nothing to rank
+    for (int c = 0; c < N; c++) {
       const bool valid = c * 64 + lane < (int)size;
       r += (uint32_t)popc64(__ballot(valid && e[c] < kj));
       ep[c] += kj < e[c] ? 1u : 0u;
@@ -72,6 +73,15 @@ __device__ __forceinline__ uint32_t merge_ranks(const uint64_t (&e)[CPL], uint32
     ap += (kj < akey ? 1u : 0u) + (lane == j ? r : 0u);
   }
   return ap;
+}
+template <int CPL>
+__device__ __forceinline__ uint32_t merge_ranks(const uint64_t (&e)[CPL], uint32_t (&ep)[CPL],
+                                                uint32_t size, uint64_t amask, uint64_t akey) {
+  if (size <= 64) return merge_ranks_n<CPL, 1>(e, ep, size, amask, akey);
+  if constexpr (CPL > 2) {
+    if (size <= 128) return merge_ranks_n<CPL, 2>(e, ep, size, amask, akey);
+  }
+  return merge_ranks_n<CPL, CPL>(e, ep, size, amask, akey);
 }
 
 // W <- the `cap` smallest of W[0..size) (held in e[]) and the batch
@@ -177,8 +187,7 @@ __device__ __forceinline__ void list_load(const uint64_t *L, uint32_t size, uint
 #pragma unroll
   for (int c = 0; c < CPL; c++) {
     const uint32_t i = c * 64 + lane;
-    e[c] = ~0ull;
-    if (c == 0 || c * 64 < (int)size) e[c] = i < size ? L[i] : ~0ull;
+    e[c] = i < size ? L[i] : ~0ull;
   }
 }
 
