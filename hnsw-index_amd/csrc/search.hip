@@ -81,6 +81,12 @@ __device__ __forceinline__ uint32_t merge_ranks(const uint64_t (&e)[CPL], uint32
   if constexpr (CPL > 2) {
     if (size <= 128) return merge_ranks_n<CPL, 2>(e, ep, size, amask, akey);
   }
+  if constexpr (CPL > 4) {
+    if (size <= 256) return merge_ranks_n<CPL, 4>(e, ep, size, amask, akey);
+  }
+  if constexpr (CPL > 8) {
+    if (size <= 512) return merge_ranks_n<CPL, 8>(e, ep, size, amask, akey);
+  }
   return merge_ranks_n<CPL, CPL>(e, ep, size, amask, akey);
 }
 
