@@ -90,6 +90,27 @@ __device__ __forceinline__ uint32_t merge_ranks(const uint64_t (&e)[CPL], uint32
   return merge_ranks_n<CPL, CPL>(e, ep, size, amask, akey);
 }
 
+// The crate's sequential acceptance (search.rs:118-132) counted for a
+// neighbour batch: for each surviving neighbour (lanes in sm, distance d),
+// the results at or below d plus the admitted survivors before it at or
+// below d.  N: the chunks of W that can hold results.
+template <int SW, int N>
+__device__ __forceinline__ uint32_t accept_counts(const uint64_t (&we)[SW], uint32_t wsize,
+                                                  uint64_t sm, uint64_t am, float myd) {
+  const int lane = lane_id();
+  uint32_t below = 0;
+  for (uint64_t m = sm; m; m &= m - 1) {
+    const int j = __ffsll((unsigned long long)m) - 1;
+    const float dj = rlf(myd, j);
+    uint32_t cw = 0;  // results at or below d_j
+#pragma unroll
+    for (int c = 0; c < N; c++)
+      cw += (uint32_t)popc64(__ballot(c * 64 + lane < (int)wsize && !(ckey_d(we[c]) > dj)));
+    below += (lane == j ? cw : 0u) + (((am >> j) & 1) && j < lane && !(dj > myd) ? 1u : 0u);
+  }
+  return below;
+}
+
 // W <- the `cap` smallest of W[0..size) (held in e[]) and the batch
 // elements in `amask`; returns the new worst (largest) word.
 template <int WPL>
@@ -645,15 +666,10 @@ hnsw_search_kernel(const SearchArgs P) {
         }
         uint32_t below = 0;
         if (!all_pass) {
-          for (uint64_t m = sm; m; m &= m - 1) {
-            const int j = __ffsll((unsigned long long)m) - 1;
-            const float dj = rlf(myd, j);
-            uint32_t cw = 0;  // results at or below d_j
-#pragma unroll
-            for (int c = 0; c < SW; c++)
-              cw += (uint32_t)popc64(__ballot(c * 64 + lane < (int)wsize && !(ckey_d(we[c]) > dj)));
-            below += (lane == j ? cw : 0u) + (((am >> j) & 1) && j < lane && !(dj > myd) ? 1u : 0u);
-          }
+          if (wsize <= 64) below = accept_counts<SW, 1>(we, wsize, sm, am, myd);
+          else if (SW > 2 && wsize <= 128) below = accept_counts<SW, (SW > 2 ? 2 : SW)>(we, wsize, sm, am, myd);
+          else if (SW > 4 && wsize <= 256) below = accept_counts<SW, (SW > 4 ? 4 : SW)>(we, wsize, sm, am, myd);
+          else below = accept_counts<SW, SW>(we, wsize, sm, am, myd);
         }
         const bool acc = surv && (all_pass || below < ef);
         const uint64_t amask = __ballot(acc);
