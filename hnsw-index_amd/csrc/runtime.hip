@@ -204,7 +204,8 @@ struct Workspace {
   uint64_t *seed = nullptr;
   uint32_t *order = nullptr;
   uint32_t *spill = nullptr;
-  uint64_t *cpool = nullptr;  // per-wave candidate pools (kCandPool entries)
+  uint64_t *cpool = nullptr;  // per-wave candidate pools (pool_cap entries each)
+  uint32_t pool_cap = 0;
   uint32_t *slog = nullptr;   // per-wave spill logs (kSpillLog entries)
   uint64_t spill_slots = 0;
   uint64_t spill_wcap = 0;   // bitset words per slot the allocation holds
@@ -312,8 +313,12 @@ struct Workspace {
     dfree(spill);
     dfree(cpool);
     dfree(slog);
+    // a query's candidates are distinct nodes, so a pool of n entries (the
+    // bitset covers 32 * words >= n nodes) can never overflow
+    const uint32_t pcap =
+        (uint32_t)std::min<uint64_t>(kCandPool, (std::max<uint64_t>(32 * words, 64) + 63) / 64 * 64);
     int st = dalloc(&spill, slots * words);
-    if (!st) st = dalloc(&cpool, slots * kCandPool);
+    if (!st) st = dalloc(&cpool, slots * pcap);
     if (!st) st = dalloc(&slog, slots * kSpillLog);
     if (st) {
       spill_slots = 0;
@@ -322,6 +327,7 @@ struct Workspace {
     HH_HIP(hipMemset(spill, 0, slots * words * sizeof(uint32_t)));
     spill_slots = slots;
     spill_wcap = words;
+    pool_cap = pcap;
     return HH_OK;
   }
   int ensure_part(uint64_t entries, uint64_t lists, bool f64) {
@@ -719,7 +725,7 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.spill = ws.spill;
   a.spill_words = cfg.spill_words;
   a.cpool = ws.cpool;
-  a.cpool_cap = kCandPool;
+  a.cpool_cap = ws.pool_cap;
   a.spill_log = ws.slog;
   a.spill_log_cap = kSpillLog;
   a.stats = idx->stats_ws.stats;
