@@ -271,12 +271,12 @@ int hh_index_download_codes(hh_index *idx, int8_t *codes);
  * one and at most 16 queries per resident wavefront runs in two passes,
  * the upper-layer descent of every query, then layer 0 with the queries of
  * the largest descent distance first, so that long queries do not start
- * last.  By default it applies from ef 32 to batches of one precision on
- * rows of 128 or 256 dimensions with m <= 16, and above ef 64 otherwise;
- * hh_index_autotune replaces that rule, per ef bucket, by whichever of one
- * and two passes it measured faster.  The descent pass of a batch of one
- * precision with m <= 32 runs 4 (m <= 16) or 2 queries per wavefront.
- * 0: one pass in query order. */
+ * last.  By default it applies from ef 32 on rows of 128 or 256
+ * dimensions with m <= 16, and above ef 64 otherwise (ef: the batch's
+ * largest); hh_index_autotune replaces that rule, per ef bucket, by
+ * whichever of one and two passes it measured faster.  The descent pass
+ * runs 4 (m <= 16) or 2 (m <= 32) queries per wavefront; above m = 32 the
+ * search kernel runs it.  0: one pass in query order. */
 int hh_index_set_query_order(hh_index *idx, int longest_first);
 
 /* Search statistics accumulated by the kernels since the last reset. */
