@@ -759,15 +759,18 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   const int eb = ef_bucket(std::max<uint32_t>(efb, 1));
   const int tuned = eb >= 0 ? idx->tuned_order[eb] : 0;
   const bool two_pass = tuned ? tuned == 2 : ((fast_descent && narrow && efb > 16) || efb > 64);
-  if (idx->longest_first && idx->top >= 1 && c.nq > waves && c.nq <= 16 * waves && two_pass) {
+  // Mixed batches are ordered by ef first (their cost spread is mostly
+  // ef's), which pays however many queries each wave takes.
+  if (idx->longest_first && idx->top >= 1 && c.nq > waves &&
+      (c.nq <= 16 * waves || (c.specs && !c.build)) && two_pass) {
     a.phase = 1;
     if (fast_descent) {
       if (int st = launch_descent(idx, c.nq, a, stream)) return st;
     } else {
       HH_HIP(hipLaunchKernel(cfg.kern, grid, block, args, lds, stream));
     }
-    hipLaunchKernelGGL(hhk::order_queries_kernel, dim3(1), dim3(1024), 0, stream, ws.seed, c.nq,
-                       ws.order, ws.counter);
+    hipLaunchKernelGGL(hhk::order_queries_kernel, dim3(1), dim3(1024), 0, stream, ws.seed,
+                       (const hh_query_spec *)c.specs, c.nq, ws.order, ws.counter);
     HH_HIP(hipGetLastError());
     a.phase = 2;
   }

@@ -1053,7 +1053,20 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
 // seeds' order bins in LDS, bins taken from the largest descent distance
 // down (queries within a bin in any order); then resets the search kernel's
 // query counter for the second pass.
-__global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *seed, uint64_t nq,
+__device__ __forceinline__ uint32_t query_order_bin(const uint64_t *seed, const hh_query_spec *specs,
+                                                    uint64_t i) {
+  const uint32_t b = order_bin(__uint_as_float((uint32_t)(G(seed)[i] >> 32)));
+  if (!specs) return b;
+  // mixed batches: ef (raised to k) first, 512 levels, then the descent
+  // distance's top 3 bin bits
+  const hh_query_spec sp = G(specs)[i];
+  uint32_t ef = sp.ef > sp.k ? sp.ef : sp.k;
+  ef = ef < 1 ? 1 : (ef > 512 ? 512 : ef);
+  return (ef - 1) * 8 + (b >> 9);
+}
+
+__global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *seed,
+                                                             const hh_query_spec *specs, uint64_t nq,
                                                              uint32_t *order, uint32_t *counter) {
   static_assert(kOrderBins == 4 * 1024, "four bins per thread");
   __shared__ uint32_t off[kOrderBins];
@@ -1062,8 +1075,7 @@ __global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *see
 #pragma unroll
   for (int i = 0; i < 4; i++) off[4 * t + i] = 0u;
   __syncthreads();
-  for (uint64_t i = t; i < nq; i += 1024)
-    atomicAdd(&off[order_bin(__uint_as_float((uint32_t)(G(seed)[i] >> 32)))], 1u);
+  for (uint64_t i = t; i < nq; i += 1024) atomicAdd(&off[query_order_bin(seed, specs, i)], 1u);
   __syncthreads();
   // thread t owns the descending-order bins 4t..4t+3 (bin kOrderBins-1-r)
   uint32_t c[4], sum = 0;
@@ -1089,8 +1101,7 @@ __global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *see
   if (t == 0) GM(counter)[0] = 0u;
   __syncthreads();
   for (uint64_t i = t; i < nq; i += 1024) {
-    const float best = __uint_as_float((uint32_t)(G(seed)[i] >> 32));
-    const uint32_t pos = atomicAdd(&off[order_bin(best)], 1u);
+    const uint32_t pos = atomicAdd(&off[query_order_bin(seed, specs, i)], 1u);
     if (pos < nq) GM(order)[pos] = (uint32_t)i;
   }
 }

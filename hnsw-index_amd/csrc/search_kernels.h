@@ -100,10 +100,11 @@ constexpr int descent_code_words() { return (32 * TMAX + 32 + 127) / 128 * 16; }
 template <int TMAX, int LPQ, bool EXACT>
 __global__ void descent_kernel(const SearchArgs P);
 // One block of 1024 threads: order[] = the nq queries by descending
-// order_bin of their seed distance; resets the search kernel's query
-// counter for the next pass.
-__global__ void order_queries_kernel(const uint64_t *seed, uint64_t nq, uint32_t *order,
-                                     uint32_t *counter);
+// order_bin of their seed distance (with specs, a mixed batch: by
+// descending ef first, then that distance); resets the search kernel's
+// query counter for the next pass.
+__global__ void order_queries_kernel(const uint64_t *seed, const hh_query_spec *specs, uint64_t nq,
+                                     uint32_t *order, uint32_t *counter);
 __global__ void prepare_queries_kernel(const PrepArgs A);
 
 }  // namespace hhk
