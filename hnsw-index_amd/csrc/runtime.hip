@@ -759,8 +759,9 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   const int eb = ef_bucket(std::max<uint32_t>(efb, 1));
   const int tuned = eb >= 0 ? idx->tuned_order[eb] : 0;
   const bool two_pass = tuned ? tuned == 2 : ((fast_descent && narrow && efb > 16) || efb > 64);
-  // Mixed batches are ordered by ef first (their cost spread is mostly
-  // ef's), which pays however many queries each wave takes.
+  // Mixed batches are ordered by their estimated work (ef over the filter's
+  // pass rate: most of their cost spread), which pays however many queries
+  // each wave takes.
   if (idx->longest_first && idx->top >= 1 && c.nq > waves &&
       (c.nq <= 16 * waves || (c.specs && !c.build)) && two_pass) {
     a.phase = 1;
@@ -770,7 +771,8 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
       HH_HIP(hipLaunchKernel(cfg.kern, grid, block, args, lds, stream));
     }
     hipLaunchKernelGGL(hhk::order_queries_kernel, dim3(1), dim3(1024), 0, stream, ws.seed,
-                       (const hh_query_spec *)c.specs, c.nq, ws.order, ws.counter);
+                       c.specs, c.nq, c.filters, a.filter_words, c.filters ? c.nfilters : 0u,
+                       ws.order, ws.counter);
     HH_HIP(hipGetLastError());
     a.phase = 2;
   }

@@ -1053,21 +1053,53 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
 // seeds' order bins in LDS, bins taken from the largest descent distance
 // down (queries within a bin in any order); then resets the search kernel's
 // query counter for the second pass.
+// Mixed batches: by an estimate of the query's layer-0 work, ef (raised
+// to k) over its filter's pass rate (a query keeps expanding until ef
+// admitted nodes are found), on a log scale; plain batches: by the
+// descent distance.
 __device__ __forceinline__ uint32_t query_order_bin(const uint64_t *seed, const hh_query_spec *specs,
-                                                    uint64_t i) {
+                                                    const float *rate, uint32_t nrate, uint64_t i) {
   const uint32_t b = order_bin(__uint_as_float((uint32_t)(G(seed)[i] >> 32)));
   if (!specs) return b;
-  // mixed batches: ef (raised to k) first, 512 levels, then the descent
-  // distance's top 3 bin bits
   const hh_query_spec sp = G(specs)[i];
   uint32_t ef = sp.ef > sp.k ? sp.ef : sp.k;
   ef = ef < 1 ? 1 : (ef > 512 ? 512 : ef);
-  return (ef - 1) * 8 + (b >> 9);
+  float r = sp.filter < nrate ? rate[sp.filter] : 1.0f;
+  r = r < (1.0f / 256.0f) ? (1.0f / 256.0f) : r;
+  const float cost = __log2f((float)ef / r);  // 0 .. 17
+  const int c = (int)(cost * 224.0f);
+  return (uint32_t)(c < 0 ? 0 : (c > 4095 ? 4095 : c));
 }
 
 __global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *seed,
                                                              const hh_query_spec *specs, uint64_t nq,
-                                                             uint32_t *order, uint32_t *counter) {
+                                                             const uint32_t *filters, uint32_t filter_words,
+                                                             uint32_t nfilters, uint32_t *order,
+                                                             uint32_t *counter) {
+  constexpr uint32_t kRated = 256;     // filters with a pass-rate estimate
+  constexpr uint32_t kSampled = 4096;  // bitmap words sampled per filter
+  __shared__ float rate[kRated];
+  __shared__ uint32_t ones[kRated];
+  const uint32_t nrate = specs && filters ? (nfilters < kRated ? nfilters : kRated) : 0u;
+  {
+    const uint32_t t0 = threadIdx.x;
+    if (t0 < kRated) ones[t0] = 0u;
+    __syncthreads();
+    const uint32_t sw = filter_words < kSampled ? filter_words : kSampled;
+    if (nrate && sw) {
+      // threads t, t + nrate, ... of filter t % nrate sum strided words
+      const uint32_t per = 1024 / nrate;
+      const uint32_t f = t0 % nrate, part = t0 / nrate;
+      if (part < per) {
+        uint32_t sum = 0;
+        for (uint32_t w = part; w < sw; w += per)
+          sum += __popc(G(filters)[(size_t)f * filter_words + w]);
+        atomicAdd(&ones[f], sum);
+      }
+    }
+    __syncthreads();
+    if (t0 < nrate) rate[t0] = sw ? (float)ones[t0] / (32.0f * (float)sw) : 1.0f;
+  }
   static_assert(kOrderBins == 4 * 1024, "four bins per thread");
   __shared__ uint32_t off[kOrderBins];
   __shared__ uint32_t part[1024];
@@ -1075,7 +1107,8 @@ __global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *see
 #pragma unroll
   for (int i = 0; i < 4; i++) off[4 * t + i] = 0u;
   __syncthreads();
-  for (uint64_t i = t; i < nq; i += 1024) atomicAdd(&off[query_order_bin(seed, specs, i)], 1u);
+  for (uint64_t i = t; i < nq; i += 1024)
+    atomicAdd(&off[query_order_bin(seed, specs, rate, nrate, i)], 1u);
   __syncthreads();
   // thread t owns the descending-order bins 4t..4t+3 (bin kOrderBins-1-r)
   uint32_t c[4], sum = 0;
@@ -1101,7 +1134,7 @@ __global__ void __launch_bounds__(1024) order_queries_kernel(const uint64_t *see
   if (t == 0) GM(counter)[0] = 0u;
   __syncthreads();
   for (uint64_t i = t; i < nq; i += 1024) {
-    const uint32_t pos = atomicAdd(&off[query_order_bin(seed, specs, i)], 1u);
+    const uint32_t pos = atomicAdd(&off[query_order_bin(seed, specs, rate, nrate, i)], 1u);
     if (pos < nq) GM(order)[pos] = (uint32_t)i;
   }
 }

@@ -101,10 +101,12 @@ template <int TMAX, int LPQ, bool EXACT>
 __global__ void descent_kernel(const SearchArgs P);
 // One block of 1024 threads: order[] = the nq queries by descending
 // order_bin of their seed distance (with specs, a mixed batch: by
-// descending ef first, then that distance); resets the search kernel's
-// query counter for the next pass.
+// descending estimated work, ef over the filter's pass rate sampled from
+// the first 4096 words of each of up to 256 bitmaps); resets the search
+// kernel's query counter for the next pass.
 __global__ void order_queries_kernel(const uint64_t *seed, const hh_query_spec *specs, uint64_t nq,
-                                     uint32_t *order, uint32_t *counter);
+                                     const uint32_t *filters, uint32_t filter_words,
+                                     uint32_t nfilters, uint32_t *order, uint32_t *counter);
 __global__ void prepare_queries_kernel(const PrepArgs A);
 
 }  // namespace hhk
