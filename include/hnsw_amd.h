@@ -274,9 +274,12 @@ int hh_index_download_codes(hh_index *idx, int8_t *codes);
  * last.  By default it applies from ef 32 on rows of 128 or 256
  * dimensions with m <= 16, and above ef 64 otherwise (ef: the batch's
  * largest); hh_index_autotune replaces that rule, per ef bucket, by
- * whichever of one and two passes it measured faster.  The descent pass
- * runs 4 (m <= 16) or 2 (m <= 32) queries per wavefront; above m = 32 the
- * search kernel runs it.  0: one pass in query order. */
+ * whichever of one and two passes it measured faster.  Mixed batches
+ * (hh_search_batch_spec*) of any size above one query per resident
+ * wavefront are ordered by estimated work instead: ef over the filter's
+ * pass rate.  The descent pass runs 4 (m <= 16) or 2 (m <= 32) queries per
+ * wavefront; above m = 32 the search kernel runs it.  0: one pass in query
+ * order. */
 int hh_index_set_query_order(hh_index *idx, int longest_first);
 
 /* Search statistics accumulated by the kernels since the last reset. */
