@@ -471,6 +471,37 @@ def test_autotune_keeps_results_in_every_shape_and_order():
     assert np.array_equal(got[0], ref[0]) and same_bits(got[1], ref[1])
 
 
+def test_work_ordered_mixed_batch_matches_one_pass_and_oracle():
+    """A mixed batch larger than 16 queries per resident wave, with per-query
+    k, ef up to 512 and filters of 3-60% pass rate, runs in two passes
+    ordered by estimated work (ef over the sampled pass rate); results and
+    statuses equal the one-pass launch's, and a sample equals the oracle."""
+    d = H.synthetic.Dataset.lognormal(6000, 40000, 16, 0.5, 20, 23)
+    b, q, cat = d.base(), d.queries(), d.categories
+    idx = build(b, H.Metric.InnerProduct, m=8, threads=8)
+    rng = np.random.default_rng(9)
+    nq = q.shape[0]
+    masks = [cat < c for c in (1, 2, 4, 8, 12)] + [rng.random(6000) < 0.03]
+    ks = rng.choice([1, 10, 50], nq)
+    efs = rng.integers(10, 513, nq)
+    fids = rng.integers(-1, len(masks), nq)
+    out = {}
+    for longest_first in (False, True):
+        print(f"case mixed longest_first={longest_first}", flush=True)
+        idx.set_query_order(longest_first)
+        out[longest_first] = idx.search_mixed(q, ks, efs, filter_ids=fids, filters=masks)
+    for a, b_ in zip(out[False], out[True]):
+        assert np.array_equal(a.view(np.uint8), b_.view(np.uint8))
+    gk, gd, gc, gs = out[True]
+    assert (gs == 0).all()
+    g = oracle_graph(idx)
+    for i in range(0, nq, 997):
+        allow = None if fids[i] < 0 else masks[fids[i]].astype(np.uint8)
+        ok, od, oc, _ = g.search_batch(q[i:i + 1], int(ks[i]), int(efs[i]), allow=allow)
+        n = int(oc[0])
+        assert gc[i] == n and np.array_equal(gk[i, :n], ok[0, :n]) and same_bits(gd[i, :n], od[0, :n]), i
+
+
 @pytest.mark.parametrize("metric", [H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine])
 @pytest.mark.parametrize("dim", [128, 256])
 def test_descent_kernel_two_pass_matches_one_pass(metric, dim):
