@@ -294,6 +294,18 @@ def main():
         f"({stats['n_upper'] / nq:.0f} in the upper layers), {stats['n_expand'] / nq:.1f} layer-0 "
         f"expansions/query; {achieved:.0f} GB/s of gathered rows")
 
+    # what the device delivers to this gather pattern on the same rows with
+    # no dependent work (random rows, 16 waves per CU, 4 rows per quarter
+    # in flight): the practical ceiling the search's gathered rate is read
+    # against
+    gather_probe = None
+    gbps, pms = ctypes.c_double(), ctypes.c_float()
+    if H.lib.hh_index_gather_probe(h, 64, ctypes.byref(gbps), ctypes.byref(pms)) == 0:
+        gather_probe = {"gbps": round(gbps.value, 1), "ms": round(pms.value, 4),
+                        "search_fraction": round(achieved / gbps.value, 3)}
+        log(f"gather probe: {gbps.value:.0f} GB/s for random {dim * 4}-byte rows with no "
+            f"dependent work; the search gathers at {achieved / gbps.value:.0%} of it")
+
     variants, serve, cpu = {}, None, None
     if args.quick:
         emit(args, locals())
@@ -735,6 +747,7 @@ def emit(args, v):
         "step_ms_p99": round(float(np.percentile(step_ms, 99)), 3),
         "search_kernel_ms": round(kern_ms, 4),
         "two_stream_qps": round(v["pipe_qps"]),
+        "gather_probe": v.get("gather_probe"),
         "index_sync": v.get("mutation"),
         "device_assisted_build": v.get("device_build"),
         "dist_evals_per_query": round(n_dist / nq, 1),

@@ -51,6 +51,40 @@ void dfree(T *&p) {
   p = nullptr;
 }
 
+// The search's row gather without its dependent work: each 16-lane
+// quarter loads 4 rows per round (lane i: float4s at 4i + 64c), the rows
+// drawn by a hash of (wave, round, slot) over [0, n), all loads of a round
+// issued before any is used; the sums go to out so nothing is dropped.
+__global__ void __launch_bounds__(256) gather_probe_kernel(const float *rows, uint64_t n,
+                                                           uint32_t stride, uint32_t rounds,
+                                                           uint64_t seed, float *out) {
+  const uint32_t lane = threadIdx.x & 63, i = lane & 15, quarter = lane >> 4;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  float acc = 0.0f;
+  for (uint32_t r = 0; r < rounds; r++) {
+    const float *row[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      uint64_t h = (wave * rounds + r) * 16 + 4 * b + quarter + seed;
+      h ^= h >> 33; h *= 0xff51afd7ed558ccdull; h ^= h >> 33; h *= 0xc4ceb9fe1a85ec53ull;
+      h ^= h >> 33;
+      row[b] = rows + (h % n) * stride;
+    }
+    float4 v[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        v[b][c] = 4 * i + 64 * c < stride ? *(const float4 *)(row[b] + 4 * i + 64 * c)
+                                          : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc += v[b][c].x + v[b][c].y + v[b][c].z + v[b][c].w;
+  }
+  out[(uint64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 // natural row-major rows -> accumulator-major rows (RowLayout::pos)
 __global__ void permute_rows_kernel(const float *src, float *dst, uint64_t n, uint32_t dim,
                                     uint32_t stride, int T) {
@@ -1433,6 +1467,42 @@ int hh_index_last_kernel_ms(hh_index *idx, float *ms) {
   HH_HIP(hipSetDevice(idx->device));
   HH_HIP(hipEventSynchronize(idx->t1));
   HH_HIP(hipEventElapsedTime(ms, idx->t0, idx->t1));
+  return HH_OK;
+}
+
+int hh_index_gather_probe(hh_index *idx, uint32_t rounds, double *gbps, float *ms) {
+  if (!idx || !gbps || rounds == 0) return fail(HH_INVALID_ARGUMENT, "null argument");
+  if (idx->n == 0) return fail(HH_INVALID_ARGUMENT, "empty index");
+  if (idx->lay.stride > 256) return fail(HH_UNSUPPORTED, "the probe covers rows of up to 256 floats");
+  HH_HIP(hipSetDevice(idx->device));
+  if (int st = drain(idx)) return st;
+  // 16 waves per CU, the search kernel's residency at ef <= 64
+  const uint32_t blocks = 4u * (uint32_t)num_cus(idx->device);
+  float *out = nullptr;
+  HH_HIP(hipMalloc((void **)&out, (size_t)blocks * 256 * sizeof(float)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  float best = 0.0f;
+  for (int rep = 0; rep < 4 && e == hipSuccess; rep++) {
+    e = hipEventRecord(e0, idx->stream);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL(gather_probe_kernel, dim3(blocks), dim3(256), 0, idx->stream, idx->rows,
+                         idx->n, idx->lay.stride, rounds, (uint64_t)rep * 0x9E3779B97F4A7C15ull, out);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, idx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e == hipSuccess && rep > 0 && (best == 0.0f || t < best)) best = t;  // first run warms up
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  if (e != hipSuccess) return fail(HH_DEVICE_ERROR, std::string("gather probe: ") + hipGetErrorString(e));
+  const double bytes = (double)blocks * 4 * rounds * 16 * idx->dim * 4.0;
+  *gbps = bytes / (best * 1e-3) / 1e9;
+  if (ms) *ms = best;
   return HH_OK;
 }
 

@@ -94,6 +94,8 @@ _sig("hh_index_set_timing", INT, P, INT)
 _sig("hh_index_autotune", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, ctypes.POINTER(U32),
      ctypes.POINTER(INT))
 _sig("hh_index_last_kernel_ms", INT, P, ctypes.POINTER(ctypes.c_float))
+_sig("hh_index_gather_probe", INT, P, U32, ctypes.POINTER(ctypes.c_double),
+     ctypes.POINTER(ctypes.c_float))
 _sig("hh_search_batch_device", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, P)
 _sig("hh_search_batch_spec_device", INT, P, P, U64, P, U32, U32, P, U32, P, P, P, P, P)
 _sig("hh_search_batch_spec", INT, P, P, U64, P, U32, P, U32, P, P, P, P)

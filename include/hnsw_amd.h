@@ -317,6 +317,13 @@ int hh_index_set_launch_shape(hh_index *idx, uint32_t ef, uint32_t hash_cap, int
 int hh_index_set_timing(hh_index *idx, int on);
 /* Milliseconds of the search kernel of the last timed launch (waits for it). */
 int hh_index_last_kernel_ms(hh_index *idx, float *ms);
+/* Measurement (no reference counterpart): the rate at which the device
+ * delivers the search's row gather on this index's rows with no dependent
+ * work, random rows, 16 wavefronts per CU, each 16-lane group keeping four
+ * rows in flight (rounds of 16 rows per wavefront).  gbps: payload bytes
+ * (dim * 4 per row) per second, best of three timed launches; ms: that
+ * launch's time.  Rows of up to 256 floats.  Synchronous. */
+int hh_index_gather_probe(hh_index *idx, uint32_t rounds, double *gbps, float *ms);
 
 /*
  * Batched HNSW search (HnswIndex::search_batch / search_with /

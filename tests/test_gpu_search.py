@@ -122,6 +122,22 @@ def test_search_above_1024_dims_is_unsupported_not_wrong():
                 idx.search_arrays(b[:3], 5, 16)
 
 
+def test_gather_probe_measures_and_refuses_wide_rows():
+    """hh_index_gather_probe times the row gather on the index's rows and
+    reports a positive rate; rows wider than 256 floats are refused."""
+    import ctypes
+    from hnsw_amd import _lib as L
+    b, _ = O.dataset_gaussian(5000, 1, 128, 3)
+    idx = build(b)
+    h = idx.device_index()
+    gbps, ms = ctypes.c_double(), ctypes.c_float()
+    assert L.lib.hh_index_gather_probe(h, 4, ctypes.byref(gbps), ctypes.byref(ms)) == 0, L.last_error()
+    assert gbps.value > 0 and ms.value > 0
+    assert L.lib.hh_index_gather_probe(h, 0, ctypes.byref(gbps), ctypes.byref(ms)) != 0
+    wide = build(np.random.default_rng(2).standard_normal((50, 300)).astype(np.float32))
+    assert L.lib.hh_index_gather_probe(wide.device_index(), 4, ctypes.byref(gbps), None) != 0
+
+
 def test_filter_matches_oracle_and_is_respected():
     """src/hnsw/tests.rs:293-313"""
     b, q = O.dataset_gaussian(2000, 10, 8, 8)
