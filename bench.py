@@ -93,10 +93,12 @@ def main():
     ap.add_argument("--launch-shape", default="",
                     help="CAP,WPB: visited-set capacity and waves per block instead of the "
                          "autotuner's choice")
-    ap.add_argument("--config", choices=["c3", "c4", "c5"], default="",
-                    help="measure one of the other workloads instead of the headline: c3 "
-                         "1M x 960 cosine M=32 in f32/int8/bf16, c4 2M x 256 inner product "
-                         "M=24 filtered mixed batches, c5 10M x 96 L2 in bf16/int8 + serving")
+    ap.add_argument("--workload", choices=["cosine-1m960", "filtered-2m256", "l2-10m96"],
+                    default="",
+                    help="measure one of the other workloads instead of the headline: "
+                         "cosine-1m960 (1M x 960 cosine, M=32, f32/int8/bf16), filtered-2m256 "
+                         "(2M x 256 inner product, M=24, filtered mixed batches), l2-10m96 "
+                         "(10M x 96 L2, M=16, f32/int8/bf16 + serving)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each query is "
@@ -112,7 +114,7 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     threads = args.threads or min(16, os.cpu_count() or 1)
-    if args.config:
+    if args.workload:
         return run_config(args, threads)
     efs = [int(x) for x in args.efs.split(",")]
     n, nq, dim, k = args.n, args.nq, args.dim, args.k
@@ -500,7 +502,7 @@ def run_config(args, threads):
     import hnsw_amd as H
     from hnsw_amd import _lib as L
     from hnsw_amd.device import Buffer
-    out = {"metric": f"query-path measurements, workload {args.config}", "n_gpus": 1,
+    out = {"metric": f"query-path measurements, workload {args.workload}", "n_gpus": 1,
            "data": "synthetic", "higher_is_better": True, "unit": "queries/s"}
 
     def uniform_search(idx, batch, k, ef, prec):
@@ -525,8 +527,8 @@ def run_config(args, threads):
                                for i in range(nf)]))
         return bf, tk, tc, fk, fd, agree
 
-    if args.config in ("c3", "c5"):
-        if args.config == "c3":
+    if args.workload in ("cosine-1m960", "l2-10m96"):
+        if args.workload == "cosine-1m960":
             n, nq, dim, m, metric = args.n, 10_000, 960, 32, H.Metric.Cosine
             log(f"generating {n} x {dim} U(0,1) rows, L2-normalised, + {nq} queries")
             data = H.synthetic.Dataset.unit_uniform(n, nq, dim, 42, threads=threads)
@@ -581,7 +583,7 @@ def run_config(args, threads):
                                "graph_build_s": round(build_s, 1),
                                "quantizer_train_encode_s": round(quant_s, 3),
                                "truth_vs_f64_agreement": round(agree, 5)}})
-        if args.config == "c5":
+        if args.workload == "l2-10m96":
             serve = []
             qs = data.queries()
             for rate in (50_000.0, 200_000.0, 500_000.0):
@@ -609,7 +611,7 @@ def run_config(args, threads):
         print(json.dumps(out), flush=True)
         return
 
-    # c4: filtered mixed batches
+    # filtered-2m256: filtered mixed batches
     n = 2_000_000 if args.n == 1_000_000 else args.n
     nq, dim, m, cats = 50_000, 256, 24, 20
     log(f"generating {n} x {dim} N(0,1) rows with lognormal(0,0.5) norms, {cats} categories")
@@ -695,7 +697,7 @@ def run_config(args, threads):
     print(json.dumps(out), flush=True)
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "search_c2_1m128_traffic.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "search_1m128_l2_traffic.md")
 
 
 def measured_traffic():
