@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--k", type=int, default=10)
-    ap.add_argument("--efs", default="16,32,64,72,96,128,256")
+    ap.add_argument("--efs", default="16,32,64,66,68,72,96,128,256")
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-sample", type=int, default=10000)
     ap.add_argument("--threads", type=int, default=0)
