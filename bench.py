@@ -79,7 +79,13 @@ def main():
     ap.add_argument("--efs", default="16,32,64,66,68,72,96,128,256")
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-sample", type=int, default=10000)
-    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--threads", type=int, default=0,
+                    help="host threads for the CPU baseline and the device-assisted build "
+                         "(default: up to 16)")
+    ap.add_argument("--build-threads", type=int, default=1,
+                    help="host threads for the benchmark graph's build; 1 (the default) builds "
+                         "the same graph every run, so recall at each ef is reproducible (a "
+                         "parallel build, like the crate's insert_batch, is not deterministic)")
     ap.add_argument("--f64-check", type=int, default=500)
     ap.add_argument("--quick", action="store_true",
                     help="search measurement only: no precision variants, serving or CPU "
@@ -127,8 +133,8 @@ def main():
     t0 = time.time()
     for s in range(0, n, chunk):
         e = min(n, s + chunk)
-        idx.insert_batch(keys[s:e], data.base()[s:e], threads=threads)
-        log(f"graph build: {e}/{n} linked ({time.time() - t0:.1f} s, {threads} threads)")
+        idx.insert_batch(keys[s:e], data.base()[s:e], threads=args.build_threads)
+        log(f"graph build: {e}/{n} linked ({time.time() - t0:.1f} s, {args.build_threads} threads)")
     build_s = time.time() - t0
 
     # device buffers and the launch stream, on the library's own HIP runtime
@@ -439,8 +445,10 @@ def main():
         dr = H.lib.hh_recall_at_k(nq, tk.ctypes.data, tc.ctypes.data, k, dk.ctypes.data,
                                   dc.ctypes.data, k, k)
         device_build = {"seconds": round(dev_s, 2), "host_build_s": round(build_s, 2),
-                        "threads": threads, "recall_at_10": round(dr, 4), "ef": ef}
-        log(f"device-assisted build: {dev_s:.1f} s (host build {build_s:.1f} s), recall@{k} "
+                        "host_build_threads": args.build_threads, "threads": threads,
+                        "recall_at_10": round(dr, 4), "ef": ef}
+        log(f"device-assisted build: {dev_s:.1f} s on {threads} host threads (the benchmark "
+            f"graph's host build: {build_s:.1f} s on {args.build_threads}), recall@{k} "
             f"at ef={ef} {dr:.4f} (host-built graph {rec:.4f})")
         del di
     except Exception as e:
@@ -765,6 +773,7 @@ def emit(args, v):
                          "streaming_kernel_s": round(stream_s, 3),
                          "gemm_tflops_2QND": round(gemm_tflops, 2)},
         "graph_build_s": round(build_s, 1),
+        "graph_build_threads": args.build_threads,
         "launch_shape": {"visited_set_capacity": cap.value, "waves_per_block": wpb.value},
     }
     print(json.dumps(line), flush=True)
