@@ -245,3 +245,90 @@ def test_recall_at_k_known_answers():
     assert O.recall_at_k(one, [1], one, [1], 10) == 1.0
     assert O.recall_at_k(np.zeros((0, 1), np.uint64), [], np.zeros((0, 1), np.uint64), [], 10) == 1.0
     assert O.recall_at_k(np.array([[1, 2]], np.uint64), [2], np.zeros((1, 2), np.uint64), [0], 2) == 0.0
+
+
+def _splitmix_py(seed):
+    """Vigna's SplitMix64 in Python integers (pinned above by the crate's KAT
+    and the closed form), used as an independent stream below."""
+    s = seed
+    while True:
+        s = (s + 0x9E3779B97F4A7C15) % 2**64
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) % 2**64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) % 2**64
+        yield z ^ (z >> 31)
+
+
+def test_next_below_is_lemires_multiply_shift():
+    """src/synthetic.rs:43-45: ((u as u128 * n) >> 64) over the raw stream.
+    A modulo reduction, a 32-bit product or a shift of the wrong width each
+    fail this (checked on the last line: the modulo form differs here)."""
+    bounds = [1, 2, 7, 100, 1000, 20, 2**32 - 1, 2**32 + 5, 2**40 + 7, 2**63 + 11, 2**64 - 1]
+    r = O.SplitMix64(271828)
+    raw = _splitmix_py(271828)
+    want, modulo = [], []
+    for i in range(400):
+        n = bounds[i % len(bounds)]
+        u = next(raw)
+        want.append((u * n) >> 64)
+        modulo.append(u % n)
+        assert r.next_below(n) == want[-1], (i, n)
+    assert want != modulo
+
+
+def test_clustered_follows_the_crate_draw_order():
+    """src/synthetic.rs:102-124 restated in Python floats from the raw
+    stream: all centres first (f64 Box-Muller), then per point one Lemire
+    index followed by dim spread draws, base before queries, rounded to f32
+    once.  Python's math.log/cos are the platform libm the crate calls."""
+    base, queries = O.dataset_clustered(6, 4, 5, 3, 99)
+    raw = _splitmix_py(99)
+
+    def f64():
+        return (next(raw) >> 11) * 2.0**-53
+
+    def gauss():
+        u1 = 1.0 - f64()
+        u2 = f64()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(math.tau * u2)
+
+    centers = [gauss() for _ in range(3 * 5)]
+    for out in (base, queries):
+        for row in out:
+            c = (next(raw) * 3) >> 64
+            want = np.array([centers[c * 5 + d] + 0.35 * gauss() for d in range(5)], np.float32)
+            assert np.array_equal(row.view(np.uint32), want.view(np.uint32))
+
+
+def _sq_distance_crate(metric, scale, a, b):
+    """src/quantization.rs:85-92 in numpy: exact integer sum, converted to
+    f32 once (round to nearest even), times s2 = scale*scale in f32; cosine
+    subtracts that product from 1 in f32 (no fused multiply-add)."""
+    a64, b64 = a.astype(np.int64), b.astype(np.int64)
+    s2 = np.float32(scale) * np.float32(scale)
+    if metric == O.L2:
+        return np.float32(int(((a64 - b64) ** 2).sum())) * s2
+    dot = np.float32(int((a64 * b64).sum()))
+    return -(dot * s2) if metric == O.IP else np.float32(1.0) - dot * s2
+
+
+def test_sq_distance_rounds_like_the_crate():
+    """Sums above 2^24 make the i64 -> f32 conversion round; a product taken
+    in f64, a fused cosine step or s2 formed in f64 each change some of
+    these values (the last assertions show the f64 form does)."""
+    rng = np.random.default_rng(5)
+    f64_differs = 0
+    for trial in range(300):
+        dim = int(rng.choice([1, 16, 128, 960, 1024, 4096]))
+        a = rng.integers(-127, 128, dim).astype(np.int8)
+        b = rng.integers(-127, 128, dim).astype(np.int8)
+        scale = np.float32(rng.uniform(1e-3, 0.1))
+        for metric in (O.L2, O.IP, O.COSINE):
+            got = O.sq_distance(metric, scale, a, b)
+            want = _sq_distance_crate(metric, scale, a, b)
+            assert np.array_equal(np.float32(got).view(np.uint32), np.float32(want).view(np.uint32)), \
+                (trial, dim, metric)
+        l2 = int(((a.astype(np.int64) - b.astype(np.int64)) ** 2).sum())
+        once = np.float32(float(l2) * float(scale) * float(scale))
+        f64_differs += int(once != _sq_distance_crate(O.L2, scale, a, b))
+    assert f64_differs > 0
