@@ -34,9 +34,20 @@ int graph_view(const hh_graph *g, hh_graph_desc *d);
 int index_device(const hh_index *idx);
 uint32_t index_dim(const hh_index *idx);
 uint64_t index_nodes(const hh_index *idx);
-// Wavefronts the search kernel keeps resident on the whole GPU at ef <= 64.
-uint32_t index_resident_waves(hh_index *idx);
+// Wavefronts the search kernel keeps resident on the whole GPU at the
+// launch shape of ef's list-size bucket (ef <= 64, 128, 256, 512).
+uint32_t index_resident_waves(hh_index *idx, uint32_t ef = 64);
 int index_reserve(hh_index *idx, uint64_t nq, uint32_t count);
+// Serving loops open on the index (changes of the index are refused while
+// any is): +1 at open, -1 at close.
+void index_serving(hh_index *idx, int delta);
+// hh_search_batch_spec_device with the filter table's own stride in words
+// (a serving loop's table), which must cover the index's nodes.
+int search_spec_device_strided(hh_index *idx, const float *queries, uint64_t nq,
+                               const hh_query_spec *specs, uint32_t k_stride, uint32_t ef_max,
+                               const uint32_t *filters, uint32_t nfilters, uint64_t filter_words,
+                               uint64_t *out_keys, float *out_dist, uint32_t *out_count,
+                               int32_t *status, void *stream);
 
 // Row layout in HBM.  A dimension splits as D = 32*T + 8*U + R (the block
 // structure of the crate's AVX2 kernel, src/kernels/x86.rs:47-77).  The

@@ -191,6 +191,99 @@ __global__ void __launch_bounds__(256) sq_encode_kernel(SqArgs a) {
   }
 }
 
+// ------------------------------------------- per-dimension min/max codes
+
+// Monotone int key of the f32 total order (-0 below +0).
+__device__ __forceinline__ int32_t pd_key(float f) {
+  const int32_t i = __float_as_int(f);
+  return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1);
+}
+__device__ __forceinline__ float pd_unkey(int32_t k) {
+  return __int_as_float(k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1));
+}
+// Row position of dimension d (the inverse of dim_at).
+__device__ __forceinline__ uint32_t pos_of(uint32_t d, uint32_t T) {
+  if (d >= 32u * T) return d;
+  const uint32_t t = d >> 5, j = d & 31, b = t >> 2;
+  const uint32_t tw = T - 4 * b < 4 ? T - 4 * b : 4;
+  return b * 128 + j * tw + (t & 3);
+}
+
+__global__ void __launch_bounds__(256) pd_minmax_kernel(PdArgs a) {
+  // thread t of a block owns one position (group) over a chunk of rows
+  const uint32_t s4 = a.stride / 4;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * kChunk; r0 < a.n; r0 += (uint64_t)gridDim.x * kChunk) {
+    const uint32_t nr = a.n - r0 < kChunk ? (uint32_t)(a.n - r0) : kChunk;
+    gptr<float4> base = (gptr<float4>)(a.rows + r0 * a.stride);
+    for (uint32_t c = threadIdx.x; c < s4; c += 256) {
+      int32_t lo[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+      int32_t hi[4] = {INT32_MIN, INT32_MIN, INT32_MIN, INT32_MIN};
+      for (uint32_t r = 0; r < nr; r++) {
+        const float4 x = base[(size_t)r * s4 + c];
+        const int32_t k[4] = {pd_key(x.x), pd_key(x.y), pd_key(x.z), pd_key(x.w)};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          lo[u] = min(lo[u], k[u]);
+          hi[u] = max(hi[u], k[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        atomicMin(a.lo_key + 4 * c + u, lo[u]);
+        atomicMax(a.hi_key + 4 * c + u, hi[u]);
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) pd_params_kernel(PdArgs a) {
+  const uint32_t d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= a.dim) return;
+  const uint32_t p = pos_of(d, a.T);
+  const float lo = pd_unkey(G(a.lo_key)[p]), hi = pd_unkey(G(a.hi_key)[p]);
+  const float s = __fdiv_rn(__fsub_rn(hi, lo), 254.0f);
+  GM(a.scale)[d] = s > 0.0f ? s : 1.0f;
+  GM(a.mid)[d] = __fadd_rn(__fmul_rn(__fadd_rn(lo, hi), 0.5f), 0.0f);
+}
+
+// One wavefront per row: lane l takes dimensions l, l + 64, ...; the code
+// row is written whole (padding zero); the L2 row term sums the lanes'
+// f64 squares in dimension order by a uniform chain.
+__global__ void __launch_bounds__(256) pd_encode_kernel(PdArgs a) {
+  const int lane = (int)(threadIdx.x & 63);
+  const uint64_t waves = (uint64_t)gridDim.x * 4;
+  for (uint64_t r = a.first + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); r < a.n; r += waves) {
+    gptr<float> row = G(a.rows) + r * a.stride;
+    gmut<int8_t> out = GM(a.codes) + r * a.cstride;
+    double acc = 0.0;
+    for (uint32_t d0 = 0; d0 < a.cstride; d0 += 64) {
+      const uint32_t d = d0 + (uint32_t)lane;
+      int8_t c = 0;
+      if (d < a.dim) {
+        const float x = row[pos_of(d, a.T)];
+        const float inv = __fdiv_rn(1.0f, G(a.scale)[d]);
+        float v = roundf(__fmul_rn(__fsub_rn(x, G(a.mid)[d]), inv));
+        v = fminf(127.0f, fmaxf(-127.0f, v));
+        c = (int8_t)v;
+      }
+      out[d] = c;
+      if (a.rnorm && d0 < a.dim) {
+        const double y = d < a.dim ? __dmul_rn((double)G(a.scale)[d], (double)c) : 0.0;
+        const double sq = __dmul_rn(y, y);
+        const int cnt = a.dim - d0 < 64 ? (int)(a.dim - d0) : 64;
+        const int lo = __double2loint(sq), hi = __double2hiint(sq);
+        for (int l = 0; l < cnt; l++)
+          acc = __dadd_rn(acc, __hiloint2double(__builtin_amdgcn_readlane(hi, l),
+                                                __builtin_amdgcn_readlane(lo, l)));
+      }
+    }
+    if (a.rnorm) {
+      const float rn = (float)acc;
+      if (lane == 0) GM(a.rnorm)[r] = rn;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(64) pack_codes_kernel(const int8_t *src, uint32_t cstride,
                                                         uint32_t dim, uint64_t rows, int8_t *dst) {
   for (uint64_t r = blockIdx.x; r < rows; r += gridDim.x)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <tuple>
 #include <cmath>
@@ -11,6 +12,7 @@
 #include <condition_variable>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <unordered_set>
 #include <vector>
@@ -122,6 +124,13 @@ constexpr uint32_t kCandPool = 65536;
 // Nodes a query may mark in its HBM visited bitset (after leaving the LDS
 // table) and still clear word by word at its end; beyond, the whole bitset.
 constexpr uint32_t kSpillLog = 4096;
+// Re-run pass of queries whose pool dropped a candidate: pools of n
+// entries, as many as fit this many bytes (and at most kBigPoolSlots
+// wavefronts); allocated on the first search that could need them.
+constexpr uint64_t kBigPoolBytes = 1ull << 30;
+constexpr uint64_t kBigPoolSlots = 256;
+// search counters (hh_search_stats)
+constexpr int kStatCount = 9;
 
 // prepare_queries_kernel: four queries (one per wavefront) per block
 int prep_grid(uint64_t nq) {
@@ -241,6 +250,12 @@ struct Workspace {
   uint64_t *cpool = nullptr;  // per-wave candidate pools (pool_cap entries each)
   uint32_t pool_cap = 0;
   uint32_t *slog = nullptr;   // per-wave spill logs (kSpillLog entries)
+  uint32_t *redo = nullptr;   // queries to re-run with the large pools (cap_q)
+  float2 *qpd = nullptr;      // per-query (sigma, qconst) of per-dimension codes (cap_q)
+  // Pools of `big_cap` >= n entries for the re-run of queries whose pool
+  // dropped a candidate: big_slots of them (the re-run's wavefronts).
+  uint64_t *bigpool = nullptr;
+  uint32_t big_slots = 0, big_cap = 0;
   uint64_t spill_slots = 0;
   uint64_t spill_wcap = 0;   // bitset words per slot the allocation holds
   // brute-force partial lists
@@ -279,9 +294,9 @@ struct Workspace {
   int init() {
     HH_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     int st = dalloc(&counter, 4);
-    if (!st) st = dalloc(&stats, 8);
+    if (!st) st = dalloc(&stats, kStatCount);
     if (st) return st;
-    HH_HIP(hipMemset(stats, 0, 8 * sizeof(unsigned long long)));
+    HH_HIP(hipMemset(stats, 0, kStatCount * sizeof(unsigned long long)));
     HH_HIP(hipMemset(counter, 0, 4 * sizeof(uint32_t)));
     return HH_OK;
   }
@@ -297,6 +312,9 @@ struct Workspace {
     dfree(spill);
     dfree(cpool);
     dfree(slog);
+    dfree(redo);
+    dfree(qpd);
+    dfree(bigpool);
     dfree(part_d);
     dfree(part_d64);
     dfree(part_k);
@@ -324,12 +342,16 @@ struct Workspace {
     dfree(status);
     dfree(seed);
     dfree(order);
+    dfree(redo);
+    dfree(qpd);
     uint64_t cap = std::max<uint64_t>(nq, 1024);
     int st = dalloc(&qprep, cap * stride);
     if (!st) st = dalloc(&qcode, cap * std::max<uint32_t>(cstride, 1));
     if (!st) st = dalloc(&status, cap);
     if (!st) st = dalloc(&seed, cap);
     if (!st) st = dalloc(&order, cap);
+    if (!st) st = dalloc(&redo, cap);
+    if (!st) st = dalloc(&qpd, cap);
     if (st) {
       cap_q = q_stride = q_cstride = 0;
       return st;
@@ -362,6 +384,24 @@ struct Workspace {
     spill_slots = slots;
     spill_wcap = words;
     pool_cap = pcap;
+    return HH_OK;
+  }
+  // Pools for the re-run pass: n entries each (a query's candidates are
+  // distinct nodes, so such a pool never drops one), as many as fit
+  // kBigPoolBytes, a multiple of four (the largest waves per block; the
+  // re-run's grid is a whole number of blocks) and at least four.
+  int ensure_bigpool(uint64_t n) {
+    const uint64_t cap = (std::max<uint64_t>(n, 64) + 63) / 64 * 64;
+    const uint64_t fit = std::min<uint64_t>(kBigPoolBytes / (cap * 8), kBigPoolSlots);
+    const uint64_t want = std::max<uint64_t>(4, fit / 4 * 4);
+    if (bigpool && cap <= big_cap && want <= big_slots) return HH_OK;
+    HH_HIP(hipEventSynchronize(done));
+    dfree(bigpool);
+    big_slots = big_cap = 0;
+    int st = dalloc(&bigpool, want * cap);
+    if (st) return st;
+    big_slots = (uint32_t)want;
+    big_cap = (uint32_t)cap;
     return HH_OK;
   }
   int ensure_part(uint64_t entries, uint64_t lists, bool f64) {
@@ -437,6 +477,18 @@ struct hh_index {
   float *sq_offset = nullptr;
   float sq_scale = 1.0f;
   std::vector<float> sq_offset_host;  // the resident quantizer's offsets
+  // per-dimension min/max codes (hh_index_quantize_perdim)
+  int8_t *pd_codes = nullptr;
+  float *pd_rnorm = nullptr;          // L2 only
+  float *pd_mid = nullptr, *pd_scale = nullptr;
+  // Searches hold mu shared while they read the index's arrays into a
+  // launch; every change of the resident index (sync, quantizers, bf16
+  // rows, tombstones) holds it exclusively, waits for the launches in
+  // flight and bumps `generation`, which graph-captured plans check.
+  // Changes are refused while a serving loop is open on the index.
+  std::shared_mutex mu;
+  std::atomic<uint64_t> generation{0};
+  std::atomic<int> servers{0};
   // capacities for hh_index_sync (per-node arrays, upper link words)
   uint64_t cap_n = 0, upper_words = 0, cap_words = 0;
   // Workspaces: one search runs in a workspace at a time; calls on
@@ -473,6 +525,10 @@ struct hh_index {
     dfree(keys);
     dfree(deleted_bits);
     dfree(sq_offset);
+    dfree(pd_codes);
+    dfree(pd_rnorm);
+    dfree(pd_mid);
+    dfree(pd_scale);
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -585,6 +641,8 @@ struct SearchCall {
   // and tombstoned nodes are candidates like any other
   bool build = false;
   uint64_t rows_first = 0;
+  // words per filter bitmap in `filters` (0: the index's (n + 31) / 32)
+  uint64_t filter_words = 0;
 };
 
 // Largest max(ef, k) a launch must serve: the call's own for a uniform
@@ -599,7 +657,7 @@ int check_call(hh_index *idx, const SearchCall &c) {
   if (!c.queries || !c.out_count || (c.k_stride > 0 && (!c.out_keys || !c.out_dist)))
     return fail(HH_INVALID_ARGUMENT, "null device pointer");
   if (!tmax_for(idx->lay)) return fail(HH_UNSUPPORTED, "search kernels support dim <= 1024");
-  if (!c.specs && c.precision > 2) return fail(HH_INVALID_ARGUMENT, "unknown precision");
+  if (!c.specs && c.precision > 3) return fail(HH_INVALID_ARGUMENT, "unknown precision");
   const uint32_t efmax = spec_ef_bound(c);
   if (!slots_for(std::max<uint32_t>(efmax, 1)))
     return fail(HH_UNSUPPORTED, "ef (raised to k) above 512 is not supported");
@@ -690,9 +748,13 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   pa.stride = idx->lay.stride;
   pa.cstride = idx->cstride;
   pa.qprep = ws.qprep;
-  pa.qcode = idx->codes ? ws.qcode : nullptr;
+  pa.qcode = idx->codes || idx->pd_codes ? ws.qcode : nullptr;
   pa.sq_offset = idx->sq_offset;
   pa.sq_scale = idx->sq_scale;
+  pa.pd_mid = idx->pd_mid;
+  pa.pd_scale = idx->pd_scale;
+  pa.qpd = ws.qpd;
+  pa.has_pd = idx->pd_codes != nullptr;
   pa.precision = c.precision;
   pa.specs = c.specs;
   pa.k_stride = c.k_stride;
@@ -709,7 +771,7 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
     hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(c.nq)), dim3(256), 0, stream,
                        reinterpret_cast<uint32_t *>(pa.status), c.nq, 0u);
     hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(1), dim3(64), 0, stream, ws.counter,
-                       (uint64_t)1, 0u);
+                       (uint64_t)4, 0u);
   } else {
     hipLaunchKernelGGL(hhk::prepare_queries_kernel, dim3(prep_grid(c.nq)), dim3(256), 0, stream,
                        pa);
@@ -740,8 +802,11 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.stride = idx->lay.stride;
   a.cstride = idx->cstride;
   a.sq_scale = idx->sq_scale;
+  a.pd_codes = idx->pd_codes;
+  a.pd_rnorm = idx->pd_rnorm;
   a.qprep = ws.qprep;
   a.qcode = ws.qcode;
+  a.qpd = ws.qpd;
   a.status = pa.status;
   a.nq = c.nq;
   a.specs = c.specs;
@@ -750,7 +815,7 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.precision = c.precision;
   a.allow = c.allow;
   a.filters = c.filters;
-  a.filter_words = (uint32_t)((idx->n + 31) / 32);
+  a.filter_words = (uint32_t)(c.filter_words ? c.filter_words : (idx->n + 31) / 32);
   a.k_stride = c.k_stride;
   a.out_keys = c.out_keys;
   a.out_dist = c.out_dist;
@@ -768,6 +833,11 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.lds_words = cfg.lds_words;
   a.seed = ws.seed;
   a.order = ws.order;
+  // a pool smaller than the node count can drop a candidate: such queries
+  // are listed and re-run below with pools of n entries
+  const bool redo = ws.pool_cap < idx->n && ws.bigpool;
+  a.redo = redo ? ws.redo : nullptr;
+  a.redo_count = ws.counter + 2;
   void *args[] = {&a};
   if (before_search) HH_HIP(hipEventRecord(before_search, stream));
   const dim3 grid((unsigned)cfg.blocks), block(cfg.wpb * 64);
@@ -811,6 +881,16 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
     a.phase = 2;
   }
   HH_HIP(hipLaunchKernel(cfg.kern, grid, block, args, lds, stream));
+  if (redo) {
+    // the re-run: layer 0 of the listed queries from their seeds, on
+    // big_slots wavefronts whose pools hold n entries each
+    const uint64_t slots = std::min<uint64_t>(ws.big_slots, waves);
+    a.phase = 3;
+    a.cpool = ws.bigpool;
+    a.cpool_cap = ws.big_cap;
+    const dim3 rgrid((unsigned)((slots + cfg.wpb - 1) / cfg.wpb));
+    HH_HIP(hipLaunchKernel(cfg.kern, rgrid, block, args, lds, stream));
+  }
   return HH_OK;
 }
 
@@ -865,6 +945,7 @@ int launch_search(hh_index *idx, const float *queries, uint64_t nq, uint32_t k, 
 
 int launch_call(hh_index *idx, const SearchCall &c, hipStream_t stream) {
   const uint64_t nq = c.nq;
+  std::shared_lock<std::shared_mutex> rd(idx->mu);
   int st = check_call(idx, c);
   if (st) return st;
   HH_HIP(hipSetDevice(idx->device));
@@ -880,6 +961,7 @@ int launch_call(hh_index *idx, const SearchCall &c, hipStream_t stream) {
   } rel{idx, ws};
   st = ws->ensure_queries(nq, idx->lay.stride, idx->cstride);
   if (!st) st = ws->ensure_spill(cfg.blocks * cfg.wpb, cfg.spill_words);
+  if (!st && ws->pool_cap < idx->n) st = ws->ensure_bigpool(idx->n);
   if (st) return st;
   HH_HIP(hipStreamWaitEvent(stream, ws->done, 0));
   st = enqueue_search(idx, *ws, c, cfg, stream, idx->timing ? idx->t0 : nullptr);
@@ -895,14 +977,31 @@ namespace hh {
 int index_device(const hh_index *idx) { return idx->device; }
 uint32_t index_dim(const hh_index *idx) { return idx->dim; }
 uint64_t index_nodes(const hh_index *idx) { return idx->n; }
-uint32_t index_resident_waves(hh_index *idx) {
+uint32_t index_resident_waves(hh_index *idx, uint32_t ef) {
   SearchCall c{};
   c.nq = 1u << 30;
-  c.k = 10;
-  c.ef = 64;
+  c.k = 1;
+  c.ef = std::max<uint32_t>(1, std::min<uint32_t>(ef, 512));
   SearchConfig cfg;
   if (!tmax_for(idx->lay) || search_config(idx, c, cfg)) return 1024;
   return (uint32_t)std::max<uint64_t>(1, cfg.blocks * cfg.wpb);
+}
+
+void index_serving(hh_index *idx, int delta) { idx->servers += delta; }
+
+int search_spec_device_strided(hh_index *idx, const float *queries, uint64_t nq,
+                               const hh_query_spec *specs, uint32_t k_stride, uint32_t ef_max,
+                               const uint32_t *filters, uint32_t nfilters, uint64_t filter_words,
+                               uint64_t *out_keys, float *out_dist, uint32_t *out_count,
+                               int32_t *status, void *stream) {
+  if (!idx || !specs) return fail(HH_INVALID_ARGUMENT, "null argument");
+  if (nq == 0) return HH_OK;
+  if (filters && filter_words < (idx->n + 31) / 32)
+    return fail(HH_INVALID_ARGUMENT, "filter bitmaps shorter than the index");
+  SearchCall c{queries, nq, 0u, ef_max, 0u, nullptr, specs, k_stride, filters, nfilters,
+               out_keys, out_dist, out_count, status};
+  c.filter_words = filters ? filter_words : 0;
+  return launch_call(idx, c, (hipStream_t)stream);
 }
 
 // Sizes `count` of the index's workspaces for launches of up to nq queries
@@ -921,6 +1020,7 @@ int index_reserve(hh_index *idx, uint64_t nq, uint32_t count) {
     held.push_back(w);
     st = w->ensure_queries(nq, idx->lay.stride, idx->cstride);
     if (!st) st = w->ensure_spill(slots, words);
+    if (!st && w->pool_cap < idx->n) st = w->ensure_bigpool(idx->n);
   }
   for (Workspace *w : held) release_ws(idx, w);
   return st;
@@ -928,6 +1028,15 @@ int index_reserve(hh_index *idx, uint64_t nq, uint32_t count) {
 }  // namespace hh
 
 namespace {
+
+// Changes of the resident index are refused while a serving loop is open
+// on it (its launches read the arrays a change would move or free).
+int refuse_if_served(hh_index *idx) {
+  if (idx->servers.load() > 0)
+    return fail(HH_INVALID_ARGUMENT,
+                "a serving loop is open on this index: close it before changing the index");
+  return HH_OK;
+}
 
 // Waits for every launch on the index's workspaces.
 int drain(hh_index *idx) {
@@ -1121,6 +1230,9 @@ void hh_index_free(hh_index *idx) { delete idx; }
 
 int hh_index_update_deleted(hh_index *idx, const uint8_t *deleted, uint64_t live) {
   if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  std::unique_lock<std::shared_mutex> wr(idx->mu);
+  if (int st = refuse_if_served(idx)) return st;
+  idx->generation++;
   if (idx->n == 0) return HH_OK;
   HH_HIP(hipSetDevice(idx->device));
   std::vector<uint32_t> bits;
@@ -1134,7 +1246,10 @@ int hh_index_update_deleted(hh_index *idx, const uint8_t *deleted, uint64_t live
 
 int hh_index_enable_bf16(hh_index *idx) {
   if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  std::unique_lock<std::shared_mutex> wr(idx->mu);
   if (idx->bf16 || idx->n == 0) return HH_OK;
+  if (int st = refuse_if_served(idx)) return st;
+  idx->generation++;
   HH_HIP(hipSetDevice(idx->device));
   const uint64_t count = idx->n * idx->lay.stride;
   int st = dalloc(&idx->bf16, std::max(idx->cap_n, idx->n) * idx->lay.stride);
@@ -1147,6 +1262,9 @@ int hh_index_enable_bf16(hh_index *idx) {
 
 int hh_index_quantize(hh_index *idx, float *offset, float *scale) {
   if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  std::unique_lock<std::shared_mutex> wr(idx->mu);
+  if (int st = refuse_if_served(idx)) return st;
+  idx->generation++;
   HH_HIP(hipSetDevice(idx->device));
   int st = drain(idx);
   if (st) return st;
@@ -1223,6 +1341,9 @@ int hh_index_quantize(hh_index *idx, float *offset, float *scale) {
 
 int hh_index_sync(hh_index *idx, const hh_graph_desc *d, const uint32_t *dirty, uint64_t ndirty) {
   if (!idx || !d || (ndirty && !dirty)) return fail(HH_INVALID_ARGUMENT, "null argument");
+  std::unique_lock<std::shared_mutex> wr(idx->mu);
+  if (int st = refuse_if_served(idx)) return st;
+  idx->generation++;
   if (d->dim != idx->dim || d->metric != idx->metric || d->m != idx->m || d->m0 != idx->m0)
     return fail(HH_INVALID_ARGUMENT, "graph does not match the resident index: upload it again");
   if (d->n < idx->n) return fail(HH_INVALID_ARGUMENT, "graph has fewer nodes than the resident index");
@@ -1280,6 +1401,8 @@ int hh_index_sync(hh_index *idx, const hh_graph_desc *d, const uint32_t *dirty, 
     st = grow(&idx->rows, n0 * stride, cap * stride);
     if (!st && idx->bf16) st = grow(&idx->bf16, n0 * stride, cap * stride);
     if (!st && codes) st = grow(&idx->codes, n0 * idx->cstride, cap * idx->cstride);
+    if (!st && idx->pd_codes) st = grow(&idx->pd_codes, n0 * idx->cstride, cap * idx->cstride);
+    if (!st && idx->pd_rnorm) st = grow(&idx->pd_rnorm, n0, cap);
     if (!st) st = grow(&idx->adj0, n0 * m0, cap * m0);
     if (!st) st = grow(&idx->upper_off, n0, cap);
     if (!st) st = grow(&idx->keys, n0, cap);
@@ -1307,6 +1430,13 @@ int hh_index_sync(hh_index *idx, const hh_graph_desc *d, const uint32_t *dirty, 
         std::memcpy(packed.data() + i * idx->cstride, d->codes + (n0 + i) * dim, dim);
       HH_HIP(hipMemcpy(idx->codes + n0 * idx->cstride, packed.data(), packed.size(),
                        hipMemcpyHostToDevice));
+    }
+    if (idx->pd_codes) {
+      // new rows take the resident per-dimension quantizer's codes
+      hhk::PdArgs pa{idx->rows, n0, n1, stride, dim, idx->lay.T, idx->cstride, idx->metric,
+                     nullptr, nullptr, idx->pd_mid, idx->pd_scale, idx->pd_codes, idx->pd_rnorm};
+      hipLaunchKernelGGL(hhk::pd_encode_kernel, dim3(grid_for(add * 64)), dim3(256), 0, 0, pa);
+      HH_HIP(hipGetLastError());
     }
     HH_HIP(hipMemcpy(idx->adj0 + n0 * m0, ell_new.data(), ell_new.size() * 4,
                      hipMemcpyHostToDevice));
@@ -1395,12 +1525,9 @@ int hh_index_build_candidates(hh_index *idx, uint64_t first, uint64_t count, uin
   return HH_OK;
 }
 
-int hh_index_download_codes(hh_index *idx, int8_t *codes) {
-  if (!idx || !codes) return fail(HH_INVALID_ARGUMENT, "null argument");
-  if (!idx->codes) return fail(HH_UNSUPPORTED, "no quantizer codes resident");
-  if (idx->n == 0) return HH_OK;
-  HH_HIP(hipSetDevice(idx->device));
-  if (int st = drain(idx)) return st;
+namespace {
+// Copies n code rows (cstride bytes apart on the device) into n*dim bytes.
+int download_code_rows(hh_index *idx, const int8_t *src, int8_t *codes) {
   // rows are cstride bytes apart on the device: pack chunks of them densely
   // there (one kernel), then copy each chunk in one piece
   const uint64_t chunk = std::min<uint64_t>(idx->n, std::max<uint64_t>(1, (64ull << 20) / idx->dim));
@@ -1410,7 +1537,7 @@ int hh_index_download_codes(hh_index *idx, int8_t *codes) {
   for (uint64_t r0 = 0; r0 < idx->n && e == hipSuccess; r0 += chunk) {
     const uint64_t rows = std::min(chunk, idx->n - r0);
     hipLaunchKernelGGL(hhk::pack_codes_kernel, dim3((unsigned)std::min<uint64_t>(rows, 65536)),
-                       dim3(64), 0, idx->stream, idx->codes + r0 * idx->cstride, idx->cstride,
+                       dim3(64), 0, idx->stream, src + r0 * idx->cstride, idx->cstride,
                        idx->dim, rows, stage);
     e = hipGetLastError();
     if (e == hipSuccess)
@@ -1423,13 +1550,136 @@ int hh_index_download_codes(hh_index *idx, int8_t *codes) {
     return fail(HH_DEVICE_ERROR, std::string("code download: ") + hipGetErrorString(e));
   return HH_OK;
 }
+}  // namespace
+
+int hh_index_download_codes(hh_index *idx, int8_t *codes) {
+  if (!idx || !codes) return fail(HH_INVALID_ARGUMENT, "null argument");
+  std::shared_lock<std::shared_mutex> rd(idx->mu);
+  if (!idx->codes) return fail(HH_UNSUPPORTED, "no quantizer codes resident");
+  if (idx->n == 0) return HH_OK;
+  HH_HIP(hipSetDevice(idx->device));
+  if (int st = drain(idx)) return st;
+  return download_code_rows(idx, idx->codes, codes);
+}
+
+int hh_index_quantize_perdim(hh_index *idx, float *mid, float *scale) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  std::unique_lock<std::shared_mutex> wr(idx->mu);
+  if (int st = refuse_if_served(idx)) return st;
+  idx->generation++;
+  HH_HIP(hipSetDevice(idx->device));
+  int st = drain(idx);
+  if (st) return st;
+  const uint32_t dim = idx->dim, stride = idx->lay.stride;
+  const uint32_t cstride = (dim + 127) / 128 * 128;
+  const uint64_t n = idx->n, rows_cap = std::max<uint64_t>({idx->cap_n, n, 1});
+  hipStream_t s = idx->stream;
+  int8_t *codes = nullptr;
+  float *rnorm = nullptr, *md = nullptr, *sc = nullptr;
+  int32_t *lo = nullptr, *hi = nullptr;
+  auto release = [&] {
+    dfree(codes);
+    dfree(rnorm);
+    dfree(md);
+    dfree(sc);
+    dfree(lo);
+    dfree(hi);
+  };
+  st = dalloc(&codes, rows_cap * cstride);
+  if (!st && idx->metric == HH_L2) st = dalloc(&rnorm, rows_cap);
+  if (!st) st = dalloc(&md, std::max<uint32_t>(dim, 1));
+  if (!st) st = dalloc(&sc, std::max<uint32_t>(dim, 1));
+  if (!st) st = dalloc(&lo, stride);
+  if (!st) st = dalloc(&hi, stride);
+  if (st) {
+    release();
+    return st;
+  }
+  hhk::PdArgs a{idx->rows, 0, n, stride, dim, idx->lay.T, cstride, idx->metric, lo, hi, md, sc,
+                codes, rnorm};
+  std::vector<float> mid_h(dim, 0.0f), scale_h(dim, 1.0f);
+  hipError_t e = hipSuccess;
+  if (n > 0) {
+    hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(stride)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t *>(lo), (uint64_t)stride, 0x7FFFFFFFu);
+    hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(stride)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t *>(hi), (uint64_t)stride, 0x80000000u);
+    hipLaunchKernelGGL(hhk::pd_minmax_kernel, dim3((unsigned)std::min<uint64_t>((n + 1023) / 1024, 4096)),
+                       dim3(256), 0, s, a);
+    hipLaunchKernelGGL(hhk::pd_params_kernel, dim3((dim + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(hhk::pd_encode_kernel, dim3(grid_for(n * 64)), dim3(256), 0, s, a);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(mid_h.data(), md, dim * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(scale_h.data(), sc, dim * 4, hipMemcpyDeviceToHost, s);
+  } else {
+    // no rows: the rule's value for an empty set (mid 0, scale 1)
+    e = hipMemcpyAsync(md, mid_h.data(), dim * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(sc, scale_h.data(), dim * 4, hipMemcpyHostToDevice, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    release();
+    return fail(HH_DEVICE_ERROR, std::string("per-dimension quantizer: ") + hipGetErrorString(e));
+  }
+  dfree(idx->pd_codes);
+  dfree(idx->pd_rnorm);
+  dfree(idx->pd_mid);
+  dfree(idx->pd_scale);
+  idx->pd_codes = codes;
+  idx->pd_rnorm = rnorm;
+  idx->pd_mid = md;
+  idx->pd_scale = sc;
+  idx->cstride = cstride;
+  codes = nullptr;
+  rnorm = md = sc = nullptr;
+  release();
+  if (mid && dim) std::memcpy(mid, mid_h.data(), (size_t)dim * 4);
+  if (scale && dim) std::memcpy(scale, scale_h.data(), (size_t)dim * 4);
+  return HH_OK;
+}
+
+int hh_index_download_perdim(hh_index *idx, int8_t *codes, float *rnorm) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  std::shared_lock<std::shared_mutex> rd(idx->mu);
+  if (!idx->pd_codes) return fail(HH_UNSUPPORTED, "no per-dimension codes resident");
+  if (idx->n == 0) return HH_OK;
+  HH_HIP(hipSetDevice(idx->device));
+  if (int st = drain(idx)) return st;
+  if (codes)
+    if (int st = download_code_rows(idx, idx->pd_codes, codes)) return st;
+  if (rnorm) {
+    if (!idx->pd_rnorm) return fail(HH_UNSUPPORTED, "row terms are kept for L2 only");
+    HH_HIP(hipMemcpy(rnorm, idx->pd_rnorm, idx->n * 4, hipMemcpyDeviceToHost));
+  }
+  return HH_OK;
+}
+
+int hh_index_download_bf16(hh_index *idx, uint16_t *out) {
+  if (!idx || !out) return fail(HH_INVALID_ARGUMENT, "null argument");
+  std::shared_lock<std::shared_mutex> rd(idx->mu);
+  if (!idx->bf16) return fail(HH_UNSUPPORTED, "bf16 rows not resident");
+  HH_HIP(hipSetDevice(idx->device));
+  if (int st = drain(idx)) return st;
+  const uint32_t dim = idx->dim, stride = idx->lay.stride;
+  const uint64_t chunk = std::max<uint64_t>(1, (32ull << 20) / stride);
+  std::vector<uint16_t> stage;
+  for (uint64_t r0 = 0; r0 < idx->n; r0 += chunk) {
+    const uint64_t rows = std::min(chunk, idx->n - r0);
+    stage.resize(rows * stride);
+    HH_HIP(hipMemcpy(stage.data(), idx->bf16 + r0 * stride, rows * stride * 2,
+                     hipMemcpyDeviceToHost));
+    for (uint64_t r = 0; r < rows; r++)
+      for (uint32_t d = 0; d < dim; d++) out[(r0 + r) * dim + d] = stage[r * stride + idx->lay.pos(d)];
+  }
+  return HH_OK;
+}
 
 int hh_index_stats(hh_index *idx, hh_search_stats *out) {
   if (!idx || !out) return fail(HH_INVALID_ARGUMENT, "null argument");
   HH_HIP(hipSetDevice(idx->device));
   int st = drain(idx);
   if (st) return st;
-  unsigned long long v[8] = {0};
+  unsigned long long v[kStatCount] = {0};
   HH_HIP(hipMemcpy(v, idx->stats_ws.stats, sizeof v, hipMemcpyDeviceToHost));
   out->queries = v[0];
   out->n_dist = v[1];
@@ -1439,6 +1689,7 @@ int hh_index_stats(hh_index *idx, hh_search_stats *out) {
   out->n_overflow = v[5];
   out->n_visited_spill = v[6];
   out->n_cand_spill = v[7];
+  out->n_redo = v[8];
   return HH_OK;
 }
 
@@ -1607,7 +1858,7 @@ int hh_index_reset_stats(hh_index *idx) {
   HH_HIP(hipSetDevice(idx->device));
   int st = drain(idx);
   if (st) return st;
-  HH_HIP(hipMemset(idx->stats_ws.stats, 0, 8 * sizeof(unsigned long long)));
+  HH_HIP(hipMemset(idx->stats_ws.stats, 0, kStatCount * sizeof(unsigned long long)));
   return HH_OK;
 }
 
@@ -1759,6 +2010,7 @@ int hh_search_batch_spec(hh_index *idx, const float *queries, uint64_t nq,
 
 struct hh_search_plan {
   hh_index *idx = nullptr;
+  uint64_t generation = 0;  // the index's generation the graph was captured at
   Workspace ws;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1786,16 +2038,19 @@ int hh_search_plan_create(hh_index *idx, const float *queries, uint64_t nq, uint
                      specs ? nullptr : allow_bits, specs, specs ? k_stride : k,
                      specs ? filters : nullptr, specs ? nfilters : 0u, out_keys, out_dist,
                      out_count, status};
+  std::shared_lock<std::shared_mutex> rd(idx->mu);
   int st = check_call(idx, c);
   if (st) return st;
   HH_HIP(hipSetDevice(idx->device));
   std::unique_ptr<hh_search_plan> p(new hh_search_plan());
   p->idx = idx;
+  p->generation = idx->generation.load();
   SearchConfig cfg;
   st = search_config(idx, c, cfg);
   if (!st) st = p->ws.init();
   if (!st) st = p->ws.ensure_queries(nq, idx->lay.stride, idx->cstride);
   if (!st) st = p->ws.ensure_spill(cfg.blocks * cfg.wpb, cfg.spill_words);
+  if (!st && p->ws.pool_cap < idx->n) st = p->ws.ensure_bigpool(idx->n);
   if (st) return st;
   HH_HIP(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
   HH_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
@@ -1812,6 +2067,10 @@ int hh_search_plan_create(hh_index *idx, const float *queries, uint64_t nq, uint
 
 int hh_search_plan_launch(hh_search_plan *p, void *stream) {
   if (!p) return fail(HH_INVALID_ARGUMENT, "null plan");
+  std::shared_lock<std::shared_mutex> rd(p->idx->mu);
+  if (p->generation != p->idx->generation.load())
+    return fail(HH_INVALID_ARGUMENT,
+                "the index changed since the plan was created: create the plan again");
   HH_HIP(hipSetDevice(p->idx->device));
   hipStream_t s = (hipStream_t)stream;
   HH_HIP(hipStreamWaitEvent(s, p->ws.done, 0));

@@ -26,7 +26,7 @@ using namespace hhd;
 namespace hhk {
 
 struct Stats {
-  unsigned long long q, dist, up, exp, rr, ovf, spill, cspill;
+  unsigned long long q, dist, up, exp, rr, ovf, spill, cspill, redo;
 };
 
 // A candidate as one 64-bit word ordered like (distance total order, id).
@@ -277,9 +277,6 @@ __device__ __forceinline__ void gather_rows(const SearchArgs &P, const QLanes<TM
     gather_round<TMAX, EXTRA, EXACT, E>(P, q, rows, ids, cnt, base, out);
 }
 
-// Distances from the query to rows ids[0..cnt) (ids in LDS), written to
-// out[0..cnt) (LDS), with traversal precision `prec` (0 exact f32,
-// 1 int8 codes, 2 bf16 rows).  All 64 lanes must call it.
 // 128-byte code chunks a lane group holds: cstride / 128 for the exact
 // layouts, else the most any dim of that TMAX needs (dim <= 32 * TMAX + 31,
 // and dim <= 1024).
@@ -288,61 +285,106 @@ constexpr int code_chunks() {
   return EXACT ? TMAX / 4 : ((TMAX * 32 + 31 + 127) / 128 < 8 ? (TMAX * 32 + 31 + 127) / 128 : 8);
 }
 
-template <int TMAX, bool EXTRA, bool EXACT>
-__device__ __forceinline__ void gather_dists(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
-                                             const uint2 (&qc)[code_chunks<TMAX, EXACT>()], int prec,
-                                             const uint32_t *ids, int cnt, float *out) {
+// Per-query constants of the integer traversals: the crate's L2 (sum of
+// the query code's squares, so a row costs two exact dot products per word:
+// sum (a-b)^2 = sum a^2 + sum (b^2 - 2ab)) and the per-dimension codes
+// (sigma and qconst of the query's weights).
+struct QCodeInfo {
+  int qaa;
+  float sigma, qconst;
+};
+
+// Code rows a quarter gathers per round: 8 (32 per round) while the codes
+// take at most 2 x 128 bytes per row, 4 beyond.
+template <int CC>
+constexpr int code_rows_per_quarter() { return CC <= 2 ? 8 : 4; }
+
+// Integer-code distances of rows ids[0..cnt): prec 1 the crate's codes
+// (exact sums, src/kernels/int8.rs:13-48, rescaled as
+// src/quantization.rs:85-92), prec 3 the per-dimension codes (exact
+// sum_d qcode_d * code_d, then t = sigma * acc and L2 (qconst + rnorm) -
+// (t + t), inner product -(qconst + t), cosine 1 - (qconst + t)).  A
+// quarter takes several rows per round, every load issued first.
+template <int CC>
+__device__ __forceinline__ void gather_codes(const SearchArgs &P, const uint2 (&qc)[CC],
+                                             const QCodeInfo &qs, int prec, const uint32_t *ids,
+                                             int cnt, float *out, bool exact_layout) {
   const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
-  if (prec == 1) {
-    // int8 codes of the crate's quantizer: exact integer sums
-    // (src/kernels/int8.rs:13-48), rescaled as src/quantization.rs:85-92.
-    // Four rows per quarter per round, loads first.
-    constexpr int CC = code_chunks<TMAX, EXACT>();
-    const int chunks = (int)(P.cstride >> 7);
-    const float s2 = __fmul_rn(P.sq_scale, P.sq_scale);
-    for (int base = 0; base < cnt; base += 16) {
-      const int groups = (cnt - base + 3) >> 2;  // four-row groups needed (wave-uniform)
-      uint2 cb[4][CC];
+  constexpr int B = code_rows_per_quarter<CC>();
+  const bool pd = prec == 3;
+  const int8_t *codes = pd ? P.pd_codes : P.codes;
+  const int chunks = (int)(P.cstride >> 7);
+  const bool l2 = P.metric == 0;
+  const float s2 = __fmul_rn(P.sq_scale, P.sq_scale);
+  for (int base = 0; base < cnt; base += 4 * B) {
+    const int groups = (cnt - base + 3) >> 2;  // four-row groups needed (wave-uniform)
+    uint2 cb[B][CC];
+    float rn[B];
 #pragma unroll
-      for (int b = 0; b < 4; b++) {
-        const int row = base + 4 * b + quarter;
-        gptr<uint2> cr = (gptr<uint2>)(P.codes + (size_t)ids[row < cnt ? row : cnt - 1] * P.cstride);
+    for (int b = 0; b < B; b++) {
+      const int row = base + 4 * b + quarter;
+      const uint32_t id = ids[row < cnt ? row : cnt - 1];
+      gptr<uint2> cr = (gptr<uint2>)(codes + (size_t)id * P.cstride);
+      const bool need = b == 0 || b < groups;
 #pragma unroll
-        for (int c = 0; c < CC; c++)
-          cb[b][c] = (b == 0 || b < groups) && (EXACT || c < chunks) ? (uint2)cr[c * 16 + i]
-                                                                   : make_uint2(0u, 0u);
-      }
+      for (int c = 0; c < CC; c++)
+        cb[b][c] = need && (exact_layout || c < chunks) ? (uint2)cr[c * 16 + i] : make_uint2(0u, 0u);
+      rn[b] = need && pd && l2 ? G(P.pd_rnorm)[id] : 0.0f;
+    }
 #pragma unroll
-      for (int b = 0; b < 4; b++) {
-        if (b > 0 && b >= groups) break;
-        const int row = base + 4 * b + quarter;
-        int acc = 0;
+    for (int b = 0; b < B; b++) {
+      if (b > 0 && b >= groups) break;
+      const int row = base + 4 * b + quarter;
+      int acc = 0;
+      if (!pd && l2) {
+        int ab = 0, bb = 0;
 #pragma unroll
         for (int c = 0; c < CC; c++) {
-          const uint2 bb = cb[b][c];
-          const uint2 a = qc[c];
-          if (P.metric == 0) {
-            acc += __builtin_amdgcn_sdot4((int)a.x, (int)a.x, 0, false) -
-                   2 * __builtin_amdgcn_sdot4((int)a.x, (int)bb.x, 0, false) +
-                   __builtin_amdgcn_sdot4((int)bb.x, (int)bb.x, 0, false);
-            acc += __builtin_amdgcn_sdot4((int)a.y, (int)a.y, 0, false) -
-                   2 * __builtin_amdgcn_sdot4((int)a.y, (int)bb.y, 0, false) +
-                   __builtin_amdgcn_sdot4((int)bb.y, (int)bb.y, 0, false);
-          } else {
-            acc += __builtin_amdgcn_sdot4((int)a.x, (int)bb.x, 0, false);
-            acc += __builtin_amdgcn_sdot4((int)a.y, (int)bb.y, 0, false);
-          }
+          ab = __builtin_amdgcn_sdot4((int)qc[c].x, (int)cb[b][c].x, ab, false);
+          ab = __builtin_amdgcn_sdot4((int)qc[c].y, (int)cb[b][c].y, ab, false);
+          bb = __builtin_amdgcn_sdot4((int)cb[b][c].x, (int)cb[b][c].x, bb, false);
+          bb = __builtin_amdgcn_sdot4((int)cb[b][c].y, (int)cb[b][c].y, bb, false);
         }
-        acc = quarter_sum_i(acc);
-        if (i == 0 && row < cnt) {
-          float v;
-          if (P.metric == 0) v = __fmul_rn(__uint2float_rn((uint32_t)acc), s2);
-          else if (P.metric == 1) v = __fmul_rn(-__int2float_rn(acc), s2);
-          else v = __fsub_rn(1.0f, __fmul_rn(__int2float_rn(acc), s2));
-          out[row] = v;
+        acc = bb - 2 * ab;
+      } else {
+#pragma unroll
+        for (int c = 0; c < CC; c++) {
+          acc = __builtin_amdgcn_sdot4((int)qc[c].x, (int)cb[b][c].x, acc, false);
+          acc = __builtin_amdgcn_sdot4((int)qc[c].y, (int)cb[b][c].y, acc, false);
         }
       }
+      acc = quarter_sum_i(acc);
+      if (i == 0 && row < cnt) {
+        float v;
+        if (pd) {
+          const float t = __fmul_rn(qs.sigma, __int2float_rn(acc));
+          if (l2) v = __fsub_rn(__fadd_rn(qs.qconst, rn[b]), __fadd_rn(t, t));
+          else if (P.metric == 1) v = -__fadd_rn(qs.qconst, t);
+          else v = __fsub_rn(1.0f, __fadd_rn(qs.qconst, t));
+        } else if (l2) {
+          v = __fmul_rn(__uint2float_rn((uint32_t)(acc + qs.qaa)), s2);
+        } else if (P.metric == 1) {
+          v = __fmul_rn(-__int2float_rn(acc), s2);
+        } else {
+          v = __fsub_rn(1.0f, __fmul_rn(__int2float_rn(acc), s2));
+        }
+        out[row] = v;
+      }
     }
+  }
+}
+
+// Distances from the query to rows ids[0..cnt) (ids in LDS), written to
+// out[0..cnt) (LDS), with traversal precision `prec` (0 exact f32, 1 the
+// crate's int8 codes, 2 bf16 rows, 3 per-dimension int8 codes).  All 64
+// lanes must call it.
+template <int TMAX, bool EXTRA, bool EXACT>
+__device__ __forceinline__ void gather_dists(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
+                                             const uint2 (&qc)[code_chunks<TMAX, EXACT>()],
+                                             const QCodeInfo &qs, int prec, const uint32_t *ids,
+                                             int cnt, float *out) {
+  if (prec == 1 || prec == 3) {
+    gather_codes<code_chunks<TMAX, EXACT>()>(P, qc, qs, prec, ids, cnt, out, EXACT);
     return;
   }
   if (prec == 2) gather_rows<TMAX, EXTRA, EXACT, uint16_t>(P, q, P.bf16, ids, cnt, out);
@@ -379,7 +421,7 @@ hnsw_search_kernel(const SearchArgs P) {
   const uint32_t hcap = P.hash_cap;
   const Shape shp{P.T, P.U, P.R, P.stride};
   gptr<uint32_t> adj0 = G(P.adj0);
-  Stats st = {0, 0, 0, 0, 0, 0, 0, 0};
+  Stats st = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   // Candidates beyond the register list go to a per-wave pool in HBM, so
   // the candidate set is as unbounded as the crate's heap up to the pool's
   // capacity.  Invariant: every pooled entry orders after every entry of C,
@@ -400,10 +442,18 @@ hnsw_search_kernel(const SearchArgs P) {
     if (P.phase == 1) {
       qi = static_next;
       static_next += total_waves;
+    } else if (P.phase == 3) {
+      qi = rl(atomicAdd(P.counter + 1, lane == 0 ? 1u : 0u), 0);
     } else {
       qi = rl(atomicAdd(P.counter, lane == 0 ? 1u : 0u), 0);
     }
-    if (qi >= P.nq) break;
+    if (P.phase == 3) {
+      const uint32_t nredo = rl(G(P.redo_count)[0], 0);
+      if (qi >= (nredo < P.nq ? nredo : (uint32_t)P.nq)) break;
+      qi = rl(G(P.redo)[qi], 0);
+    } else if (qi >= P.nq) {
+      break;
+    }
     if (P.phase == 2) qi = rl(G(P.order)[qi], 0);
     uint32_t k = P.k, ef = P.ef, prec = P.precision, filt = 0xFFFFFFFFu;
     if (P.specs) {
@@ -428,7 +478,7 @@ hnsw_search_kernel(const SearchArgs P) {
     const uint32_t ccap = ef <= 64 ? 128 : ef <= 128 ? 256 : ef <= 256 ? 512 : 1024;
     const uint32_t *allow = P.allow;
     if (P.specs) allow = filt == 0xFFFFFFFFu ? nullptr : P.filters + (size_t)filt * P.filter_words;
-    if (P.phase != 1) st.q++;
+    if (P.phase == 0 || P.phase == 2) st.q++;
 
     // query, in the row layout, plus its int8 code words
     QLanes<TMAX, EXTRA> q;
@@ -440,12 +490,26 @@ hnsw_search_kernel(const SearchArgs P) {
     uint2 qc[code_chunks<TMAX, EXACT>()];
 #pragma unroll
     for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++) qc[c] = make_uint2(0, 0);
-    if (prec == 1) {
+    QCodeInfo qs{0, 1.0f, 0.0f};
+    if (prec == 1 || prec == 3) {
       gptr<uint2> cq = (gptr<uint2>)(P.qcode + (size_t)qi * P.cstride);
       const int chunks = (int)(P.cstride >> 7);
 #pragma unroll
       for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++)
         if (c < chunks) qc[c] = cq[c * 16 + i16];
+      if (prec == 1 && P.metric == 0) {
+        int aa = 0;
+#pragma unroll
+        for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++) {
+          aa = __builtin_amdgcn_sdot4((int)qc[c].x, (int)qc[c].x, aa, false);
+          aa = __builtin_amdgcn_sdot4((int)qc[c].y, (int)qc[c].y, aa, false);
+        }
+        qs.qaa = quarter_sum_i(aa);
+      } else if (prec == 3) {
+        const float2 v = G(P.qpd)[qi];
+        qs.sigma = __builtin_bit_cast(float, rl(__float_as_uint(v.x), 0));
+        qs.qconst = __builtin_bit_cast(float, rl(__float_as_uint(v.y), 0));
+      }
     }
 
     int mode = 0;
@@ -455,7 +519,7 @@ hnsw_search_kernel(const SearchArgs P) {
     // ---- entry point and greedy descent (search_graph, search.rs:146-161)
     uint32_t cur;
     float best;
-    if (P.phase == 2) {
+    if (P.phase >= 2) {
       const uint64_t sd = G(P.seed)[qi];
       cur = rl((uint32_t)sd, 0);
       best = __uint_as_float(rl((uint32_t)(sd >> 32), 0));
@@ -468,7 +532,7 @@ hnsw_search_kernel(const SearchArgs P) {
       uint64_t cur_off = P.top >= 1 ? (uint64_t)G(P.upper_off)[cur] : 0;
       st_id[0] = cur;
       wave_sync();
-      gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, 1, st_d);
+      gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, qs, (int)prec, st_id, 1, st_d);
       wave_sync();
       best = rlf(st_d[0], 0);
       st.up++;
@@ -488,7 +552,7 @@ hnsw_search_kernel(const SearchArgs P) {
             const uint64_t noff = lane < c ? (uint64_t)G(P.upper_off)[nid] : 0;
             if (lane < c) st_id[lane] = nid;
             wave_sync();
-            gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, c, st_d);
+            gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, qs, (int)prec, st_id, c, st_d);
             wave_sync();
             const float dv = lane < c ? st_d[lane] : 0.0f;
             // first list position holding the IEEE-smallest distance
@@ -524,6 +588,11 @@ hnsw_search_kernel(const SearchArgs P) {
     }
 
     // ---- layer 0 (search_layer, search.rs:81-142)
+    // counters as they stand before layer 0: restored if this attempt is
+    // dropped for a re-run
+    const Stats st0 = st;
+    const uint32_t ep_id = cur;
+    const float ep_d = best;
     for (uint32_t t = lane; t < hcap; t += 64) table[t] = kInvalid;
     wave_sync();
     // C and W: sorted candidate words in LDS (C from `head`, W from 0).
@@ -610,7 +679,7 @@ hnsw_search_kernel(const SearchArgs P) {
         wave_sync();
         const uint32_t myid = lane < nn ? st_id[lane] : kInvalid;
         const bool adm = lane < nn && admit(P, allow, myid);
-        gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, (int)prec, st_id, nn, st_d);
+        gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, qs, (int)prec, st_id, nn, st_d);
         wave_sync();
         const float myd = lane < nn ? st_d[lane] : 0.0f;
         const uint64_t am = __ballot(adm);
@@ -710,6 +779,18 @@ hnsw_search_kernel(const SearchArgs P) {
       __threadfence();
     }
 
+    if (P.redo && P.phase != 3 && st.ovf != st0.ovf) {
+      // the pool dropped a candidate: list the query for the phase-3 launch
+      // (which starts from the same layer-0 entry) and drop this attempt
+      st = st0;
+      st.redo++;
+      seed_query(P, qi, ep_id, ep_d);
+      const uint32_t at = rl(atomicAdd(P.redo_count, lane == 0 ? 1u : 0u), 0);
+      if (at < P.nq) GM(P.redo)[at] = qi;
+      wave_sync();
+      continue;
+    }
+
     // ---- results: optional exact re-rank, keys, (distance, key) order
     float *res_d = reinterpret_cast<float *>(table);
     uint32_t *res_id = table + 64 * SW;
@@ -723,7 +804,7 @@ hnsw_search_kernel(const SearchArgs P) {
     if (prec != 0) {
       for (int base = 0; base < wsize; base += 64) {
         const int c = wsize - base < 64 ? wsize - base : 64;
-        gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, 0, res_id + base, c, res_d + base);
+        gather_dists<TMAX, EXTRA, EXACT>(P, q, qc, qs, 0, res_id + base, c, res_d + base);
         wave_sync();
       }
       st.rr += wsize;
@@ -767,6 +848,7 @@ hnsw_search_kernel(const SearchArgs P) {
     atomicAdd(&P.stats[5], st.ovf);
     atomicAdd(&P.stats[6], st.spill);
     atomicAdd(&P.stats[7], st.cspill);
+    atomicAdd(&P.stats[8], st.redo);
   }
 }
 
@@ -907,31 +989,46 @@ __device__ __forceinline__ float lane_distance_any(const float *q, gptr<E> row, 
   return h;
 }
 
-// int8 codes: the exact integer sums of the crate's quantized distance
-// (src/kernels/int8.rs:13-48) in any order over the code row (zero padded to
-// cstride on both sides), rescaled as the search kernel does
-// (src/quantization.rs:85-92).
+// Integer codes: the exact integer sums over the whole code row (zero
+// padded to cstride on both sides) in any order, rescaled as the search
+// kernel does: prec 1 the crate's quantized distance
+// (src/kernels/int8.rs:13-48, src/quantization.rs:85-92), prec 3 the
+// per-dimension codes.
 // WORDS: the code row's uint2 count when known at compile time (exact
 // layouts: cstride = D), else 0 and cstride / 8 is read at run time.
 template <int WORDS>
-__device__ __forceinline__ float lane_distance_i8(const SearchArgs &P, const uint2 *qc,
-                                                  gptr<uint2> row) {
+__device__ __forceinline__ float lane_distance_i8(const SearchArgs &P, uint32_t prec,
+                                                  const QCodeInfo &qs, const uint2 *qc,
+                                                  uint32_t id) {
   const int words = WORDS ? WORDS : (int)(P.cstride >> 3);
+  const bool pd = prec == 3;
+  gptr<uint2> row = (gptr<uint2>)((pd ? P.pd_codes : P.codes) + (size_t)id * P.cstride);
+  const float rn = pd && P.metric == 0 ? G(P.pd_rnorm)[id] : 0.0f;
   int acc = 0;
+  if (!pd && P.metric == 0) {
 #pragma unroll 8
-  for (int c = 0; c < words; c++) {
-    const uint2 a = qc[c], b = row[c];
-    if (P.metric == 0) {
+    for (int c = 0; c < words; c++) {
+      const uint2 a = qc[c], b = row[c];
       acc += __builtin_amdgcn_sdot4((int)a.x, (int)a.x, 0, false) -
              2 * __builtin_amdgcn_sdot4((int)a.x, (int)b.x, 0, false) +
              __builtin_amdgcn_sdot4((int)b.x, (int)b.x, 0, false);
       acc += __builtin_amdgcn_sdot4((int)a.y, (int)a.y, 0, false) -
              2 * __builtin_amdgcn_sdot4((int)a.y, (int)b.y, 0, false) +
              __builtin_amdgcn_sdot4((int)b.y, (int)b.y, 0, false);
-    } else {
-      acc += __builtin_amdgcn_sdot4((int)a.x, (int)b.x, 0, false);
-      acc += __builtin_amdgcn_sdot4((int)a.y, (int)b.y, 0, false);
     }
+  } else {
+#pragma unroll 8
+    for (int c = 0; c < words; c++) {
+      const uint2 a = qc[c], b = row[c];
+      acc = __builtin_amdgcn_sdot4((int)a.x, (int)b.x, acc, false);
+      acc = __builtin_amdgcn_sdot4((int)a.y, (int)b.y, acc, false);
+    }
+  }
+  if (pd) {
+    const float t = __fmul_rn(qs.sigma, __int2float_rn(acc));
+    if (P.metric == 0) return __fsub_rn(__fadd_rn(qs.qconst, rn), __fadd_rn(t, t));
+    if (P.metric == 1) return -__fadd_rn(qs.qconst, t);
+    return __fsub_rn(1.0f, __fadd_rn(qs.qconst, t));
   }
   const float s2 = __fmul_rn(P.sq_scale, P.sq_scale);
   if (P.metric == 0) return __fmul_rn(__uint2float_rn((uint32_t)acc), s2);
@@ -939,15 +1036,15 @@ __device__ __forceinline__ float lane_distance_i8(const SearchArgs &P, const uin
   return __fsub_rn(1.0f, __fmul_rn(__int2float_rn(acc), s2));
 }
 
-// The traversal distance of the batch's precision (0 f32, 1 int8 codes,
-// 2 bf16 rows), with the bits the search kernel computes.
+// The traversal distance of the batch's precision (0 f32, 1 the crate's
+// int8 codes, 2 bf16 rows, 3 per-dimension codes), with the bits the search
+// kernel computes.
 template <int TMAX, bool EXACT>
 __device__ __forceinline__ float descent_distance(const SearchArgs &P, uint32_t prec,
                                                   const Shape &shp, const float *q,
-                                                  const uint2 *qc, uint32_t id) {
-  if (prec == 1)
-    return lane_distance_i8<EXACT ? 4 * TMAX : 0>(
-        P, qc, (gptr<uint2>)(P.codes + (size_t)id * P.cstride));
+                                                  const uint2 *qc, const QCodeInfo &qs,
+                                                  uint32_t id) {
+  if (prec == 1 || prec == 3) return lane_distance_i8<EXACT ? 4 * TMAX : 0>(P, prec, qs, qc, id);
   float s;
   if (prec == 2) {
     gptr<uint16_t> row = G(P.bf16) + (size_t)id * P.stride;
@@ -997,16 +1094,22 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
       gptr<float4> src = (gptr<float4>)(P.qprep + qi * P.stride);
       float4 *dst = reinterpret_cast<float4 *>(qs[qslot]);
       for (int c = il; c < (int)(P.stride / 4); c += LPQ) dst[c] = src[c];
-      if (prec == 1) {
+      QCodeInfo qk{0, 1.0f, 0.0f};
+      if (prec == 1 || prec == 3) {
         gptr<uint2> csrc = (gptr<uint2>)(P.qcode + qi * P.cstride);
         for (int c = il; c < (int)(P.cstride / 8); c += LPQ) qcs[qslot][c] = csrc[c];
+        if (prec == 3) {
+          const float2 v = G(P.qpd)[qi];
+          qk.sigma = v.x;
+          qk.qconst = v.y;
+        }
       }
       wave_sync();
       const float *q = qs[qslot];
       const uint2 *qc = qcs[qslot];
       uint32_t cur = P.entry;
       uint64_t cur_off = G(P.upper_off)[cur];
-      float best = descent_distance<TMAX, EXACT>(P, prec, shp, q, qc, cur);
+      float best = descent_distance<TMAX, EXACT>(P, prec, shp, q, qc, qk, cur);
       up++;
       for (int layer = P.top; layer >= 1; layer--) {
         for (;;) {
@@ -1016,7 +1119,7 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
           const bool valid = (uint32_t)il < cnt;
           const uint32_t nid = valid ? blk[1 + il] : kInvalid;
           const uint64_t noff = valid ? (uint64_t)G(P.upper_off)[nid] : 0;
-          float md = valid ? descent_distance<TMAX, EXACT>(P, prec, shp, q, qc, nid) : 0.0f;
+          float md = valid ? descent_distance<TMAX, EXACT>(P, prec, shp, q, qc, qk, nid) : 0.0f;
           int mi = valid ? il : LPQ;
 #pragma unroll
           for (int m = 1; m < LPQ; m <<= 1) {
@@ -1170,7 +1273,7 @@ __device__ __forceinline__ void prepare_one(const PrepArgs &A, uint64_t qi, int 
     const hh_query_spec sp = G(A.specs)[qi];
     prec = sp.precision;
     if (status == 0) {
-      if (sp.precision > 2) status = 103;
+      if (sp.precision > 3) status = 103;
       else if (sp.k > A.k_stride) status = 103;
       else if (sp.filter != 0xFFFFFFFFu && sp.filter >= A.nfilters) status = 103;
       else if ((sp.ef > sp.k ? sp.ef : sp.k) > A.ef_cap) status = 102;
@@ -1178,6 +1281,7 @@ __device__ __forceinline__ void prepare_one(const PrepArgs &A, uint64_t qi, int 
   }
   if (status == 0 && prec == 1 && !A.has_codes) status = 8;
   if (status == 0 && prec == 2 && !A.has_bf16) status = 102;
+  if (status == 0 && prec == 3 && !A.has_pd) status = 8;
   GM(A.status)[qi] = status;
   if (status != 0) return;
   const double inv = A.metric == 2 ? 1.0 / sqrt(norm_sq) : 1.0;
@@ -1199,6 +1303,58 @@ __device__ __forceinline__ void prepare_one(const PrepArgs &A, uint64_t qi, int 
     }
     out[pos] = x;
   }
+  if (code && prec == 3) {
+    // per-dimension codes: weights w_d = (x_d - mid_d) * scale_d (L2) or
+    // x_d * scale_d, sigma = max |w_d| / 127 (1 when zero), code_d =
+    // clamp(round_half_away(w_d * (1 / sigma)), -127, 127); qconst =
+    // sum_d (x_d - mid_d)^2 (L2) or sum_d x_d * mid_d in f64, summed in
+    // dimension order by a uniform chain, rounded to f32 once.
+    float wmax = 0.0f;
+    double qconst = 0.0;
+    for (uint32_t base = 0; base < A.dim; base += 64) {
+      const uint32_t e = base + (uint32_t)lane;
+      float w = 0.0f;
+      double term = 0.0;
+      if (e < A.dim) {
+        float x = v[e];
+        if (A.metric == 2) x = (float)__dmul_rn((double)x, inv);
+        const float md = G(A.pd_mid)[e], sc = G(A.pd_scale)[e];
+        if (A.metric == 0) {
+          w = __fmul_rn(__fsub_rn(x, md), sc);
+          const double t = __dsub_rn((double)x, (double)md);
+          term = __dmul_rn(t, t);
+        } else {
+          w = __fmul_rn(x, sc);
+          term = __dmul_rn((double)x, (double)md);
+        }
+      }
+      wmax = fmaxf(wmax, fabsf(w));
+      const int cnt = A.dim - base < 64 ? (int)(A.dim - base) : 64;
+      const int lo = __double2loint(term), hi = __double2hiint(term);
+      for (int l = 0; l < cnt; l++)
+        qconst = __dadd_rn(qconst, __hiloint2double(__builtin_amdgcn_readlane(hi, l),
+                                                    __builtin_amdgcn_readlane(lo, l)));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o, 64));
+    float sigma = __fdiv_rn(wmax, 127.0f);
+    if (!(sigma > 0.0f)) sigma = 1.0f;
+    const float winv = __fdiv_rn(1.0f, sigma);
+    for (uint32_t e = (uint32_t)lane; e < A.cstride; e += 64) {
+      int8_t cv = 0;
+      if (e < A.dim) {
+        float x = v[e];
+        if (A.metric == 2) x = (float)__dmul_rn((double)x, inv);
+        const float md = G(A.pd_mid)[e], sc = G(A.pd_scale)[e];
+        const float w = A.metric == 0 ? __fmul_rn(__fsub_rn(x, md), sc) : __fmul_rn(x, sc);
+        float c = roundf(__fmul_rn(w, winv));
+        c = fminf(127.0f, fmaxf(-127.0f, c));
+        cv = (int8_t)c;
+      }
+      code[e] = cv;
+    }
+    if (lane == 0) GM(A.qpd)[qi] = make_float2(sigma, (float)qconst);
+  }
   if (encode) {
     for (uint32_t e = (uint32_t)lane; e < A.cstride; e += 64) {
       int8_t cv = 0;
@@ -1217,7 +1373,7 @@ __device__ __forceinline__ void prepare_one(const PrepArgs &A, uint64_t qi, int 
 // grid: any number of 256-thread blocks (four queries per block at a time)
 __global__ void __launch_bounds__(256) prepare_queries_kernel(const PrepArgs A) {
   const int lane = (int)(threadIdx.x & 63);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && A.counter) GM(A.counter)[0] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < 4 && A.counter) GM(A.counter)[threadIdx.x] = 0u;
   for (uint64_t qi = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); qi < A.nq;
        qi += (uint64_t)gridDim.x * 4)
     prepare_one(A, qi, lane);

@@ -26,9 +26,12 @@ struct SearchArgs {
   int T, U, R;
   uint32_t stride, cstride;
   float sq_scale;
+  const int8_t *pd_codes;       // n x cstride per-dimension codes, or null
+  const float *pd_rnorm;        // n row terms of the L2 expansion (L2 only), or null
   // queries (prepared by prepare_queries_kernel)
   const float *qprep;
-  const int8_t *qcode;
+  const int8_t *qcode;          // crate codes (precision 1) or per-dimension weights (3)
+  const float2 *qpd;            // per query: (sigma, qconst) of precision 3
   const int32_t *status;
   uint64_t nq;
   const hh_query_spec *specs;   // null = uniform k/ef/precision/allow
@@ -49,12 +52,19 @@ struct SearchArgs {
   uint32_t cpool_cap;
   uint32_t *spill_log;          // per-wave ids marked in the bitset (spill_log_cap each)
   uint32_t spill_log_cap;
-  unsigned long long *stats;    // 8 counters
+  unsigned long long *stats;    // 9 counters
+  // Queries whose candidate pool dropped an entry are listed here (count in
+  // redo_count) and run again by a phase-3 launch whose per-wave pools hold
+  // cpool_cap >= n entries, so no query returns other results than the
+  // crate's unbounded heap gives.  Null: no list (the pool cannot overflow).
+  uint32_t *redo;
+  uint32_t *redo_count;
   // Two-pass launches (phase 1 then 2) take long queries first: phase 0
   // runs each query whole; phase 1 only descends the upper layers, writing
   // seed[qi] = (best distance bits << 32) | layer-0 entry; phase 2 runs
   // layer 0 from seed[], taking queries in the order of `order` (largest
-  // descent distance first).
+  // descent distance first); phase 3 re-runs layer 0 of the queries in
+  // redo[] from their seeds.
   uint32_t phase;
   uint64_t *seed;
   const uint32_t *order;
@@ -73,10 +83,12 @@ struct PrepArgs {
   int8_t *qcode;
   const float *sq_offset;
   float sq_scale;
+  const float *pd_mid, *pd_scale;  // per-dimension quantizer, or null
+  float2 *qpd;
   uint32_t precision;
   const hh_query_spec *specs;
   uint32_t k_stride, nfilters, ef_cap;
-  int has_codes, has_bf16;
+  int has_codes, has_bf16, has_pd;
   int32_t *status;
   uint32_t *counter;
 };

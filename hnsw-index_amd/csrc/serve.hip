@@ -9,10 +9,17 @@
 // into pinned staging, launches the mixed-spec search and the copy of the
 // results back, and records an event.  Requests arriving while batches
 // run are admitted as soon as a slot frees, so their work overlaps the
-// running batches on the device.  Admission is bounded by the slot budget:
-// max_inflight * max_batch queries in flight, sized from the number of
-// wavefronts the search kernel keeps resident and from the memory each
-// slot's buffers take.
+// running batches on the device.
+//
+// Admission follows the device budget of a launch.  A launch's LDS shape
+// (per-wave lists and visited table) is set by its largest ef, so requests
+// wait in one queue per ef bucket (max(ef, k) <= 64, 128, 256, 512) and a
+// launch takes requests of one bucket only: the bucket of the oldest
+// waiting request, up to the wavefronts the kernel keeps resident at that
+// bucket's shape.  Small-ef requests therefore never run at the occupancy
+// of a large-ef one, and requests are served in arrival order across
+// buckets.  Device memory is fixed at open: each slot's buffers hold the
+// largest batch, and the index's workspaces are reserved for it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -84,19 +91,27 @@ struct Slot {
   bool busy = false;
 };
 
+// Queue of a request: the list-size bucket of max(ef, k).
+int bucket_of(const hh_query_spec &sp) {
+  const uint32_t e = std::max(sp.ef, sp.k);
+  return e <= 64 ? 0 : e <= 128 ? 1 : e <= 256 ? 2 : 3;
+}
+
 }  // namespace
 
 struct hh_server {
   hh_index *idx = nullptr;
   int device = 0;
   uint32_t dim = 0, max_batch = 0, k_max = 0, max_inflight = 0;
-  uint64_t n_words = 0;
+  uint32_t bucket_batch[4] = {0, 0, 0, 0};  // launch size per ef bucket
+  uint64_t n_words = 0;  // words per registered filter bitmap
   std::vector<std::unique_ptr<Slot>> slots;
   uint32_t *d_filters = nullptr;  // registered filters, filter_cap bitmaps
   uint32_t filter_cap = 0, nfilters = 0;
   std::mutex mu;
   std::condition_variable cv, done_cv;
-  std::deque<Request> pending;
+  std::deque<Request> pending[4];  // one queue per ef bucket
+  uint64_t npending = 0;
   std::unordered_map<uint64_t, Result> results;
   uint64_t next_ticket = 1;
   uint64_t served = 0, batches = 0, max_queue = 0;
@@ -106,9 +121,27 @@ struct hh_server {
   std::thread worker;
 
   void run();
-  int launch(Slot &s);
+  int launch(Slot &s, uint32_t nfilt);
   void finish(Slot &s);
+  // Moves the next launch's requests into s.batch: the bucket of the
+  // oldest waiting request, up to that bucket's launch size.
+  void admit(Slot &s);
 };
+
+void hh_server::admit(Slot &s) {
+  int b = -1;
+  for (int i = 0; i < 4; i++)
+    if (!pending[i].empty() && (b < 0 || pending[i].front().ticket < pending[b].front().ticket))
+      b = i;
+  s.batch.clear();
+  if (b < 0) return;
+  auto &q = pending[b];
+  while (!q.empty() && s.batch.size() < bucket_batch[b]) {
+    s.batch.push_back(std::move(q.front()));
+    q.pop_front();
+  }
+  npending -= s.batch.size();
+}
 
 void hh_server::run() {
   (void)hipSetDevice(device);
@@ -127,14 +160,13 @@ void hh_server::run() {
     // admit waiting requests into free slots
     for (auto &sp : slots) {
       Slot &s = *sp;
-      if (s.busy || pending.empty()) continue;
-      s.batch.clear();
-      while (!pending.empty() && s.batch.size() < max_batch) {
-        s.batch.push_back(std::move(pending.front()));
-        pending.pop_front();
-      }
+      if (s.busy || npending == 0) continue;
+      admit(s);
+      // the filter table as registered so far (register_filter appends
+      // under the lock; launches read only entries below this count)
+      const uint32_t nfilt = nfilters;
       lock.unlock();
-      const int st = launch(s);
+      const int st = launch(s, nfilt);
       lock.lock();
       if (st) {
         error = st;
@@ -152,19 +184,19 @@ void hh_server::run() {
     }
     bool any_busy = false;
     for (auto &sp : slots) any_busy |= sp->busy;
-    if (stop && !any_busy && pending.empty()) break;
+    if (stop && !any_busy && npending == 0) break;
     if (any_busy) {
       // poll the in-flight events without holding the lock for long
       lock.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(20));
       lock.lock();
     } else {
-      cv.wait(lock, [&] { return stop || !pending.empty(); });
+      cv.wait(lock, [&] { return stop || npending > 0; });
     }
   }
 }
 
-int hh_server::launch(Slot &s) {
+int hh_server::launch(Slot &s, uint32_t nfilt) {
   const size_t nb = s.batch.size();
   uint32_t ef_max = 1;
   for (size_t i = 0; i < nb; i++) {
@@ -178,8 +210,9 @@ int hh_server::launch(Slot &s) {
   if (hipMemcpyAsync(s.d_q, s.h_q, nb * dim * 4, hipMemcpyHostToDevice, s.stream) != hipSuccess ||
       hipMemcpyAsync(s.d_s, s.h_s, nb * sizeof(hh_query_spec), hipMemcpyHostToDevice, s.stream) != hipSuccess)
     return fail(HH_DEVICE_ERROR, "staging copy failed");
-  int st = hh_search_batch_spec_device(idx, s.d_q, nb, s.d_s, k_max, ef_max, d_filters, nfilters, s.d_k,
-                                       s.d_d, s.d_c, s.d_st, s.stream);
+  int st = hh::search_spec_device_strided(idx, s.d_q, nb, s.d_s, k_max, ef_max,
+                                          nfilt ? d_filters : nullptr, nfilt, n_words, s.d_k,
+                                          s.d_d, s.d_c, s.d_st, s.stream);
   if (st) return st;
   if (hipMemcpyAsync(s.h_k, s.d_k, nb * k_max * 8, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
       hipMemcpyAsync(s.h_d, s.d_d, nb * k_max * 4, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
@@ -224,6 +257,11 @@ int hh_serve_open(hh_index *idx, const hh_serve_config *cfg, hh_server **out) {
   s->max_inflight = std::max<uint32_t>(1, std::min<uint32_t>(cfg->max_inflight ? cfg->max_inflight : 2, 4));
   s->max_batch = cfg->max_batch ? cfg->max_batch : hh::index_resident_waves(idx);
   s->max_batch = std::max<uint32_t>(1, s->max_batch);
+  for (int b = 0; b < 4; b++) {
+    // auto: the resident wavefronts at the bucket's launch shape
+    const uint32_t waves = hh::index_resident_waves(idx, 64u << b);
+    s->bucket_batch[b] = cfg->max_batch ? s->max_batch : std::min(s->max_batch, std::max(1u, waves));
+  }
   s->max_queue = cfg->max_queue ? cfg->max_queue : (uint64_t)s->max_batch * 64;
   s->filter_cap = cfg->max_filters;
   if (hipSetDevice(s->device) != hipSuccess) return fail(HH_DEVICE_ERROR, "hipSetDevice failed");
@@ -259,6 +297,7 @@ int hh_serve_open(hh_index *idx, const hh_serve_config *cfg, hh_server **out) {
     if (st) return st;
   }
   hh_server *raw = s.get();
+  hh::index_serving(idx, +1);
   raw->worker = std::thread([raw] { raw->run(); });
   *out = s.release();
   return HH_OK;
@@ -286,11 +325,12 @@ int hh_serve_submit(hh_server *s, const float *query, const hh_query_spec *spec,
   {
     std::lock_guard<std::mutex> lock(s->mu);
     if (s->stop) return fail(HH_INVALID_ARGUMENT, "server is closing");
-    if (s->pending.size() >= s->max_queue)
+    if (s->npending >= s->max_queue)
       return fail(HH_CAPACITY_EXCEEDED, "request queue is full");
     r.ticket = s->next_ticket++;
     *ticket = r.ticket;
-    s->pending.push_back(std::move(r));
+    s->pending[bucket_of(r.spec)].push_back(std::move(r));
+    s->npending++;
   }
   s->cv.notify_one();
   return HH_OK;
@@ -302,7 +342,7 @@ int hh_serve_poll(hh_server *s, uint64_t ticket, int wait, uint64_t *keys, float
   std::unique_lock<std::mutex> lock(s->mu);
   auto it = s->results.find(ticket);
   if (it == s->results.end() && wait) {
-    s->done_cv.wait(lock, [&] { return s->results.count(ticket) > 0 || (s->stop && s->pending.empty()); });
+    s->done_cv.wait(lock, [&] { return s->results.count(ticket) > 0 || (s->stop && s->npending == 0); });
     it = s->results.find(ticket);
   }
   if (it == s->results.end()) {
@@ -325,7 +365,7 @@ int hh_serve_stats(hh_server *s, uint64_t *served, uint64_t *batches, uint64_t *
   std::lock_guard<std::mutex> lock(s->mu);
   if (served) *served = s->served;
   if (batches) *batches = s->batches;
-  if (queued) *queued = s->pending.size();
+  if (queued) *queued = s->npending;
   return s->error;
 }
 
@@ -412,6 +452,7 @@ int hh_serve_close(hh_server *s) {
     (void)hipStreamDestroy(sl->stream);
   }
   if (s->d_filters) (void)hipFree(s->d_filters);
+  hh::index_serving(s->idx, -1);
   const int err = s->error;
   delete s;
   return err;

@@ -105,9 +105,13 @@ class Backend(enum.Enum):
 
 
 class Precision(enum.IntEnum):
+    """Traversal precision (hh_precision).  INT8 is the crate's quantized
+    traversal; BF16 and INT8_PERDIM (per-dimension min/max codes) have no
+    crate counterpart.  Every mode but F32 re-ranks exactly."""
     F32 = 0
     INT8 = 1
     BF16 = 2
+    INT8_PERDIM = 3
 
 
 @dataclass(frozen=True)
@@ -237,6 +241,8 @@ class HnswIndex:
         self._dirty = True
         self._stale = False
         self._bf16 = False
+        self._perdim = False
+        self._servers = 0
 
     def __del__(self):
         try:
@@ -296,6 +302,8 @@ class HnswIndex:
         self._dirty = True
         self._stale = False
         self._bf16 = False
+        self._perdim = False
+        self._servers = 0
         return self
 
     @classmethod
@@ -328,6 +336,13 @@ class HnswIndex:
         return set(v["keys"][live].tolist())
 
     # -- mutation (CPU, src/hnsw/mod.rs:290-400)
+    def _refuse_if_served(self):
+        """A Server launches searches on the resident index: while one is
+        open the index (host graph and device copy) must not change."""
+        if self._servers > 0:
+            raise InvalidArgument("a Server is open on this index: close it before changing "
+                                  "the index")
+
     def insert(self, key, vector):
         self.insert_batch([key], np.asarray(vector, np.float32).reshape(1, -1), threads=1)
 
@@ -336,6 +351,7 @@ class HnswIndex:
         the layer-0 searches of INSERT run on the GPU (see _link_on_device);
         the graph is of the same kind as a parallel host build, not equal to
         it."""
+        self._refuse_if_served()
         keys = np.ascontiguousarray(keys, np.uint64)
         vectors = _f32(vectors)
         if vectors.size != keys.size * self._dim:
@@ -394,6 +410,7 @@ class HnswIndex:
             done += take
 
     def remove(self, key):
+        self._refuse_if_served()
         r = ctypes.c_int(0)
         _check(lib.hh_graph_remove(self._g, key, ctypes.byref(r)))
         if r.value and self._idx and not self._dirty and not self._stale:
@@ -407,6 +424,7 @@ class HnswIndex:
         over the resident rows (bit-identical parameters and codes), and the
         host graph receives both, so later inserts encode with it as the
         crate does."""
+        self._refuse_if_served()
         if not on_device:
             _check(lib.hh_graph_enable_quantization(self._g))
             self._dirty = True
@@ -430,8 +448,44 @@ class HnswIndex:
         return v["sq_offset"].copy(), float(v["sq_scale"])
 
     def disable_quantization(self):
+        self._refuse_if_served()
         _check(lib.hh_graph_disable_quantization(self._g))
         self._dirty = True
+
+    def enable_quantization_perdim(self):
+        """Trains the per-dimension min/max int8 quantizer on the resident
+        rows and encodes them on the GPU (hh_index_quantize_perdim; no crate
+        counterpart).  Searches with Precision.INT8_PERDIM then traverse on
+        those codes and re-rank exactly.  Inserts synced later are encoded
+        with the same parameters; a full re-upload of the index trains it
+        again.  Returns (mid, scale), dim values each."""
+        self._refuse_if_served()
+        idx = self.device_index()
+        mid = np.zeros(self._dim, np.float32)
+        scale = np.zeros(self._dim, np.float32)
+        _check(lib.hh_index_quantize_perdim(idx, _ptr(mid), _ptr(scale)))
+        self._perdim = True
+        return mid, scale
+
+    def perdim_codes(self):
+        """(codes n x dim, row terms n or None) of the resident per-dimension
+        quantizer."""
+        idx = self.device_index()
+        n = self.node_count()
+        codes = np.zeros((n, self._dim), np.int8)
+        rn = np.zeros(n, np.float32) if self._metric == Metric.L2 else None
+        if n:
+            _check(lib.hh_index_download_perdim(idx, _ptr(codes), _ptr(rn)))
+        return codes, rn
+
+    def bf16_rows(self):
+        """The resident bf16 rows (n x dim uint16 bit patterns)."""
+        if not self._bf16:
+            self.enable_bf16()
+        out = np.zeros((self.node_count(), self._dim), np.uint16)
+        if out.size:
+            _check(lib.hh_index_download_bf16(self.device_index(), _ptr(out)))
+        return out
 
     def is_quantized(self):
         return bool(self._view().codes)
@@ -468,6 +522,8 @@ class HnswIndex:
     # -- device residency
     def device_index(self):
         """The uploaded hh_index handle (uploading first if needed)."""
+        if (self._dirty or self._stale or self._idx is None) and self._servers > 0:
+            self._refuse_if_served()
         if self._dirty or self._idx is None:
             if self._idx:
                 lib.hh_index_free(self._idx)
@@ -480,6 +536,8 @@ class HnswIndex:
             self._bf16 = False
             self._take_dirty()  # the upload carried every link
             self._stale = False
+            if self._perdim:
+                _check(lib.hh_index_quantize_perdim(h, None, None))
         elif self._stale:
             # inserts and removals since the upload: move only the changes
             ids = self._take_dirty()
@@ -507,6 +565,8 @@ class HnswIndex:
         _check(lib.hh_index_set_query_order(self.device_index(), int(bool(longest_first))))
 
     def enable_bf16(self):
+        if not self._bf16:
+            self._refuse_if_served()
         _check(lib.hh_index_enable_bf16(self.device_index()))
         self._bf16 = True
 
@@ -623,9 +683,12 @@ class Server:
         self._dim = index.dim()
         cfg = L.ServeConfig(max_batch, max_inflight, k_max, max_filters, max_queue)
         h = ctypes.c_void_p()
+        self._h = None
         _check(lib.hh_serve_open(index.device_index(), ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
         self._k_max = k_max
+        # the index refuses changes (and keeps its device copy) while open
+        index._servers += 1
 
     def register_filter(self, mask):
         """mask: bool array over node ids. Returns the filter id."""
@@ -688,6 +751,7 @@ class Server:
     def close(self):
         if self._h:
             h, self._h = self._h, None
+            self._index._servers -= 1
             _check(lib.hh_serve_close(h))
 
     def __enter__(self):

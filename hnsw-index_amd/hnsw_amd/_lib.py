@@ -44,7 +44,7 @@ class FormatInfo(ctypes.Structure):
 class SearchStats(ctypes.Structure):
     _fields_ = [("queries", U64), ("n_dist", U64), ("n_upper", U64), ("n_expand", U64),
                 ("n_rerank", U64), ("n_overflow", U64), ("n_visited_spill", U64),
-                ("n_cand_spill", U64)]
+                ("n_cand_spill", U64), ("n_redo", U64)]
 
 
 class QuerySpec(ctypes.Structure):
@@ -88,6 +88,9 @@ _sig("hh_graph_append", INT, P, P, P, U64, ctypes.POINTER(U64), ctypes.POINTER(U
 _sig("hh_graph_link_batch", INT, P, U64, U64, P, P, P, U32, INT)
 _sig("hh_index_build_candidates", INT, P, U64, U64, U32, P, P, P)
 _sig("hh_index_download_codes", INT, P, P)
+_sig("hh_index_quantize_perdim", INT, P, P, P)
+_sig("hh_index_download_perdim", INT, P, P, P)
+_sig("hh_index_download_bf16", INT, P, P)
 _sig("hh_index_stats", INT, P, ctypes.POINTER(SearchStats))
 _sig("hh_index_reset_stats", INT, P)
 _sig("hh_index_set_timing", INT, P, INT)

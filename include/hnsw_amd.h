@@ -56,6 +56,8 @@ typedef enum hh_precision {
   HH_PREC_F32 = 0,          /* exact f32 traversal (SearchOptions::new) */
   HH_PREC_INT8 = 1,         /* crate quantizer codes (SearchOptions::quantized) */
   HH_PREC_BF16 = 2,         /* bf16 rows, f32 accumulation (no crate counterpart) */
+  HH_PREC_INT8_PERDIM = 3,  /* per-dimension min/max int8 codes (no crate counterpart;
+                               hh_index_quantize_perdim) */
 } hh_precision;
 
 /* Last error message of the calling thread (never NULL). */
@@ -250,7 +252,13 @@ int hh_index_quantize(hh_index *idx, float *offset, float *scale);
  * arrays grow by half their size when full.  desc must describe the same
  * graph (dim, metric, m, m0 and the resident quantizer) with at least the
  * resident number of nodes, else HH_INVALID_ARGUMENT: upload it again.
- * Searches must not run on the index meanwhile; plans must be re-created. */
+ * Every change of a resident index (this call, hh_index_update_deleted,
+ * hh_index_enable_bf16, hh_index_quantize, hh_index_quantize_perdim)
+ * waits for the searches in flight and excludes new ones while it runs;
+ * it is refused (HH_INVALID_ARGUMENT) while a serving loop is open on the
+ * index, and plans created before it fail at launch with
+ * HH_INVALID_ARGUMENT (create them again).  New rows take the resident
+ * per-dimension codes too. */
 int hh_index_sync(hh_index *idx, const hh_graph_desc *desc, const uint32_t *dirty,
                   uint64_t ndirty);
 /* Layer-0 construction candidates of the resident rows [first, first +
@@ -266,6 +274,30 @@ int hh_index_build_candidates(hh_index *idx, uint64_t first, uint64_t count, uin
 /* Copies the resident codes to host memory as n*dim bytes (the crate's
  * code layout), for hh_graph_set_quantizer.  HH_UNSUPPORTED without codes. */
 int hh_index_download_codes(hh_index *idx, int8_t *codes);
+
+/* Per-dimension min/max int8 quantizer (no crate counterpart: the crate's
+ * ScalarQuantizer, src/quantization.rs:32-56, has one global scale; ADR
+ * 0004 chose that for the CPU).  Trains on the resident rows and encodes
+ * them on the device; inserts synced later are encoded with it.  Per
+ * dimension d: lo/hi the minimum/maximum over the rows, scale_d =
+ * (hi - lo) / 254 (1 when not positive), mid_d = (lo + hi) * 0.5 (+0 for a
+ * zero sum), code = clamp(round_half_away((x - mid_d) / scale_d), +-127).
+ * A query's weights w_d = (q_d - mid_d) * scale_d (L2) or q_d * scale_d
+ * are coded the same way with one scale sigma = max|w_d| / 127, so the
+ * traversal distance is one exact integer dot product acc per row: L2
+ * (qconst + rnorm_row) - 2 * sigma * acc, inner product -(qconst +
+ * sigma * acc), cosine 1 - (qconst + sigma * acc), where qconst =
+ * sum (q_d - mid_d)^2 or sum q_d * mid_d and rnorm_row = sum (scale_d *
+ * code_d)^2.  Searches with HH_PREC_INT8_PERDIM traverse on these codes and
+ * re-rank the final candidates exactly, like HH_PREC_INT8.
+ * mid/scale (optional) receive the dim parameters. */
+int hh_index_quantize_perdim(hh_index *idx, float *mid, float *scale);
+/* The resident per-dimension codes as n*dim bytes and (L2) the n row terms
+ * rnorm; either pointer may be NULL. */
+int hh_index_download_perdim(hh_index *idx, int8_t *codes, float *rnorm);
+/* The resident bf16 rows (hh_index_enable_bf16) as n*dim values in
+ * dimension order. */
+int hh_index_download_bf16(hh_index *idx, uint16_t *out);
 /* Query order of large batches (no counterpart in the crate; results never
  * depend on it).  longest_first != 0 (the default): a batch of more than
  * one and at most 16 queries per resident wavefront runs in two passes,
@@ -292,6 +324,9 @@ typedef struct hh_search_stats {
   uint64_t n_overflow;      /* candidates dropped: the per-query candidate pool was full */
   uint64_t n_visited_spill; /* queries whose visited set left LDS */
   uint64_t n_cand_spill;    /* candidates moved from registers to the HBM pool */
+  uint64_t n_redo;          /* queries run again with an n-entry candidate pool because their
+                               first pool (65,536 entries) dropped a candidate; their results
+                               are those of the unbounded heap (src/hnsw/search.rs:125-135) */
 } hh_search_stats;
 int hh_index_stats(hh_index *idx, hh_search_stats *out);
 int hh_index_reset_stats(hh_index *idx);
@@ -377,8 +412,10 @@ int hh_search_batch_spec(hh_index *idx, const float *queries, uint64_t nq,
  * hh_search_batch_spec_device; then k, precision and allow_bits are ignored
  * and ef is the batch's ef_max).  The plan owns its workspace; its launches are ordered with an
  * event, so launches of one plan never overlap.  A plan captures the index
- * as it is at creation: re-create it after hh_index_update_deleted,
- * hh_index_enable_bf16 or hh_index_quantize. */
+ * as it is at creation: after any change of the index (hh_index_sync,
+ * hh_index_update_deleted, hh_index_enable_bf16, hh_index_quantize,
+ * hh_index_quantize_perdim) hh_search_plan_launch returns
+ * HH_INVALID_ARGUMENT and launches nothing; create the plan again. */
 typedef struct hh_search_plan hh_search_plan;
 int hh_search_plan_create(hh_index *idx, const float *queries, uint64_t nq, uint32_t k,
                           uint32_t ef, uint32_t precision, const uint32_t *allow_bits,

@@ -319,6 +319,114 @@ float or_sq_distance(int metric, float scale, const int8_t *a, const int8_t *b, 
   }
 }
 
+/* ---------------------------------------------------- bf16 traversal */
+
+/* f32 -> bf16, round to nearest with ties to even, widened back to f32 (the
+ * bf16 rows the device engine traverses; no crate counterpart). */
+void or_bf16_round(const float *in, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) {
+    uint32_t u;
+    memcpy(&u, &in[i], 4);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) {
+      u |= 0x00400000u; /* NaN stays a (quiet) NaN */
+    } else {
+      u += 0x7FFFu + ((u >> 16) & 1u);
+    }
+    u &= 0xFFFF0000u;
+    memcpy(&out[i], &u, 4);
+  }
+}
+
+/* ------------------------------------- per-dimension min/max int8 codes */
+
+/* The device engine's per-dimension quantizer (no crate counterpart; the
+ * crate's own is the global-scale one above).  Per dimension d over every
+ * row: lo_d / hi_d the f32 minimum / maximum, scale_d = (hi_d - lo_d) / 254
+ * (1 when that is not positive), mid_d = (lo_d + hi_d) * 0.5 + 0 (the + 0
+ * turns a zero sum into +0, so the sign of a zero minimum or maximum never
+ * matters), all in f32. */
+void or_pd_train(size_t dim, size_t rows, const float *vectors, float *mid, float *scale) {
+  for (size_t d = 0; d < dim; d++) {
+    float lo = rows ? vectors[d] : 0.0f, hi = lo;
+    for (size_t r = 1; r < rows; r++) {
+      float x = vectors[r * dim + d];
+      if (x < lo) lo = x;
+      if (x > hi) hi = x;
+    }
+    float s = (hi - lo) / 254.0f;
+    scale[d] = s > 0.0f ? s : 1.0f;
+    mid[d] = (lo + hi) * 0.5f + 0.0f;
+  }
+}
+
+/* code_d = clamp(round_half_away((x_d - mid_d) * (1 / scale_d)), -127, 127). */
+void or_pd_encode(const float *mid, const float *scale, size_t dim, const float *v, int8_t *out) {
+  for (size_t d = 0; d < dim; d++) {
+    float inv = 1.0f / scale[d];
+    float x = roundf((v[d] - mid[d]) * inv);
+    if (x < -127.0f) x = -127.0f;
+    if (x > 127.0f) x = 127.0f;
+    out[d] = (int8_t)x;
+  }
+}
+
+/* sum_d (scale_d * code_d)^2, products and sum in f64 in dimension order,
+ * rounded to f32 once: the row's term of the L2 expansion. */
+float or_pd_rnorm(const float *scale, size_t dim, const int8_t *codes) {
+  double s = 0.0;
+  for (size_t d = 0; d < dim; d++) {
+    double y = (double)scale[d] * (double)codes[d];
+    s += y * y;
+  }
+  return (float)s;
+}
+
+/* Query side.  Weights w_d = (q_d - mid_d) * scale_d (L2) or q_d * scale_d
+ * (inner product, cosine), in f32; sigma = max|w_d| / 127 (1 when zero);
+ * qcode_d = clamp(round_half_away(w_d * (1 / sigma)), -127, 127); qconst =
+ * sum_d (q_d - mid_d)^2 (L2) or sum_d q_d * mid_d (else), in f64 in order,
+ * rounded to f32 once. */
+void or_pd_query(int metric, size_t dim, const float *mid, const float *scale, const float *q,
+                 int8_t *code, float *sigma, float *qconst) {
+  float wmax = 0.0f;
+  double c = 0.0;
+  for (size_t d = 0; d < dim; d++) {
+    float w = metric == OR_L2 ? (q[d] - mid[d]) * scale[d] : q[d] * scale[d];
+    float a = fabsf(w);
+    if (a > wmax) wmax = a;
+    if (metric == OR_L2) {
+      double e = (double)q[d] - (double)mid[d];
+      c += e * e;
+    } else {
+      c += (double)q[d] * (double)mid[d];
+    }
+  }
+  float sg = wmax / 127.0f;
+  if (!(sg > 0.0f)) sg = 1.0f;
+  float inv = 1.0f / sg;
+  for (size_t d = 0; d < dim; d++) {
+    float w = metric == OR_L2 ? (q[d] - mid[d]) * scale[d] : q[d] * scale[d];
+    float x = roundf(w * inv);
+    if (x < -127.0f) x = -127.0f;
+    if (x > 127.0f) x = 127.0f;
+    code[d] = (int8_t)x;
+  }
+  *sigma = sg;
+  *qconst = (float)c;
+}
+
+/* acc = sum_d qcode_d * code_d exactly; t = sigma * (f32)acc; L2:
+ * (qconst + rnorm) - (t + t); inner product: -(qconst + t); cosine:
+ * 1 - (qconst + t); every step one f32 rounding. */
+float or_pd_distance(int metric, float sigma, float qconst, float rnorm, const int8_t *qcode,
+                     const int8_t *code, size_t dim) {
+  int64_t acc = or_i8_dot(qcode, code, dim);
+  float t = sigma * (float)acc;
+  if (metric == OR_L2) return (qconst + rnorm) - (t + t);
+  if (metric == OR_IP) return -(qconst + t);
+  return 1.0f - (qconst + t);
+}
+
 /* ---------------------------------------------------------- total order */
 
 /* f32::total_cmp */
@@ -607,7 +715,8 @@ typedef struct {
   const or_graph *g;
   const float *q;      /* prepared query */
   const int8_t *code;  /* encoded query, quantized traversal */
-  int quantized;
+  int quantized;       /* traversal precision, OR_PREC_* */
+  float pd_sigma, pd_qconst;
   const uint8_t *allow;
   uint32_t *marks; /* VisitedSet (src/hnsw/scratch.rs:13-47) with u32 epochs */
   uint32_t epoch;
@@ -622,9 +731,20 @@ static float exact_dist(search_ctx *s, uint32_t id) {
 static float trav_dist(search_ctx *s, uint32_t id) {
   const or_graph *g = s->g;
   s->cnt.n_dist++;
-  if (!s->quantized) return exact_dist(s, id);
-  return or_sq_distance((int)g->metric, g->sq_scale, s->code, g->codes + (size_t)id * g->dim,
-                        g->dim);
+  switch (s->quantized) {
+    case OR_PREC_INT8:
+      return or_sq_distance((int)g->metric, g->sq_scale, s->code, g->codes + (size_t)id * g->dim,
+                            g->dim);
+    case OR_PREC_BF16:
+      return or_distance((int)g->metric, OR_AVX2, s->q, g->bf16_rows + (size_t)id * g->dim,
+                         g->dim);
+    case OR_PREC_INT8_PERDIM:
+      return or_pd_distance((int)g->metric, s->pd_sigma, s->pd_qconst,
+                            g->metric == OR_L2 ? g->pd_rnorm[id] : 0.0f, s->code,
+                            g->pd_codes + (size_t)id * g->dim, g->dim);
+    default:
+      return exact_dist(s, id);
+  }
 }
 
 /* Links::for_each (src/hnsw/graph.rs:88-94) via block (:73-84) */
@@ -707,6 +827,7 @@ static size_t search_layer0(search_ctx *s, cand ep, size_t ef, cand **out) {
       if (improves) {
         cand c = {d, n};
         cheap_push(&cands, c);
+        if (cands.len > s->cnt.max_cands) s->cnt.max_cands = cands.len;
         if (admit(s, n)) {
           cheap_push(&res, c);
           if (res.len > ef) cheap_pop(&res);
@@ -737,16 +858,26 @@ static long search_one(search_ctx *s, const float *query, size_t qlen, size_t k,
   int8_t *code = NULL;
   int st = or_prepare((int)g->metric, g->dim, query, qlen, q);
   if (st) { free(q); return -st; }
-  if (quantized && !g->codes) { free(q); return -OR_QUANTIZATION_DISABLED; }
+  if (quantized == OR_PREC_INT8 && !g->codes) { free(q); return -OR_QUANTIZATION_DISABLED; }
+  if ((quantized == OR_PREC_BF16 && !g->bf16_rows) ||
+      (quantized == OR_PREC_INT8_PERDIM && !g->pd_codes)) {
+    free(q);
+    return -OR_QUANTIZATION_DISABLED;
+  }
   size_t live = 0;
   for (uint64_t i = 0; i < g->n; i++) live += !(g->deleted && g->deleted[i]);
   if (k == 0 || live == 0) { free(q); return 0; }
   if (ef < k) ef = k;
   s->q = q;
   s->quantized = quantized;
-  if (quantized) {
-    code = (int8_t *)malloc(g->dim);
+  if (quantized == OR_PREC_INT8) {
+    code = (int8_t *)malloc(g->dim ? g->dim : 1);
     or_sq_encode(g->sq_offset, g->sq_scale, g->dim, q, code);
+    s->code = code;
+  } else if (quantized == OR_PREC_INT8_PERDIM) {
+    code = (int8_t *)malloc(g->dim ? g->dim : 1);
+    or_pd_query((int)g->metric, g->dim, g->pd_mid, g->pd_scale, q, code, &s->pd_sigma,
+                &s->pd_qconst);
     s->code = code;
   }
   /* search_graph (src/hnsw/search.rs:146-161) */
@@ -809,7 +940,7 @@ typedef struct {
   uint32_t *out_count;
   atomic_size_t first_bad;
   atomic_int status;
-  atomic_ullong n_dist, n_expand, n_rerank, n_upper;
+  atomic_ullong n_dist, n_expand, n_rerank, n_upper, max_cands;
   uint32_t **marks;  /* one VisitedSet per worker */
   uint32_t *epochs;
 } hbatch_ctx;
@@ -829,6 +960,9 @@ static void hbatch_one(void *vctx, size_t i, int worker) {
   atomic_fetch_add(&c->n_expand, s.cnt.n_expand);
   atomic_fetch_add(&c->n_rerank, s.cnt.n_rerank);
   atomic_fetch_add(&c->n_upper, s.cnt.n_upper);
+  unsigned long long mc = atomic_load(&c->max_cands);
+  while (s.cnt.max_cands > mc && !atomic_compare_exchange_weak(&c->max_cands, &mc, s.cnt.max_cands)) {
+  }
   if (r < 0) {
     c->out_count[i] = 0;
     size_t cur = atomic_load(&c->first_bad);
@@ -850,7 +984,7 @@ long or_hnsw_search_batch(const or_graph *g, const float *queries, size_t nq, si
   atomic_init(&c.first_bad, (size_t)-1);
   atomic_init(&c.status, 0);
   atomic_init(&c.n_dist, 0); atomic_init(&c.n_expand, 0);
-  atomic_init(&c.n_rerank, 0); atomic_init(&c.n_upper, 0);
+  atomic_init(&c.n_rerank, 0); atomic_init(&c.n_upper, 0); atomic_init(&c.max_cands, 0);
   if (threads < 1) threads = 1;
   c.marks = (uint32_t **)malloc(sizeof(uint32_t *) * (size_t)threads);
   c.epochs = (uint32_t *)calloc((size_t)threads, sizeof(uint32_t));
@@ -864,6 +998,7 @@ long or_hnsw_search_batch(const or_graph *g, const float *queries, size_t nq, si
     counters->n_expand = atomic_load(&c.n_expand);
     counters->n_rerank = atomic_load(&c.n_rerank);
     counters->n_upper = atomic_load(&c.n_upper);
+    counters->max_cands = atomic_load(&c.max_cands);
   }
   if (first_bad) *first_bad = atomic_load(&c.first_bad);
   int st = atomic_load(&c.status);

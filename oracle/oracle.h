@@ -96,24 +96,49 @@ typedef struct {
   const int8_t *codes;
   const float *sq_offset;
   float sq_scale;
+  /* bf16 traversal: the rows rounded to bf16 (held as f32), or NULL */
+  const float *bf16_rows;
+  /* per-dimension int8 traversal: codes n*dim, mid/scale dim each, row
+   * code norms n (L2), or NULL */
+  const int8_t *pd_codes;
+  const float *pd_mid, *pd_scale, *pd_rnorm;
 } or_graph;
+
+/* Traversal precisions (same numbering as hh_precision). */
+enum { OR_PREC_F32 = 0, OR_PREC_INT8 = 1, OR_PREC_BF16 = 2, OR_PREC_INT8_PERDIM = 3 };
+
+/* bf16 rounding (round to nearest, ties to even), result widened to f32. */
+void or_bf16_round(const float *in, size_t n, float *out);
+
+/* Per-dimension min/max int8 quantizer (no crate counterpart; the device
+ * engine's own design, restated here value for value). */
+void or_pd_train(size_t dim, size_t rows, const float *vectors, float *mid, float *scale);
+void or_pd_encode(const float *mid, const float *scale, size_t dim, const float *v, int8_t *out);
+float or_pd_rnorm(const float *scale, size_t dim, const int8_t *codes);
+void or_pd_query(int metric, size_t dim, const float *mid, const float *scale, const float *q,
+                 int8_t *code, float *sigma, float *qconst);
+float or_pd_distance(int metric, float sigma, float qconst, float rnorm, const int8_t *qcode,
+                     const int8_t *code, size_t dim);
 
 typedef struct {
   uint64_t n_dist;    /* distance evaluations (traversal metric) */
   uint64_t n_expand;  /* layer-0 expansions */
   uint64_t n_rerank;  /* exact re-scores after int8 traversal */
   uint64_t n_upper;   /* distance evaluations in the upper-layer descent */
+  uint64_t max_cands; /* largest candidate heap of any query (live candidates) */
 } or_counters;
 
 /* search_impl (src/hnsw/mod.rs:460-511).  allow: optional byte per node id
- * (the host-evaluated filter).  Returns count (<= k) or -status. */
+ * (the host-evaluated filter).  precision: OR_PREC_* (1 is the crate's
+ * SearchOptions::quantized; 2 and 3 traverse bf16 rows / per-dimension
+ * codes and re-rank exactly like it).  Returns count (<= k) or -status. */
 long or_hnsw_search(const or_graph *g, const float *query, size_t qlen, size_t k, size_t ef,
-                    int quantized, const uint8_t *allow, uint64_t *out_keys, float *out_dist,
+                    int precision, const uint8_t *allow, uint64_t *out_keys, float *out_dist,
                     uint32_t *out_ids, or_counters *counters);
 /* search_batch over threads (src/hnsw/mod.rs:520-536); out arrays nq*k.
  * Returns 0 or -status of the first failing query (index in *first_bad). */
 long or_hnsw_search_batch(const or_graph *g, const float *queries, size_t nq, size_t k,
-                          size_t ef, int quantized, const uint8_t *allow, int threads,
+                          size_t ef, int precision, const uint8_t *allow, int threads,
                           uint64_t *out_keys, float *out_dist, uint32_t *out_count,
                           or_counters *counters, size_t *first_bad);
 

@@ -46,11 +46,16 @@ class Graph(ctypes.Structure):
         ("vectors", P), ("keys", P), ("levels", P), ("layer0", P),
         ("upper_off", P), ("upper", P), ("deleted", P),
         ("codes", P), ("sq_offset", P), ("sq_scale", ctypes.c_float),
+        ("bf16_rows", P), ("pd_codes", P), ("pd_mid", P), ("pd_scale", P), ("pd_rnorm", P),
     ]
 
 
+PREC_F32, PREC_INT8, PREC_BF16, PREC_INT8_PERDIM = 0, 1, 2, 3
+
+
 class Counters(ctypes.Structure):
-    _fields_ = [("n_dist", U64), ("n_expand", U64), ("n_rerank", U64), ("n_upper", U64)]
+    _fields_ = [("n_dist", U64), ("n_expand", U64), ("n_rerank", U64), ("n_upper", U64),
+                ("max_cands", U64)]
 
 
 def _declare(L):
@@ -92,6 +97,15 @@ def _declare(L):
     L.or_hnsw_search_batch.argtypes = [ctypes.POINTER(Graph), P, SZ, SZ, SZ, ctypes.c_int, P,
                                        ctypes.c_int, P, P, P, ctypes.POINTER(Counters),
                                        ctypes.POINTER(SZ)]
+    L.or_bf16_round.argtypes = [P, SZ, P]
+    L.or_pd_train.argtypes = [SZ, SZ, P, P, P]
+    L.or_pd_encode.argtypes = [P, P, SZ, P, P]
+    L.or_pd_rnorm.restype = ctypes.c_float
+    L.or_pd_rnorm.argtypes = [P, SZ, P]
+    L.or_pd_query.argtypes = [ctypes.c_int, SZ, P, P, P, P, P, P]
+    L.or_pd_distance.restype = ctypes.c_float
+    L.or_pd_distance.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 P, P, SZ]
     L.or_recall_at_k.restype = ctypes.c_double
     L.or_recall_at_k.argtypes = [SZ, P, P, SZ, P, P, SZ, SZ]
     L.or_candidate_cmp.restype = ctypes.c_int
@@ -201,6 +215,56 @@ def sq_distance(metric, scale, a, b):
     return np.float32(lib().or_sq_distance(metric, scale, _p(a), _p(b), a.size))
 
 
+def bf16_round(a):
+    """f32 -> bf16 (round to nearest even), widened back to f32."""
+    a = f32(a)
+    out = np.empty_like(a)
+    lib().or_bf16_round(_p(a), a.size, _p(out))
+    return out
+
+
+# ------------------------------------------- per-dimension int8 quantizer
+def pd_train(vectors):
+    vectors = f32(vectors)
+    rows, dim = vectors.shape
+    mid = np.empty(dim, np.float32)
+    scale = np.empty(dim, np.float32)
+    lib().or_pd_train(dim, rows, _p(vectors), _p(mid), _p(scale))
+    return mid, scale
+
+
+def pd_encode_rows(mid, scale, rows):
+    rows = f32(rows)
+    mid, scale = f32(mid), f32(scale)
+    out = np.empty(rows.shape, np.int8)
+    for i in range(rows.shape[0]):
+        lib().or_pd_encode(_p(mid), _p(scale), rows.shape[1], _p(rows[i]), _p(out[i]))
+    return out
+
+
+def pd_rnorm_rows(scale, codes):
+    scale = f32(scale)
+    codes = np.ascontiguousarray(codes, np.int8)
+    return np.array([lib().or_pd_rnorm(_p(scale), codes.shape[1], _p(codes[i]))
+                     for i in range(codes.shape[0])], np.float32)
+
+
+def pd_query(metric, mid, scale, q):
+    mid, scale, q = f32(mid), f32(scale), f32(q)
+    code = np.empty(q.size, np.int8)
+    sg = np.empty(1, np.float32)
+    qc = np.empty(1, np.float32)
+    lib().or_pd_query(metric, q.size, _p(mid), _p(scale), _p(q), _p(code), _p(sg), _p(qc))
+    return code, np.float32(sg[0]), np.float32(qc[0])
+
+
+def pd_distance(metric, sigma, qconst, rnorm, qcode, code):
+    qcode = np.ascontiguousarray(qcode, np.int8)
+    code = np.ascontiguousarray(code, np.int8)
+    return np.float32(lib().or_pd_distance(metric, sigma, qconst, rnorm, _p(qcode), _p(code),
+                                           code.size))
+
+
 # ------------------------------------------------------------ brute force
 def brute_search(metric, keys, vectors, query, k, allow=None, backend=AVX2):
     """vectors are prepared rows. Returns (keys, dists) or raises ValueError(status)."""
@@ -256,7 +320,18 @@ class OracleGraph:
     """Holds numpy arrays in the crate's storage layout and the ctypes view."""
 
     def __init__(self, dim, metric, m, m0, vectors, keys, levels, layer0, upper_off, upper,
-                 entry_id, top, deleted=None, codes=None, sq_offset=None, sq_scale=1.0):
+                 entry_id, top, deleted=None, codes=None, sq_offset=None, sq_scale=1.0,
+                 bf16=False, perdim=None):
+        """bf16: also traverse bf16-rounded rows (precision 2).  perdim:
+        (mid, scale) of a per-dimension quantizer; its codes and row norms
+        are computed here from the rows (precision 3)."""
+        vectors = f32(vectors)
+        bf16_rows = bf16_round(vectors) if bf16 else None
+        pd_codes = pd_mid = pd_scale = pd_rnorm = None
+        if perdim is not None:
+            pd_mid, pd_scale = f32(perdim[0]), f32(perdim[1])
+            pd_codes = pd_encode_rows(pd_mid, pd_scale, vectors)
+            pd_rnorm = pd_rnorm_rows(pd_scale, pd_codes)
         self._keep = dict(
             vectors=f32(vectors), keys=np.ascontiguousarray(keys, np.uint64),
             levels=np.ascontiguousarray(levels, np.uint8),
@@ -265,15 +340,22 @@ class OracleGraph:
             upper=np.ascontiguousarray(upper if len(upper) else np.zeros(1), np.uint32),
             deleted=None if deleted is None else np.ascontiguousarray(deleted, np.uint8),
             codes=None if codes is None else np.ascontiguousarray(codes, np.int8),
-            sq_offset=None if sq_offset is None else f32(sq_offset))
+            sq_offset=None if sq_offset is None else f32(sq_offset),
+            bf16_rows=bf16_rows, pd_codes=pd_codes, pd_mid=pd_mid, pd_scale=pd_scale,
+            pd_rnorm=pd_rnorm)
         k = self._keep
         self.g = Graph(dim, metric, m, m0, len(k["keys"]), entry_id, top,
                        _p(k["vectors"]), _p(k["keys"]), _p(k["levels"]), _p(k["layer0"]),
                        _p(k["upper_off"]), _p(k["upper"]), _p(k["deleted"]), _p(k["codes"]),
-                       _p(k["sq_offset"]), float(sq_scale))
+                       _p(k["sq_offset"]), float(sq_scale), _p(k["bf16_rows"]),
+                       _p(k["pd_codes"]), _p(k["pd_mid"]), _p(k["pd_scale"]), _p(k["pd_rnorm"]))
         self.dim = dim
 
-    def search(self, query, k, ef, quantized=False, allow=None):
+    @staticmethod
+    def _prec(quantized, precision):
+        return int(precision) if precision is not None else (PREC_INT8 if quantized else PREC_F32)
+
+    def search(self, query, k, ef, quantized=False, allow=None, precision=None):
         query = f32(query)
         ok = np.empty(max(k, 1), np.uint64)
         od = np.empty(max(k, 1), np.float32)
@@ -281,13 +363,14 @@ class OracleGraph:
         cnt = Counters()
         al = None if allow is None else np.ascontiguousarray(allow, np.uint8)
         r = lib().or_hnsw_search(ctypes.byref(self.g), _p(query), query.size, k, ef,
-                                 int(quantized), _p(al), _p(ok), _p(od), _p(oi),
+                                 self._prec(quantized, precision), _p(al), _p(ok), _p(od), _p(oi),
                                  ctypes.byref(cnt))
         if r < 0:
             raise ValueError(-r)
         return ok[:r].copy(), od[:r].copy(), oi[:r].copy(), cnt
 
-    def search_batch(self, queries, k, ef, quantized=False, allow=None, threads=8):
+    def search_batch(self, queries, k, ef, quantized=False, allow=None, threads=8,
+                     precision=None):
         queries = f32(queries)
         nq = queries.shape[0]
         ok = np.zeros((nq, max(k, 1)), np.uint64)
@@ -297,7 +380,8 @@ class OracleGraph:
         bad = SZ(0)
         al = None if allow is None else np.ascontiguousarray(allow, np.uint8)
         r = lib().or_hnsw_search_batch(ctypes.byref(self.g), _p(queries), nq, k, ef,
-                                       int(quantized), _p(al), threads, _p(ok), _p(od), _p(oc),
+                                       self._prec(quantized, precision), _p(al), threads, _p(ok),
+                                       _p(od), _p(oc),
                                        ctypes.byref(cnt), ctypes.byref(bad))
         if r < 0:
             raise ValueError(-r, bad.value)

@@ -6,15 +6,23 @@ import hnsw_amd as H
 import oracle as O
 
 
-def oracle_graph(idx):
+def oracle_graph(idx, bf16=False, perdim=False):
     """The oracle's view of the index's current graph (arrays copied, so the
-    view outlives later mutation of the index)."""
+    view outlives later mutation of the index).  bf16: the oracle also
+    traverses bf16-rounded rows; perdim: True trains the per-dimension
+    quantizer on the rows (the oracle's restatement), a (mid, scale) tuple
+    uses those parameters, for precision 3."""
     v = {k: (np.array(x, copy=True) if isinstance(x, np.ndarray) else x)
          for k, x in idx.graph_arrays().items()}
+    if isinstance(perdim, tuple):
+        pd = perdim  # (mid, scale) of a quantizer trained earlier
+    else:
+        pd = O.pd_train(v["vectors"]) if perdim else None
     return O.OracleGraph(v["dim"], v["metric"], v["m"], v["m0"], v["vectors"], v["keys"],
                          v["levels"], v["layer0"], v["upper_off"], v["upper"], v["entry_id"],
                          v["top"], deleted=v["deleted"], codes=v["codes"],
-                         sq_offset=v["sq_offset"], sq_scale=v["sq_scale"])
+                         sq_offset=v["sq_offset"], sq_scale=v["sq_scale"], bf16=bf16,
+                         perdim=pd)
 
 
 def build(base, metric=H.Metric.L2, m=16, keys=None, threads=1, params=None):
@@ -30,3 +38,15 @@ def same_bits(a, b):
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b, np.float32)
     return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def assert_same_results(g, o, what=""):
+    """(keys, dists, counts) of the GPU equal the oracle's: counts, keys, and
+    distance bits of every returned entry."""
+    gk, gd, gc = g
+    ok, od, oc = o
+    assert np.array_equal(gc, oc), f"{what}: counts differ"
+    for i in range(len(gc)):
+        n = int(gc[i])
+        assert np.array_equal(gk[i, :n], ok[i, :n]), f"{what}: keys differ at query {i}"
+        assert same_bits(gd[i, :n], od[i, :n]), f"{what}: distances differ at query {i}"

@@ -5,12 +5,14 @@ without distance ties every query must return the oracle's keys and
 bit-identical distances, and the kernel's distance-evaluation counters must
 equal the oracle's (src/hnsw/search.rs:81-142 evaluated step by step).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
 import hnsw_amd as H
 import oracle as O
-from helpers import build, oracle_graph, same_bits
+from helpers import assert_same_results, build, oracle_graph, same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -228,6 +230,47 @@ def test_inner_product_and_cosine_rank_correctly():
     assert abs(hits[1].distance - (1.0 - 0.7071067811865476)) < 1e-6
 
 
+def test_bf16_rows_are_round_to_nearest_even():
+    """The resident bf16 rows (hh_index_enable_bf16) are the f32 rows
+    rounded to nearest even: equal to the oracle's rounding and to torch's
+    CPU float32 -> bfloat16 conversion, including ties, on rows built so that
+    many values sit exactly half-way between two bf16 values."""
+    import torch
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal((2000, 40)).astype(np.float32)
+    # exact ties (low half 0x8000) with even and odd kept halves, and tiny values
+    u = b.view(np.uint32)
+    u[:500, :20] = (u[:500, :20] & 0xFFFF0000) | 0x8000
+    b[500:600] *= 1e-30
+    idx = build(b)
+    got = idx.bf16_rows()
+    want = (O.bf16_round(b).view(np.uint32) >> 16).astype(np.uint16)
+    ref = torch.from_numpy(b).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(want, ref)
+    assert np.array_equal(got, want)
+
+
+def test_bf16_traversal_matches_the_oracle_bit_for_bit():
+    """bf16 traversal is the crate's search over bf16-rounded rows with the
+    f32 query and the AVX2-order sums, then the exact f32 re-rank: keys,
+    distances and counters equal the oracle's bf16 mode."""
+    for metric in (H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine):
+        for dim in (64, 100, 128):
+            print(f"case bf16 metric={metric} dim={dim}", flush=True)
+            b, q = O.dataset_gaussian(3000, 64, dim, 21 + dim)
+            idx = build(b, metric, threads=4)
+            idx.enable_bf16()
+            g = oracle_graph(idx, bf16=True)
+            for k, ef in ((10, 64), (50, 200)):
+                idx.search_stats(reset=True)
+                res = idx.search_arrays(q, k, ef, precision=H.Precision.BF16)
+                st = idx.search_stats()
+                ok, od, oc, cnt = g.search_batch(q, k, ef, precision=O.PREC_BF16)
+                assert_same_results(res, (ok, od, oc), f"bf16 {metric} {dim} k={k} ef={ef}")
+                assert st["n_dist"] + st["n_upper"] == cnt.n_dist
+                assert st["n_expand"] == cnt.n_expand and st["n_rerank"] == cnt.n_rerank
+
+
 def test_bf16_traversal_reranks_exactly():
     b, q = O.dataset_gaussian(3000, 64, 64, 21)
     idx = build(b)
@@ -242,7 +285,7 @@ def test_bf16_traversal_reranks_exactly():
     fk, fd, fc = idx.search_arrays(q, 10, 64)
     r32 = O.recall_at_k(tk, tc, fk, fc, 10)
     r16 = O.recall_at_k(tk, tc, gk, gc, 10)
-    assert r16 >= r32 - 0.02, (r16, r32)
+    assert r16 >= r32 - 0.005, (r16, r32)
 
 
 def test_large_ef_spills_visited_set_and_stays_exact():
@@ -404,10 +447,14 @@ def test_loaded_index_file_searches_like_the_original():
         idx.remove(key)
     idx.enable_quantization()
     loaded = H.HnswIndex.from_bytes(idx.to_bytes())
+    g = oracle_graph(loaded)
     for prec in (H.Precision.F32, H.Precision.INT8):
         ek, ed, ec = idx.search_arrays(q, 10, 48, precision=prec)
         gk, gd, gc = loaded.search_arrays(q, 10, 48, precision=prec)
         assert np.array_equal(ec, gc) and np.array_equal(ek, gk) and same_bits(ed, gd), prec
+        # and the loaded graph searched by the oracle
+        ok, od, oc, _ = g.search_batch(q, 10, 48, precision=int(prec))
+        assert_same_results((gk, gd, gc), (ok, od, oc), f"loaded file {prec.name}")
 
 
 def test_longest_first_order_returns_what_one_pass_returns():
@@ -606,3 +653,129 @@ def test_descent_kernel_two_pass_bf16_and_int8(metric):
     g = oracle_graph(idx)
     ok, od, oc, _ = g.search_batch(q[:300], 10, 64, quantized=True)
     assert np.array_equal(k2[:300], ok) and same_bits(d2[:300], od)
+
+
+def test_pool_overflow_reruns_the_query_exactly():
+    """A 0.2% filter at ef=512 over 150,000 nodes never fills its results,
+    so every visited node stays a live candidate: the oracle's heap peaks
+    above 90,000, more than a wave's 65,536-entry pool plus its LDS list
+    hold.  Such a query is listed and run again by the re-run pass with an
+    n-entry pool, so its results, distances and expansions are the oracle's
+    unbounded heap's (src/hnsw/search.rs:125-135), no candidate is reported
+    dropped, and the re-runs are counted."""
+    n, dim = 150000, 32
+    b, q = O.dataset_gaussian(n, 24, dim, 23)
+    idx = build(b, H.Metric.L2, m=16, threads=16)
+    g = oracle_graph(idx)
+    allow = (np.arange(n) % 500) == 3
+    idx.search_stats(reset=True)
+    res = idx.search_arrays(q, 100, 512, filter=allow)
+    st = idx.search_stats()
+    print(f"case 0.2% filter over 150k nodes: {st}", flush=True)
+    ok, od, oc, cnt = g.search_batch(q, 100, 512, allow=allow.astype(np.uint8))
+    assert cnt.max_cands > 65536 + 1024, cnt.max_cands  # the first pool must overflow
+    assert st["n_redo"] > 0 and st["n_overflow"] == 0, st
+    assert st["n_expand"] == cnt.n_expand and st["n_dist"] + st["n_upper"] == cnt.n_dist, st
+    assert_same_results(res, (ok, od, oc), "0.2% filter")
+
+
+@pytest.mark.parametrize("metric", [H.Metric.L2, H.Metric.InnerProduct, H.Metric.Cosine])
+@pytest.mark.parametrize("dim", [37, 128, 256, 300])
+def test_perdim_quantizer_and_search_match_the_oracle(metric, dim):
+    """Per-dimension min/max int8 traversal: the device-trained parameters,
+    codes and L2 row terms equal the oracle's restatement bit for bit, and
+    searches (traversal on the codes, exact f32 re-rank) return the
+    oracle's keys, distances and counters."""
+    print(f"case perdim metric={metric} dim={dim}", flush=True)
+    b, q = O.dataset_gaussian(3000, 64, dim, 5 + dim)
+    idx = build(b, metric, threads=8)
+    with pytest.raises(H.QuantizationDisabled):
+        idx.search_arrays(q[:1], 10, 64, precision=H.Precision.INT8_PERDIM)
+    mid, scale = idx.enable_quantization_perdim()
+    v = idx.graph_arrays()
+    omid, oscale = O.pd_train(v["vectors"])
+    assert same_bits(mid, omid) and same_bits(scale, oscale)
+    codes, rn = idx.perdim_codes()
+    ocodes = O.pd_encode_rows(omid, oscale, v["vectors"])
+    assert np.array_equal(codes, ocodes)
+    if metric == H.Metric.L2:
+        assert same_bits(rn, O.pd_rnorm_rows(oscale, ocodes))
+    g = oracle_graph(idx, perdim=True)
+    for k, ef in ((10, 64), (1, 16), (32, 200)):
+        idx.search_stats(reset=True)
+        res = idx.search_arrays(q, k, ef, precision=H.Precision.INT8_PERDIM)
+        st = idx.search_stats()
+        ok, od, oc, cnt = g.search_batch(q, k, ef, precision=O.PREC_INT8_PERDIM)
+        assert_same_results(res, (ok, od, oc), f"perdim k={k} ef={ef}")
+        assert st["n_dist"] + st["n_upper"] == cnt.n_dist and st["n_expand"] == cnt.n_expand
+        assert st["n_rerank"] == cnt.n_rerank
+
+
+def test_perdim_two_pass_mixed_and_synced_inserts_match_the_oracle():
+    """Per-dimension codes through the descent kernel (two-pass launches),
+    in mixed batches beside the other precisions, and after inserts synced
+    to the resident index (new rows encoded with the resident parameters)."""
+    b, q = O.dataset_gaussian(5000, 12000, 128, 61)
+    idx = build(b[:4000], H.Metric.L2, m=16, threads=8)
+    pd = idx.enable_quantization_perdim()
+    g = oracle_graph(idx, perdim=pd)
+    runs = {}
+    for longest_first in (False, True):
+        idx.set_query_order(longest_first)
+        runs[longest_first] = idx.search_arrays(q, 10, 64, precision=H.Precision.INT8_PERDIM)
+    assert_same_results(runs[True], runs[False], "two-pass vs one-pass")
+    ok, od, oc, _ = g.search_batch(q[:300], 10, 64, precision=O.PREC_INT8_PERDIM)
+    assert_same_results(tuple(x[:300] for x in runs[True]), (ok, od, oc), "two-pass vs oracle")
+    rng = np.random.default_rng(2)
+    nq = 200
+    ks = rng.choice([1, 10, 40], nq)
+    efs = rng.integers(16, 200, nq)
+    precs = rng.choice([0, 3], nq)
+    gk, gd, gc, gs = idx.search_mixed(q[:nq], ks, efs, precs)
+    assert (gs == 0).all()
+    for i in range(nq):
+        okk, odd, _, _ = g.search(q[i], int(ks[i]), int(efs[i]), precision=int(precs[i]))
+        assert np.array_equal(gk[i, :gc[i]], okk) and same_bits(gd[i, :gc[i]], odd), i
+    idx.insert_batch(np.arange(4000, 5000, dtype=np.uint64), b[4000:], threads=8)
+    res = idx.search_arrays(q[:300], 10, 64, precision=H.Precision.INT8_PERDIM)
+    g2 = oracle_graph(idx, perdim=pd)
+    codes, _ = idx.perdim_codes()
+    assert np.array_equal(codes, O.pd_encode_rows(pd[0], pd[1], idx.graph_arrays()["vectors"]))
+    ok, od, oc, _ = g2.search_batch(q[:300], 10, 64, precision=O.PREC_INT8_PERDIM)
+    assert_same_results(res, (ok, od, oc), "after synced inserts")
+
+
+def test_changes_are_refused_while_served_and_stale_plans_refuse_to_launch():
+    """A serving loop reads the resident arrays: while one is open, every
+    change of the index is refused (Python and C ABI), so no launch can see
+    freed memory; a graph-captured plan made before a change refuses to
+    launch after it."""
+    from hnsw_amd.device import Buffer, Stream
+    b, q = O.dataset_gaussian(3000, 40, 32, 5)
+    idx = build(b[:2500], threads=4)
+    h = idx.device_index()
+    with H.Server(idx, k_max=10) as srv:
+        with pytest.raises(H.InvalidArgument):
+            idx.insert_batch(np.arange(2500, 3000, dtype=np.uint64), b[2500:])
+        with pytest.raises(H.InvalidArgument):
+            idx.remove(3)
+        assert H.lib.hh_index_enable_bf16(h) == H.InvalidArgument.status
+        assert H.lib.hh_index_quantize_perdim(h, None, None) == H.InvalidArgument.status
+        keys, dist, st, _ = srv.poll(srv.submit(q[0], 5, 32))
+        assert st == 0 and len(keys) == 5
+    nq, k = 40, 10
+    qd = Buffer.from_array(q)
+    outs = [Buffer(nq * k * 8), Buffer(nq * k * 4), Buffer(nq * 4), Buffer(nq * 4)]
+    plan = ctypes.c_void_p()
+    assert H.lib.hh_search_plan_create(h, qd.ptr, nq, k, 32, 0, None, None, 0, None, 0,
+                                       *[t.ptr for t in outs], ctypes.byref(plan)) == 0
+    stream = Stream()
+    try:
+        assert H.lib.hh_search_plan_launch(plan, stream.handle) == 0
+        stream.synchronize()
+        idx.insert_batch(np.arange(2500, 3000, dtype=np.uint64), b[2500:], threads=4)
+        assert idx.device_index().value == h.value  # synced in place
+        assert H.lib.hh_search_plan_launch(plan, stream.handle) == H.InvalidArgument.status
+    finally:
+        H.lib.hh_search_plan_free(plan)
+        stream.close()

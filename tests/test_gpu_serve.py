@@ -8,7 +8,7 @@ import pytest
 
 import hnsw_amd as H
 import oracle as O
-from helpers import build, same_bits
+from helpers import build, oracle_graph, same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -45,6 +45,7 @@ def test_results_are_the_same_alone_and_in_the_mix():
     assert st["batches"] > 1, "requests were admitted in several launches"
     lat = np.array([g[3] for g in got])
     assert (lat > 0).all()
+    g = oracle_graph(idx, bf16=True)
     for i in range(400):
         flt = None if fsel[i] < 0 else masks[fsel[i]]
         ak, ad, ac = idx.search_arrays(q[i:i + 1], int(ks[i]), int(efs[i]), H.Precision(int(precs[i])), flt)
@@ -52,6 +53,10 @@ def test_results_are_the_same_alone_and_in_the_mix():
         assert gs == 0
         assert np.array_equal(gk, ak[0, :ac[0]]), i
         assert same_bits(gd, ad[0, :ac[0]]), i
+        # and the oracle's search of the same request
+        ok, od, _, _ = g.search(q[i], int(ks[i]), int(efs[i]), precision=int(precs[i]),
+                                allow=None if flt is None else flt.astype(np.uint8))
+        assert np.array_equal(gk, ok) and same_bits(gd, od), i
 
 
 def test_queue_limit_and_invalid_requests():
@@ -65,3 +70,22 @@ def test_queue_limit_and_invalid_requests():
         assert status == 4 and len(keys) == 0
         with pytest.raises(H.InvalidParameter):
             srv.submit(q[0], 11, 20)
+
+
+def test_launches_take_one_ef_bucket_at_a_time():
+    """Requests queue per ef bucket and a launch takes one bucket: a burst
+    of mixed ef 16 / 256 requests is served in launches that each hold one
+    bucket (so at least two launches), and every result is the oracle's."""
+    b, q = O.dataset_gaussian(3000, 200, 32, 41)
+    idx = build(b, threads=4)
+    g = oracle_graph(idx)
+    efs = np.where(np.arange(200) % 2 == 0, 16, 256)
+    ks = np.full(200, 10)
+    with H.Server(idx, k_max=10, max_inflight=1) as srv:
+        lat, sk, sd, sc, sst = srv.replay(q, ks, efs, np.zeros(200))
+        st = srv.stats()
+    assert (sst == 0).all() and st["batches"] >= 2
+    for i in range(200):
+        ok, od, _, _ = g.search(q[i], 10, int(efs[i]))
+        c = int(sc[i])
+        assert np.array_equal(sk[i, :c], ok) and same_bits(sd[i, :c], od), i

@@ -332,3 +332,92 @@ def test_sq_distance_rounds_like_the_crate():
         once = np.float32(float(l2) * float(scale) * float(scale))
         f64_differs += int(once != _sq_distance_crate(O.L2, scale, a, b))
     assert f64_differs > 0
+
+
+def test_bf16_round_is_torch_round_to_nearest_even():
+    """or_bf16_round against torch's CPU float32 -> bfloat16 conversion
+    (round to nearest, ties to even), on random values, exact ties with even
+    and odd kept halves, subnormals, the largest finite values and infinities."""
+    import torch
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(20000).astype(np.float32) * np.float32(10.0) ** rng.integers(-30, 30, 20000).astype(np.float32)
+    u = x.view(np.uint32)
+    u[:4000] = (u[:4000] & 0xFFFF0000) | 0x8000  # exact ties
+    extra = np.array([0.0, -0.0, 1e-40, -1e-42, 3.4028235e38, -3.4028235e38, np.inf, -np.inf,
+                      1.0 + 2.0**-8, 1.0 + 3 * 2.0**-8], np.float32)
+    x = np.concatenate([x, extra])
+    got = O.bf16_round(x)
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_perdim_quantizer_closed_forms():
+    """The per-dimension quantizer (no crate counterpart): mid and scale
+    are the column midpoint and range / 254; every value is reconstructed
+    to within half a step (no value of the training rows is clamped); a
+    constant column codes to 0 with scale 1; the L2 row term is the f32
+    rounding of sum (scale_d * code_d)^2."""
+    rng = np.random.default_rng(12)
+    rows = (rng.standard_normal((400, 23)) * rng.uniform(0.01, 50, 23)).astype(np.float32)
+    rows[:, 5] = 2.5  # constant column
+    mid, scale = O.pd_train(rows)
+    lo, hi = rows.min(0), rows.max(0)
+    assert np.array_equal(mid, ((lo + hi) * np.float32(0.5)))
+    want_scale = (hi - lo) / np.float32(254)
+    want_scale[5] = 1.0
+    assert np.array_equal(scale, want_scale)
+    codes = O.pd_encode_rows(mid, scale, rows)
+    assert np.abs(codes.astype(np.int64)).max() <= 127 and (codes[:, 5] == 0).all()
+    recon = mid.astype(np.float64) + scale.astype(np.float64) * codes
+    err = np.abs(rows - recon)
+    assert (err <= scale.astype(np.float64) * 0.5 * (1 + 1e-5) + 1e-30).all()
+    rn = O.pd_rnorm_rows(scale, codes)
+    import math
+    for i in range(0, 400, 7):
+        exact = math.fsum((float(scale[d]) * int(codes[i, d])) ** 2 for d in range(23))
+        assert abs(float(rn[i]) - exact) <= 2.0**-23 * exact
+
+
+@pytest.mark.parametrize("metric", [O.L2, O.IP, O.COSINE])
+def test_perdim_distance_error_bound(metric):
+    """The per-dimension traversal distance against the exact (f64)
+    distance to the stored row: the query weights are within sigma/2 of
+    sigma*qcode, the rows within scale/2 of their reconstruction, so
+    |approx - exact| <= sigma/2 * sum|code| + sum|qw_d| * scale_d/2 (+ the
+    x - x-hat term for L2) + f32 rounding.  A dropped term (the row norm,
+    qconst) or a wrong sign breaks the bound."""
+    rng = np.random.default_rng(metric)
+    dim = 96
+    rows = rng.standard_normal((300, dim)).astype(np.float32)
+    qs = rng.standard_normal((20, dim)).astype(np.float32)
+    if metric == O.COSINE:
+        rows /= np.linalg.norm(rows, axis=1, keepdims=True)
+        qs /= np.linalg.norm(qs, axis=1, keepdims=True)
+    mid, scale = O.pd_train(rows)
+    codes = O.pd_encode_rows(mid, scale, rows)
+    rn = O.pd_rnorm_rows(scale, codes) if metric == O.L2 else np.zeros(300, np.float32)
+    s64 = scale.astype(np.float64)
+    for q in qs:
+        qcode, sigma, qconst = O.pd_query(metric, mid, scale, q)
+        q64 = q.astype(np.float64)
+        w = (q64 - mid) * s64 if metric == O.L2 else q64 * s64
+        assert np.abs(w - float(sigma) * qcode).max() <= float(sigma) * 0.5 * (1 + 1e-5)
+        for i in range(0, 300, 3):
+            x = rows[i].astype(np.float64)
+            xh = mid.astype(np.float64) + s64 * codes[i]
+            approx = float(O.pd_distance(metric, sigma, qconst, rn[i], qcode, codes[i]))
+            if metric == O.L2:
+                exact_h = float(((q64 - xh) ** 2).sum())
+                exact = float(((q64 - x) ** 2).sum())
+                # quantising the weights: 2 * |sum (w - sigma b) c|
+                bound_q = 2 * float(sigma) * 0.5 * np.abs(codes[i]).sum()
+                bound_x = abs(exact - exact_h)
+            else:
+                dot_h = float(q64 @ xh)
+                exact_h = -dot_h if metric == O.IP else 1.0 - dot_h
+                exact = -float(q64 @ x) if metric == O.IP else 1.0 - float(q64 @ x)
+                bound_q = float(sigma) * 0.5 * np.abs(codes[i]).sum()
+                bound_x = float((np.abs(q64) * s64 * 0.5).sum())
+            tol = 1e-5 * (abs(exact_h) + float(np.abs(w).sum()) * 127 + 1.0)
+            assert abs(approx - exact_h) <= bound_q + tol, (i, approx, exact_h)
+            assert abs(approx - exact) <= bound_q + bound_x + tol, (i, approx, exact)
