@@ -2503,3 +2503,137 @@ int hh_brute_search_batch_f64(hh_brute *b, const float *queries, uint64_t nq, ui
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ no-allocation search
+// hh_search_batch_device_noalloc: the search of hh_search_batch_device with
+// its whole workspace (prepared queries and codes, statuses, seeds, order,
+// re-run list, visited bitsets, candidate pools, spill logs, re-run pools)
+// carved from one static device array instead of allocated, for callers
+// that cannot allocate device memory (code under HIP graph capture, tight
+// serving loops).  Each device holds its own copy of the array for the
+// life of the process; every call on a device shares it, so calls that
+// overlap on one device race.
+namespace {
+constexpr uint64_t kScratchBytes = 64ull << 20;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+unsigned char *scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up on the
+// first call there.  The array's name is a host-side stand-in, never a
+// device address: kernels and copies get this pointer.
+int scratch_base(int device, unsigned char **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(HH_INVALID_ARGUMENT, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_addr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(HH_DEVICE_ERROR, std::string("scratch address: ") + hipGetErrorString(e));
+    scratch_addr[device] = static_cast<unsigned char *>(p);
+  }
+  *out = scratch_addr[device];
+  return HH_OK;
+}
+
+// Bump allocator over the scratch array (256-byte aligned pieces); the
+// caller checks `used` against the array's size before using any piece.
+struct Carve {
+  unsigned char *base;
+  uint64_t used;
+  template <class T>
+  T *take(uint64_t count) {
+    used = (used + 255) / 256 * 256;
+    T *p = reinterpret_cast<T *>(base + used);
+    used += count * sizeof(T);
+    return p;
+  }
+};
+
+uint64_t round256(uint64_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+int hh_search_batch_device_noalloc(hh_index *idx, const float *queries, uint64_t nq, uint32_t k,
+                                   uint32_t ef, uint32_t precision, const uint32_t *allow_bits,
+                                   uint64_t *out_keys, float *out_dist, uint32_t *out_count,
+                                   int32_t *status, void *stream) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (nq == 0) return HH_OK;
+  if (nq > kScratchBytes)
+    return fail(HH_UNSUPPORTED, "batch too large for the 64 MiB scratch of the no-allocation search");
+  const SearchCall c{queries, nq, k, ef, precision, allow_bits, nullptr, k, nullptr, 0,
+                     out_keys, out_dist, out_count, status};
+  std::shared_lock<std::shared_mutex> rd(idx->mu);
+  int st = check_call(idx, c);
+  if (st) return st;
+  HH_HIP(hipSetDevice(idx->device));
+  unsigned char *base = nullptr;
+  st = scratch_base(idx->device, &base);
+  if (st) return st;
+  SearchConfig cfg;
+  st = search_config(idx, c, cfg);
+  if (st) return st;
+  const uint64_t n = idx->n, words = cfg.spill_words;
+  const uint32_t stride = idx->lay.stride, cstride = std::max<uint32_t>(idx->cstride, 1);
+  // pools as the allocating path sizes them; when they can drop a
+  // candidate, four n-entry pools for the re-run pass
+  const uint32_t pcap =
+      (uint32_t)std::min<uint64_t>(kCandPool, (std::max<uint64_t>(32 * words, 64) + 63) / 64 * 64);
+  const uint64_t big_cap = (std::max<uint64_t>(n, 64) + 63) / 64 * 64;
+  const bool need_big = pcap < n;
+  Carve cv{base, 0};
+  Workspace ws;  // views into the scratch array only: nothing to release
+  ws.counter = cv.take<uint32_t>(4);
+  ws.qprep = cv.take<float>(nq * stride);
+  ws.qcode = cv.take<int8_t>(nq * cstride);
+  ws.status = status ? nullptr : cv.take<int32_t>(nq);
+  ws.seed = cv.take<uint64_t>(nq);
+  ws.order = cv.take<uint32_t>(nq);
+  ws.redo = cv.take<uint32_t>(nq);
+  ws.qpd = cv.take<float2>(nq);
+  if (need_big) {
+    ws.bigpool = cv.take<uint64_t>(4 * big_cap);
+    ws.big_slots = 4;
+    ws.big_cap = (uint32_t)big_cap;
+  }
+  const uint64_t fixed = round256(cv.used);
+  const uint64_t per_wave = round256(words * 4) + round256((uint64_t)pcap * 8) + round256(kSpillLog * 4);
+  // waves: the launch shape's, no more than one per query (whole blocks),
+  // and no more than the rest of the array holds
+  const uint64_t wpb = (uint64_t)cfg.wpb;
+  uint64_t waves = std::min<uint64_t>(cfg.blocks * wpb, (nq + wpb - 1) / wpb * wpb);
+  waves = fixed < kScratchBytes ? std::min<uint64_t>(waves, (kScratchBytes - fixed) / per_wave / wpb * wpb) : 0;
+  if (waves < wpb)
+    return fail(HH_UNSUPPORTED,
+                "batch and index too large for the 64 MiB scratch of the no-allocation search "
+                "(it holds the queries, " + std::to_string(wpb) +
+                    " candidate pools and visited bitsets, and the re-run pools); "
+                    "use hh_search_batch_device or a smaller batch");
+  ws.spill = cv.take<uint32_t>(waves * words);
+  ws.cpool = cv.take<uint64_t>(waves * pcap);
+  ws.slog = cv.take<uint32_t>(waves * kSpillLog);
+  if (cv.used > kScratchBytes) return fail(HH_UNSUPPORTED, "scratch layout exceeds the array");
+  ws.pool_cap = pcap;
+  ws.spill_slots = waves;
+  ws.spill_wcap = words;
+  ws.cap_q = nq;
+  cfg.blocks = waves / wpb;
+  hipStream_t s = (hipStream_t)stream;
+  // the array keeps the last call's contents: the visited bitsets must
+  // start clear (a fill kernel, so the call can be captured in a graph)
+  hipLaunchKernelGGL(hhk::fill_u32_kernel, dim3(grid_for(waves * words)), dim3(256), 0, s,
+                     ws.spill, waves * words, 0u);
+  HH_HIP(hipGetLastError());
+  return enqueue_search(idx, ws, c, cfg, s);
+}
+
+}  // extern "C"

@@ -100,6 +100,7 @@ _sig("hh_index_last_kernel_ms", INT, P, ctypes.POINTER(ctypes.c_float))
 _sig("hh_index_gather_probe", INT, P, U32, ctypes.POINTER(ctypes.c_double),
      ctypes.POINTER(ctypes.c_float))
 _sig("hh_search_batch_device", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, P)
+_sig("hh_search_batch_device_noalloc", INT, P, P, U64, U32, U32, U32, P, P, P, P, P, P)
 _sig("hh_search_batch_spec_device", INT, P, P, U64, P, U32, U32, P, U32, P, P, P, P, P)
 _sig("hh_search_batch_spec", INT, P, P, U64, P, U32, P, U32, P, P, P, P)
 _sig("hh_search_plan_create", INT, P, P, U64, U32, U32, U32, P, P, U32, P, U32, P, P, P, P,

@@ -379,6 +379,26 @@ int hh_search_batch_device(hh_index *idx, const float *queries, uint64_t nq, uin
                            uint64_t *out_keys, float *out_dist, uint32_t *out_count,
                            int32_t *status, void *stream);
 
+/* The same search with no device allocation (no counterpart in the
+ * reference: it is for callers that cannot allocate device memory, such as
+ * code under HIP graph capture or a tight serving loop).  Arguments and
+ * results as hh_search_batch_device, whose kernels it runs; its workspace
+ * (prepared queries, statuses when status is NULL, seeds, the query order,
+ * visited bitsets, candidate pools and the re-run pools) is carved from a
+ * static 64 MiB device array that each device holds for the life of the
+ * process.  Every call on a device shares that array and it keeps the last
+ * call's contents, so calls that overlap on one device race: run them on
+ * one stream (or order them) per device.  The launch takes as many
+ * wavefronts as fit beside the batch (at most one per query); a batch and
+ * index that do not fit one block of wavefronts (roughly: nq * (row +
+ * code + 32) bytes plus, per wavefront, n/8 bytes of bitset and up to
+ * 512 KiB of pool, plus 32 * n bytes of re-run pools when n > 65,536)
+ * return HH_UNSUPPORTED before anything is launched. */
+int hh_search_batch_device_noalloc(hh_index *idx, const float *queries, uint64_t nq, uint32_t k,
+                                   uint32_t ef, uint32_t precision, const uint32_t *allow_bits,
+                                   uint64_t *out_keys, float *out_dist, uint32_t *out_count,
+                                   int32_t *status, void *stream);
+
 /* Per-query parameters for one mixed launch. filter: index of a bitmap in
  * the filter table passed alongside, or 0xFFFFFFFF for none. */
 typedef struct hh_query_spec {
