@@ -569,21 +569,25 @@ int pack_deleted(const uint8_t *deleted, uint64_t n, std::vector<uint32_t> &bits
 }
 
 template <int TMAX, bool EX, bool XA, int SW>
-void *search_kernel_ptr() {
-  return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW>);
+void *search_kernel_ptr(int pc) {
+  switch (pc) {
+    case hhk::kPrecBf16: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecBf16>);
+    case hhk::kPrecCodes: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecCodes>);
+    default: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecAny>);
+  }
 }
 
-// The search kernel instance for a row layout and list size.
-void *pick_search_kernel(const hh::RowLayout &lay, int sw) {
+// The search kernel instance for a row layout, list size and precision class.
+void *pick_search_kernel(const hh::RowLayout &lay, int sw, int pc) {
   const int tmax = tmax_for(lay);
   const bool extra = lay.U > 0 || lay.R > 0;
   const bool exact = !extra && (int)lay.T == tmax;
-#define HH_PICK(TM, EX, XA)                                      \
-  if (tmax == TM && extra == EX && exact == XA) {                \
-    if (sw == 1) return search_kernel_ptr<TM, EX, XA, 1>();      \
-    if (sw == 2) return search_kernel_ptr<TM, EX, XA, 2>();      \
-    if (sw == 4) return search_kernel_ptr<TM, EX, XA, 4>();      \
-    if (sw == 8) return search_kernel_ptr<TM, EX, XA, 8>();      \
+#define HH_PICK(TM, EX, XA)                                        \
+  if (tmax == TM && extra == EX && exact == XA) {                  \
+    if (sw == 1) return search_kernel_ptr<TM, EX, XA, 1>(pc);      \
+    if (sw == 2) return search_kernel_ptr<TM, EX, XA, 2>(pc);      \
+    if (sw == 4) return search_kernel_ptr<TM, EX, XA, 4>(pc);      \
+    if (sw == 8) return search_kernel_ptr<TM, EX, XA, 8>(pc);      \
   }
   HH_PICK(4, false, false)
   HH_PICK(4, true, false)
@@ -645,6 +649,18 @@ struct SearchCall {
   uint64_t filter_words = 0;
 };
 
+// The precision class of a call: a uniform bf16 or int8-code batch runs
+// the instance of its precision (no registers or code for the others in
+// its loop); f32 batches, construction searches and mixed batches run the
+// general instance, which measured faster for f32 than an f32-only one
+// (7.6 vs 6.7 M QPS at 1M x 128, ef=64).
+int precision_class(const SearchCall &c) {
+  if (c.specs || c.build) return hhk::kPrecAny;
+  if (c.precision == 2) return hhk::kPrecBf16;
+  if (c.precision == 1 || c.precision == 3) return hhk::kPrecCodes;
+  return hhk::kPrecAny;
+}
+
 // Largest max(ef, k) a launch must serve: the call's own for a uniform
 // batch; for a mixed batch the caller's bound (c.ef, 0 = 512).  Mixed-batch
 // queries above the bound fail with HH_UNSUPPORTED in the prepare kernel.
@@ -677,7 +693,7 @@ int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
   cfg.hash_cap = idx->tuned_hash_cap[bucket] ? idx->tuned_hash_cap[bucket] : 2048u;
   cfg.hash_cap = std::max<uint32_t>(cfg.hash_cap, 256u * (uint32_t)sw);
   cfg.lds_words = lds_words_for(cfg.hash_cap, sw);
-  cfg.kern = pick_search_kernel(idx->lay, sw);
+  cfg.kern = pick_search_kernel(idx->lay, sw, precision_class(c));
   // waves per block: the tuned value, else whichever of 1, 2, 4 keeps the
   // most waves resident
   int blocks_per_cu = 1, best_waves = 0;

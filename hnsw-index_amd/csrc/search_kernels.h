@@ -95,7 +95,11 @@ struct PrepArgs {
 
 constexpr uint32_t kOrderBins = 4096;
 
-template <int TMAX, bool EXTRA, bool EXACT, int SW>
+// Precision classes of the search kernel instances: bf16 rows, int8 codes
+// (the crate's or the per-dimension ones), or every precision.
+constexpr int kPrecBf16 = 1, kPrecCodes = 2, kPrecAny = 3;
+
+template <int TMAX, bool EXTRA, bool EXACT, int SW, int PC>
 __global__ void hnsw_search_kernel(const SearchArgs P);
 // First pass of a two-pass launch for m <= LPQ (16 or 32 lanes per query):
 // descent_threads / LPQ queries per block (grid ceil(nq / that)), each
