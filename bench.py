@@ -86,10 +86,19 @@ def main():
                     help="host threads for the benchmark graph's build; 1 (the default) builds "
                          "the same graph every run, so recall at each ef is reproducible (a "
                          "parallel build, like the crate's insert_batch, is not deterministic)")
+    ap.add_argument("--graph-cache", default="",
+                    help="index file of the benchmark graph: loaded (validated) when it exists, "
+                         "else written after the build; with the default one-thread build the "
+                         "file holds the same graph every run builds, so profiler runs need not "
+                         "build it again")
     ap.add_argument("--f64-check", type=int, default=500)
     ap.add_argument("--quick", action="store_true",
                     help="search measurement only: no precision variants, serving or CPU "
                          "baseline (for profiler runs)")
+    ap.add_argument("--brute-only", action="store_true",
+                    help="time only the exact brute force (MFMA shortlist and streaming "
+                         "kernel) over the workload's rows (for profiler runs) and print its "
+                         "JSON line")
     ap.add_argument("--quantizer-only", action="store_true",
                     help="time only the device quantizer on the uploaded index (for "
                          "profiler runs) and print its JSON line")
@@ -127,14 +136,25 @@ def main():
 
     log(f"generating {n} x {dim} clustered rows + {nq} queries")
     data = H.synthetic.Dataset.uniform_clusters(n, nq, dim, 1000, 0.1, 0.12, 42, threads=threads)
-    idx = H.HnswIndex(dim, H.Metric.L2, H.HnswParams(16))
+    if args.brute_only:
+        return run_brute(args, H, data)
     keys = np.arange(n, dtype=np.uint64)
-    chunk = 100_000
     t0 = time.time()
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
-        idx.insert_batch(keys[s:e], data.base()[s:e], threads=args.build_threads)
-        log(f"graph build: {e}/{n} linked ({time.time() - t0:.1f} s, {args.build_threads} threads)")
+    if args.graph_cache and os.path.exists(args.graph_cache):
+        idx = H.HnswIndex.read_from(args.graph_cache)
+        if idx.node_count() != n or idx.dim() != dim:
+            raise SystemExit(f"{args.graph_cache} does not hold this workload's graph")
+        log(f"graph loaded from {args.graph_cache} ({time.time() - t0:.1f} s)")
+    else:
+        idx = H.HnswIndex(dim, H.Metric.L2, H.HnswParams(16))
+        chunk = 100_000
+        for s in range(0, n, chunk):
+            e = min(n, s + chunk)
+            idx.insert_batch(keys[s:e], data.base()[s:e], threads=args.build_threads)
+            log(f"graph build: {e}/{n} linked ({time.time() - t0:.1f} s, "
+                f"{args.build_threads} threads)")
+        if args.graph_cache:
+            idx.write_to(args.graph_cache)
     build_s = time.time() - t0
 
     # device buffers and the launch stream, on the library's own HIP runtime
@@ -319,20 +339,27 @@ def main():
         emit(args, locals())
         return
 
-    # precision variants at the same ef
+    # precision variants at the same ef, each timed the same way as f32 is
+    # here (vs_f32 is the ratio of these two timings)
+    reps = max(3, args.steps // 2)
+    f32_sec = timed(ef, H.Precision.F32, reps)
     idx.enable_bf16()
-    for name, prec in (("bf16", H.Precision.BF16),):
+    quant = measure_quantizer(H, idx, 3)
+    t = time.time()
+    idx.enable_quantization_perdim()
+    perdim_s = time.time() - t
+    h = idx.device_index()
+    for name, prec in (("bf16", H.Precision.BF16), ("int8", H.Precision.INT8),
+                       ("int8_perdim", H.Precision.INT8_PERDIM)):
         run(ef, prec)
         r = recall()
-        sec = timed(ef, prec, max(3, args.steps // 2))
-        variants[name] = {"recall": round(r, 4), "qps": round(nq / sec)}
-        log(f"{name} ef={ef}: recall={r:.4f} qps={nq / sec:,.0f}")
-    quant = measure_quantizer(H, idx, 3)
-    run(ef, H.Precision.INT8)
-    r = recall()
-    sec = timed(ef, H.Precision.INT8, max(3, args.steps // 2))
-    variants["int8"] = {"recall": round(r, 4), "qps": round(nq / sec), "quantizer": quant}
-    log(f"int8 ef={ef}: recall={r:.4f} qps={nq / sec:,.0f}")
+        sec = timed(ef, prec, reps)
+        variants[name] = {"recall": round(r, 4), "qps": round(nq / sec),
+                          "vs_f32": round(f32_sec / sec, 3)}
+        log(f"{name} ef={ef}: recall={r:.4f} qps={nq / sec:,.0f} ({f32_sec / sec:.2f}x f32)")
+    variants["int8"]["quantizer"] = quant
+    variants["int8_perdim"]["quantizer_s"] = round(perdim_s, 3)
+    variants["f32_same_timing"] = {"qps": round(nq / f32_sec)}
 
     # serving loop: Poisson arrivals replayed by the native load generator
     try:
@@ -366,33 +393,55 @@ def main():
         g = O.OracleGraph(v["dim"], v["metric"], v["m"], v["m0"], v["vectors"], v["keys"],
                           v["levels"], v["layer0"], v["upper_off"], v["upper"], v["entry_id"],
                           v["top"])
+        import resource
         ns = min(args.cpu_sample, nq)
         sample = data.queries()[:ns]
         load0 = os.getloadavg()[0]
+        # all the cores this process may run on, up to the host share of
+        # one GPU (16 cores on these hosts), however many the OS lists
+        cores = min(len(os.sched_getaffinity(0)), 16)
         # warm-up pass (also the recall of the CPU results), then the median
-        # of 3 timed passes on all threads, then one single-thread pass
-        ok, od, oc, _ = g.search_batch(sample, k, ef, threads=threads)
+        # of 3 timed passes on all those cores (wall clock, as the crate's
+        # 8-thread figures), then the median of 3 single-thread passes timed
+        # per CPU-second of the searching thread (getrusage, as the crate's
+        # hnsw-bench reports single-thread throughput)
+        ok, od, oc, _ = g.search_batch(sample, k, ef, threads=cores)
         cpu_recall = O.recall_at_k(tk[:ns], tc[:ns], ok, oc, k)
-        passes = []
+        passes, proc_cpu = [], []
         for _ in range(3):
+            r0 = resource.getrusage(resource.RUSAGE_SELF)
             t = time.perf_counter()
-            g.search_batch(sample, k, ef, threads=threads)
+            g.search_batch(sample, k, ef, threads=cores)
             passes.append(time.perf_counter() - t)
+            r1 = resource.getrusage(resource.RUSAGE_SELF)
+            proc_cpu.append((r1.ru_utime + r1.ru_stime) - (r0.ru_utime + r0.ru_stime))
         cpu_s = sorted(passes)[1]
         n1 = min(ns, 2000)
-        t = time.perf_counter()
-        g.search_batch(sample[:n1], k, ef, threads=1)
-        one_s = time.perf_counter() - t
+        one_wall, one_cpu = [], []
+        for _ in range(3):
+            r0 = resource.getrusage(resource.RUSAGE_THREAD)
+            t = time.perf_counter()
+            g.search_batch(sample[:n1], k, ef, threads=1)  # runs on this thread
+            one_wall.append(time.perf_counter() - t)
+            r1 = resource.getrusage(resource.RUSAGE_THREAD)
+            one_cpu.append((r1.ru_utime + r1.ru_stime) - (r0.ru_utime + r0.ru_stime))
+        one_s, one_c = sorted(one_wall)[1], sorted(one_cpu)[1]
         load1 = os.getloadavg()[0]
-        cpu = {"value": round(ns / cpu_s), "unit": "queries/s", "cores": threads, "kind": "port",
+        cpu = {"value": round(ns / cpu_s), "unit": "queries/s", "cores": cores, "kind": "port",
                "sample": f"{ns} of the {nq} queries, same graph, k={k}, ef={ef}: warm-up pass, "
                          f"then median of 3 passes ({', '.join(f'{p:.2f}' for p in passes)} s) "
-                         f"on {threads} threads; oracle C restatement of search_batch with "
-                         "AVX2-order kernels",
-               "single_thread_qps": round(n1 / one_s), "recall_at_10": round(cpu_recall, 4),
+                         f"on {cores} threads (the host's {os.cpu_count()} CPUs, of which this "
+                         f"process may use {len(os.sched_getaffinity(0))}); oracle C restatement "
+                         "of search_batch with AVX2-order kernels",
+               "host_cpus": os.cpu_count(), "cpus_allowed": len(os.sched_getaffinity(0)),
+               "process_cpu_s_per_pass": round(sorted(proc_cpu)[1], 3),
+               "single_thread_qps_per_cpu_s": round(n1 / one_c),
+               "single_thread_qps_wall": round(n1 / one_s),
+               "single_thread_sample": f"{n1} queries, median of 3 passes",
+               "recall_at_10": round(cpu_recall, 4),
                "loadavg_start_end": [round(load0, 2), round(load1, 2)]}
-        log(f"cpu baseline {ns / cpu_s:,.0f} qps on {threads} threads, "
-            f"{n1 / one_s:,.0f} qps on 1 thread, recall {cpu_recall:.4f}")
+        log(f"cpu baseline {ns / cpu_s:,.0f} qps on {cores} threads, "
+            f"{n1 / one_c:,.0f} qps per CPU-second on 1 thread, recall {cpu_recall:.4f}")
     except Exception as e:  # the baseline is reported, never required
         log(f"cpu baseline unavailable: {e}")
 
@@ -455,6 +504,41 @@ def main():
         log(f"device-assisted build unavailable: {e}")
 
     emit(args, locals())
+
+
+def run_brute(args, H, data):
+    """The exact brute force over the workload: BruteForceIndex::search_batch
+    of every query over every row, timed on the MFMA shortlist path (proven
+    exact per query, re-run by the streaming kernel otherwise) and on the
+    streaming kernel; both must return identical results.  Algorithmic work
+    2*Q*N*D flops."""
+    from hnsw_amd.device import synchronize
+    n, nq, dim, k = data.base_len(), data.query_len(), data.dim(), args.k
+    bf = H.BruteForceIndex(dim, H.Metric.L2)
+    bf.insert_batch(np.arange(n, dtype=np.uint64), data.base())
+    bf.search_arrays(data.queries()[:256], k)
+    times = []
+    for _ in range(max(args.steps, 1)):
+        synchronize()
+        t = time.perf_counter()
+        tk, td, tc = bf.search_arrays(data.queries(), k)
+        times.append(time.perf_counter() - t)
+    used, redo = bf.mfma_stats()
+    bf.set_mfma(0)
+    t = time.perf_counter()
+    ek, ed, ec = bf.search_arrays(data.queries(), k)
+    stream_s = time.perf_counter() - t
+    same = bool(np.array_equal(ek, tk) and np.array_equal(ed.view(np.uint32), td.view(np.uint32)))
+    best = min(times)
+    line = {"metric": "exact brute-force kNN, 2*Q*N*D TFLOP/s (whole call)", "unit": "TFLOP/s",
+            "value": round(2.0 * nq * n * dim / best / 1e12, 2), "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"BruteForceIndex::search_batch, {n} x {dim} L2, {nq} queries, "
+                                   f"k={k}"},
+            "mfma_call_s": round(best, 4), "streaming_call_s": round(stream_s, 4),
+            "mfma_queries": used, "rerun_exactly": redo, "identical_results": same,
+            "mfma_counters": measured_mfma()}
+    print(json.dumps(line), flush=True)
 
 
 def build_index(H, data, metric, m, threads):
@@ -555,9 +639,12 @@ def run_config(args, threads):
         idx.enable_quantization(on_device=True)
         quant_s = time.time() - t
         idx.enable_bf16()
+        t = time.time()
+        idx.enable_quantization_perdim()
+        perdim_s = time.time() - t
         batch = DeviceBatch(H, data.queries(), max(ks))
         rows = []
-        for prec in (H.Precision.F32, H.Precision.INT8, H.Precision.BF16):
+        for prec in (H.Precision.F32, H.Precision.INT8, H.Precision.INT8_PERDIM, H.Precision.BF16):
             for k in ks:
                 for ef in efs:
                     launch = uniform_search(idx, batch, k, ef, prec)
@@ -584,7 +671,8 @@ def run_config(args, threads):
                     log(f"{prec.name.lower()} k={k} ef={ef}: recall@{k}={rec:.4f} "
                         f"qps={q:,.0f} max |d - d_f64| {err:.3g}")
         out.update({"value": max(r["qps"] for r in rows if r["precision"] == "f32"),
-                    "dtype": "f32 / int8 / bf16", "results": rows,
+                    "dtype": "f32 / int8 (crate codes, per-dimension codes) / bf16",
+                    "results": rows, "perdim_quantizer_s": round(perdim_s, 3),
                     "config": {"workload": f"hnsw search_batch, {n} x {dim} "
                                            f"{'cosine' if metric == H.Metric.Cosine else 'L2'}, "
                                            f"M={m}, efC=200, {nq} queries",
@@ -706,6 +794,26 @@ def run_config(args, threads):
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "search_1m128_l2_traffic.md")
+MFMA_SUMMARY = os.path.join(ROOT, "profiles", "brute_mfma_1m128_counters.md")
+
+
+def measured_mfma():
+    """MFMA utilisation and flops of brute_mfma_kernel from the committed
+    rocprofv3 counter summary of this workload (keys mfma_util_percent,
+    mfma_tflops_counted), or None when it is absent."""
+    out = {}
+    try:
+        with open(MFMA_SUMMARY) as f:
+            for line in f:
+                for key in ("mfma_util_percent:", "mfma_flops_per_launch:", "mfma_tflops_counted:",
+                            "main_pass_ms:"):
+                    if line.startswith(key):
+                        out[key[:-1]] = float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        return None
+    if out:
+        out["source"] = os.path.relpath(MFMA_SUMMARY, ROOT)
+    return out or None
 
 
 def measured_traffic():
@@ -771,7 +879,8 @@ def emit(args, v):
         "ground_truth": {"brute_force_s": round(bf_s, 3), "f64_agreement": round(f64_agree, 5),
                          "mfma_queries": mfma_used, "mfma_rerun_exact": mfma_redo,
                          "streaming_kernel_s": round(stream_s, 3),
-                         "gemm_tflops_2QND": round(gemm_tflops, 2)},
+                         "gemm_tflops_2QND": round(gemm_tflops, 2),
+                         "mfma_counters": measured_mfma()},
         "graph_build_s": round(build_s, 1),
         "graph_build_threads": args.build_threads,
         "launch_shape": {"visited_set_capacity": cap.value, "waves_per_block": wpb.value},
