@@ -297,14 +297,43 @@ __device__ __forceinline__ float finish_metric(float sum, uint32_t metric) {
 // ------------------------------------------------------- visited set
 // Open-addressing hash of node ids in LDS, exact (no false positives or
 // negatives), any capacity (Lemire range reduction), linear probing.
-// Returns true when id was absent and is now inserted.
+// Returns true when id was absent and is now inserted.  The probe loop is
+// wave-uniform and branch-free apart from the masked atomic: per-lane
+// branches would cost several exec-mask instructions per probe.
 __device__ __forceinline__ bool lds_visit(uint32_t *table, uint32_t cap, uint32_t id) {
   uint32_t h = __umulhi(id * 0x9E3779B1u, cap);
-  for (;;) {
-    const uint32_t prev = atomicCAS(&table[h], kInvalid, id);
-    if (prev == kInvalid) return true;
-    if (prev == id) return false;
-    h = h + 1 == cap ? 0 : h + 1;
+  bool pend = id != kInvalid, fresh = false;
+  while (__ballot(pend)) {
+    uint32_t prev = id;
+    if (pend) prev = atomicCAS(&table[h], kInvalid, id);
+    const bool done = prev == kInvalid || prev == id;
+    fresh = fresh || (pend && prev == kInvalid);
+    const uint32_t nh = h + 1 == cap ? 0u : h + 1;
+    h = pend && !done ? nh : h;
+    pend = pend && !done;
+  }
+  return fresh;
+}
+
+// Two insertions per lane with both probes in flight (kInvalid: none).  The
+// first is applied first, so an id passed twice is new only once.
+__device__ __forceinline__ void lds_visit2(uint32_t *table, uint32_t cap, uint32_t a, uint32_t b,
+                                           bool &na, bool &nb) {
+  uint32_t ha = __umulhi(a * 0x9E3779B1u, cap), hb = __umulhi(b * 0x9E3779B1u, cap);
+  bool pa = a != kInvalid, pb = b != kInvalid;
+  na = nb = false;
+  while (__ballot(pa || pb)) {
+    uint32_t ra = a, rb = b;
+    if (pa) ra = atomicCAS(&table[ha], kInvalid, a);
+    if (pb) rb = atomicCAS(&table[hb], kInvalid, b);
+    const bool da = ra == kInvalid || ra == a, db = rb == kInvalid || rb == b;
+    na = na || (pa && ra == kInvalid);
+    nb = nb || (pb && rb == kInvalid);
+    const uint32_t nha = ha + 1 == cap ? 0u : ha + 1, nhb = hb + 1 == cap ? 0u : hb + 1;
+    ha = pa && !da ? nha : ha;
+    hb = pb && !db ? nhb : hb;
+    pa = pa && !da;
+    pb = pb && !db;
   }
 }
 

@@ -213,9 +213,12 @@ int tmax_for(const hh::RowLayout &lay) {
   return 0;
 }
 // LDS words per wavefront of the search kernel: the visited table, the
-// 64-entry id and distance staging, C (128 * sw candidate words) and W
-// (64 * sw result words), candidate words being 2 words each.
-uint32_t lds_words_for(uint32_t hash_cap, int sw) { return hash_cap + 128 + 384u * (uint32_t)sw; }
+// id and distance staging (st entries each: 64, or 128 for beam steps), C
+// (128 * sw candidate words) and W (64 * sw result words), candidate words
+// being 2 words each.
+uint32_t lds_words_for(uint32_t hash_cap, int sw, uint32_t st = 128) {
+  return hash_cap + 2 * st + 384u * (uint32_t)sw;
+}
 
 int slots_for(uint32_t ef) {
   if (ef <= 64) return 1;
@@ -508,6 +511,9 @@ struct hh_index {
   hipEvent_t t0 = nullptr, t1 = nullptr;
   // two-pass launches that start the longest queries first
   bool longest_first = true;
+  // candidates per layer-0 step of the bf16 and per-dimension int8
+  // traversals (hh_index_set_beam_width)
+  uint32_t beam_width = 4;
   // per ef bucket, the order hh_index_autotune measured faster: 0 = the
   // rule of thumb, 1 = one pass, 2 = two passes
   int tuned_order[4] = {0, 0, 0, 0};
@@ -618,11 +624,19 @@ int first_failure(const std::vector<int32_t> &status, uint64_t *first_bad) {
   return HH_OK;
 }
 
+// Candidates a beam step expands: the index's width, at most 4 and at most
+// 128 neighbour slots per step; lists longer than 64 slots keep the crate's
+// one-candidate rule.
+uint32_t beam_steps(const hh_index *idx) {
+  if (idx->beam_width <= 1 || idx->m0 == 0 || idx->m0 > 64) return 1;
+  return std::min<uint32_t>(std::min<uint32_t>(idx->beam_width, 4u), 128u / idx->m0);
+}
+
 // Launch shape of one search: kernel instance, visited-set capacity,
 // waves per block and grid size.
 struct SearchConfig {
   void *kern = nullptr;
-  uint32_t hash_cap = 0, lds_words = 0;
+  uint32_t hash_cap = 0, lds_words = 0, st_cap = 64;
   int wpb = 1;
   uint64_t blocks = 0, spill_words = 0;
 };
@@ -692,7 +706,10 @@ int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
   const int bucket = sw == 1 ? 0 : sw == 2 ? 1 : sw == 4 ? 2 : 3;
   cfg.hash_cap = idx->tuned_hash_cap[bucket] ? idx->tuned_hash_cap[bucket] : 2048u;
   cfg.hash_cap = std::max<uint32_t>(cfg.hash_cap, 256u * (uint32_t)sw);
-  cfg.lds_words = lds_words_for(cfg.hash_cap, sw);
+  // beam steps stage up to 128 new neighbours
+  cfg.st_cap = beam_steps(idx) > 1 && !c.build && (c.specs || c.precision == 2 || c.precision == 3)
+                   ? 128u : 64u;
+  cfg.lds_words = lds_words_for(cfg.hash_cap, sw, cfg.st_cap);
   cfg.kern = pick_search_kernel(idx->lay, sw, precision_class(c));
   // waves per block: the tuned value, else whichever of 1, 2, 4 keeps the
   // most waves resident
@@ -847,6 +864,8 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.hash_cap = cfg.hash_cap;
   a.hash_limit = cfg.hash_cap / 2 + cfg.hash_cap / 4;
   a.lds_words = cfg.lds_words;
+  a.st_cap = cfg.st_cap;
+  a.beam = cfg.st_cap == 128 ? beam_steps(idx) : 1u;
   a.seed = ws.seed;
   a.order = ws.order;
   // a pool smaller than the node count can drop a candidate: such queries
@@ -1714,6 +1733,24 @@ int hh_index_set_query_order(hh_index *idx, int longest_first) {
   int st = drain(idx);
   if (st) return st;
   idx->longest_first = longest_first != 0;
+  return HH_OK;
+}
+
+int hh_index_set_beam_width(hh_index *idx, uint32_t width) {
+  if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
+  if (width < 1 || width > 4) return fail(HH_INVALID_ARGUMENT, "beam width must be 1..4");
+  std::unique_lock<std::shared_mutex> wr(idx->mu);
+  if (int st = refuse_if_served(idx)) return st;
+  int st = drain(idx);
+  if (st) return st;
+  idx->beam_width = width;
+  idx->generation++;  // captured plans hold the old step rule
+  return HH_OK;
+}
+
+int hh_index_beam_steps(const hh_index *idx, uint32_t *steps) {
+  if (!idx || !steps) return fail(HH_INVALID_ARGUMENT, "null argument");
+  *steps = beam_steps(idx);
   return HH_OK;
 }
 

@@ -117,13 +117,15 @@ __device__ __forceinline__ uint32_t accept_counts(const uint64_t (&we)[SW], uint
 
 // W <- the `cap` smallest of W[0..size) (held in e[]) and the batch
 // elements in `amask`; returns the new worst (largest) word.
+// (The write phase, given every element's merged position: ep for W's,
+// ap for the batch's.)
 template <int WPL>
-__device__ __forceinline__ uint64_t w_merge(uint64_t *W, const uint64_t (&e)[WPL], uint32_t &size,
-                                            uint32_t cap, uint64_t amask, uint64_t akey) {
+__device__ __forceinline__ uint64_t w_write(uint64_t *W, const uint64_t (&e)[WPL],
+                                            const uint32_t (&ep)[WPL], uint32_t ap,
+                                            uint32_t &size, uint32_t cap, uint64_t amask,
+                                            uint64_t akey) {
   const int lane = lane_id();
   const bool mine = (amask >> lane) & 1;
-  uint32_t ep[WPL];
-  const uint32_t ap = merge_ranks<WPL>(e, ep, size, amask, akey);
   const uint32_t total = size + (uint32_t)popc64(amask);
   const uint32_t nsize = total < cap ? total : cap;
   uint64_t last = ~0ull;
@@ -141,6 +143,67 @@ __device__ __forceinline__ uint64_t w_merge(uint64_t *W, const uint64_t (&e)[WPL
   size = nsize;
   return last;
 }
+template <int WPL>
+__device__ __forceinline__ uint64_t w_merge(uint64_t *W, const uint64_t (&e)[WPL], uint32_t &size,
+                                            uint32_t cap, uint64_t amask, uint64_t akey) {
+  uint32_t ep[WPL];
+  const uint32_t ap = merge_ranks<WPL>(e, ep, size, amask, akey);
+  return w_write<WPL>(W, e, ep, ap, size, cap, amask, akey);
+}
+
+// One of four wave-uniform values, chosen per lane by j (explicit selects:
+// a lane-indexed private array would live in scratch memory).
+__device__ __forceinline__ uint32_t sel4(uint32_t j, uint32_t a, uint32_t b, uint32_t c,
+                                         uint32_t d) {
+  return j == 0 ? a : j == 1 ? b : j == 2 ? c : d;
+}
+
+// Adjacency prefetch of a beam step: the candidates at C's front after a
+// merge are the next step's pops (unless the pool refills C first), so their
+// layer-0 lists are loaded as soon as the merge ranks are known, in the
+// step's slot layout (slot s = lane + 64 g holds list s / m0, position
+// s % m0).  A hint only: the next step checks each list's id.
+struct FrontFetch {
+  const uint32_t *adj0;
+  uint32_t m0, n;        // n: front positions captured (the beam)
+  uint32_t sj0, sp0, sj1, sp1;
+  uint32_t id0, id1, id2, id3;
+  uint32_t v0, v1;
+};
+
+__device__ __forceinline__ void front_fetch(FrontFetch &f) {
+  gptr<uint32_t> adj = G(f.adj0);
+  const uint32_t a = sel4(f.sj0, f.id0, f.id1, f.id2, f.id3);
+  const uint32_t b = sel4(f.sj1, f.id0, f.id1, f.id2, f.id3);
+  f.v0 = f.sj0 < f.n && a != kInvalid ? adj[(size_t)a * f.m0 + f.sp0] : kInvalid;
+  f.v1 = f.sj1 < f.n && b != kInvalid ? adj[(size_t)b * f.m0 + f.sp1] : kInvalid;
+}
+
+// The merged order's first f.n entries, given C's first chunk (e0, its
+// merged positions ep0, size entries) and the batch (amask, words akey at
+// positions ap), and their lists fetched: list entries at positions below 4
+// come from the first chunk, since positions never fall below indices.
+__device__ __forceinline__ void front_capture(FrontFetch &f, uint64_t e0, uint32_t ep0, uint32_t ap,
+                                              uint64_t amask, uint64_t akey, uint32_t size) {
+  const int lane = lane_id();
+  const bool mine = (amask >> lane) & 1;
+  uint32_t id[4];
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    id[p] = kInvalid;
+    if (p < (int)f.n) {
+      const uint64_t la = __ballot(mine && ap == (uint32_t)p);
+      const uint64_t lb = __ballot(lane < (int)size && ep0 == (uint32_t)p);
+      if (la) id[p] = rl((uint32_t)akey, __ffsll((unsigned long long)la) - 1);
+      else if (lb) id[p] = rl((uint32_t)e0, __ffsll((unsigned long long)lb) - 1);
+    }
+  }
+  f.id0 = id[0];
+  f.id1 = id[1];
+  f.id2 = id[2];
+  f.id3 = id[3];
+  front_fetch(f);
+}
 
 // C[head..head+size) (held in e[]) merged with the batch elements in
 // `amask` into C[0..): the merged order's first `keep` entries stay, keep =
@@ -149,16 +212,16 @@ __device__ __forceinline__ uint64_t w_merge(uint64_t *W, const uint64_t (&e)[WPL
 // bound of the pool.  With results full, dead entries are also trimmed from
 // C's end.
 template <int CPL>
-__device__ __forceinline__ void c_merge(uint64_t *C, const uint64_t (&e)[CPL], uint32_t &head,
+__device__ __forceinline__ void c_write(uint64_t *C, const uint64_t (&e)[CPL],
+                                        const uint32_t (&ep)[CPL], uint32_t ap, uint32_t &head,
                                         uint32_t &size, uint32_t ccap, uint64_t amask,
                                         uint64_t akey, bool wfull, float worst,
                                         gmut<uint64_t> pool, bool has_pool, uint32_t pool_cap,
                                         uint32_t &pn, uint64_t &pmin, unsigned long long &ovf,
-                                        unsigned long long &moved) {
+                                        unsigned long long &moved, FrontFetch *ff = nullptr) {
   const int lane = lane_id();
   const bool mine = (amask >> lane) & 1;
-  uint32_t ep[CPL];
-  const uint32_t ap = merge_ranks<CPL>(e, ep, size, amask, akey);
+  if (ff) front_capture(*ff, e[0], ep[0], ap, amask, akey, size);
   // every entry of C is below pmin; batch entries at or above it go out
   const uint32_t below = size + (uint32_t)popc64(__ballot(mine && akey < pmin));
   const uint32_t keep = below < ccap ? below : ccap;
@@ -209,6 +272,85 @@ __device__ __forceinline__ void c_merge(uint64_t *C, const uint64_t (&e)[CPL], u
   head = 0;
   size = wfull ? live : keep;  // dead entries form C's tail: trimmed
 }
+template <int CPL>
+__device__ __forceinline__ void c_merge(uint64_t *C, const uint64_t (&e)[CPL], uint32_t &head,
+                                        uint32_t &size, uint32_t ccap, uint64_t amask,
+                                        uint64_t akey, bool wfull, float worst,
+                                        gmut<uint64_t> pool, bool has_pool, uint32_t pool_cap,
+                                        uint32_t &pn, uint64_t &pmin, unsigned long long &ovf,
+                                        unsigned long long &moved, FrontFetch *ff = nullptr) {
+  uint32_t ep[CPL];
+  const uint32_t ap = merge_ranks<CPL>(e, ep, size, amask, akey);
+  c_write<CPL>(C, e, ep, ap, head, size, ccap, amask, akey, wfull, worst, pool, has_pool,
+               pool_cap, pn, pmin, ovf, moved, ff);
+}
+
+// Ranks of one batch against C and W in a single pass over the batch (the
+// beam steps): batch elements `amask` (words akey) go into C, the admitted
+// ones `am` also into W.  Per element one broadcast; its rank in each list
+// is counted with ballots and kept by its lane.  NC: the chunks of C
+// that hold entries.
+template <int CPL, int NC, int WPL>
+__device__ __forceinline__ void joint_ranks_n(const uint64_t (&ce)[CPL], uint32_t (&cp)[CPL],
+                                              uint32_t csize, const uint64_t (&we)[WPL],
+                                              uint32_t (&wp)[WPL], uint32_t wsize, uint64_t amask,
+                                              uint64_t am, uint64_t akey, uint32_t &apc,
+                                              uint32_t &apw) {
+  const int lane = lane_id();
+  uint64_t vc[NC], vw[WPL];
+#pragma unroll
+  for (int c = 0; c < NC; c++) vc[c] = __ballot(c * 64 + lane < (int)csize);
+#pragma unroll
+  for (int c = 0; c < WPL; c++) vw[c] = __ballot(c * 64 + lane < (int)wsize);
+#pragma unroll
+  for (int c = 0; c < CPL; c++) cp[c] = c * 64 + lane;
+#pragma unroll
+  for (int c = 0; c < WPL; c++) wp[c] = c * 64 + lane;
+  uint32_t bc = 0, bw = 0;       // batch elements (admitted ones) ordering before mine
+  uint32_t rc = 0, rw = 0;       // my rank in C, in W (written by my element's step)
+  for (uint64_t m = amask; m; m &= m - 1) {
+    const int j = __ffsll((unsigned long long)m) - 1;
+    const uint64_t kj = rl64(akey, j);
+    const bool before = kj < akey;
+    uint32_t r = 0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const bool lt = ce[c] < kj;
+      r += (uint32_t)popc64(__ballot(lt) & vc[c]);
+      cp[c] += lt ? 0u : 1u;
+    }
+    bc += before ? 1u : 0u;
+    rc = lane == j ? r : rc;
+    if ((am >> j) & 1) {
+      uint32_t r2 = 0;
+#pragma unroll
+      for (int c = 0; c < WPL; c++) {
+        const bool lt = we[c] < kj;
+        r2 += (uint32_t)popc64(__ballot(lt) & vw[c]);
+        wp[c] += lt ? 0u : 1u;
+      }
+      bw += before ? 1u : 0u;
+      rw = lane == j ? r2 : rw;
+    }
+  }
+  apc = rc + bc;
+  apw = rw + bw;
+}
+template <int CPL, int WPL>
+__device__ __forceinline__ void joint_ranks(const uint64_t (&ce)[CPL], uint32_t (&cp)[CPL],
+                                            uint32_t csize, const uint64_t (&we)[WPL],
+                                            uint32_t (&wp)[WPL], uint32_t wsize, uint64_t amask,
+                                            uint64_t am, uint64_t akey, uint32_t &apc,
+                                            uint32_t &apw) {
+  if (csize <= 64) return joint_ranks_n<CPL, 1, WPL>(ce, cp, csize, we, wp, wsize, amask, am, akey, apc, apw);
+  if constexpr (CPL > 2) {
+    if (csize <= 128) return joint_ranks_n<CPL, 2, WPL>(ce, cp, csize, we, wp, wsize, amask, am, akey, apc, apw);
+  }
+  if constexpr (CPL > 4) {
+    if (csize <= 256) return joint_ranks_n<CPL, 4, WPL>(ce, cp, csize, we, wp, wsize, amask, am, akey, apc, apw);
+  }
+  return joint_ranks_n<CPL, CPL, WPL>(ce, cp, csize, we, wp, wsize, amask, am, akey, apc, apw);
+}
 
 // Loads list entries [base, base + size) into registers (element i in lane
 // i % 64, chunk i / 64; ~0 past the end).
@@ -228,13 +370,12 @@ __device__ __forceinline__ void list_load(const uint64_t *L, uint32_t size, uint
 template <int TMAX>
 constexpr int rows_per_quarter() { return TMAX <= 4 ? 4 : (TMAX <= 8 ? 2 : 1); }
 
-template <int TMAX, bool EXTRA, bool EXACT, class E>
+template <int TMAX, bool EXTRA, bool EXACT, class E, int B>
 __device__ __forceinline__ void gather_round(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
                                              const E *rows, const uint32_t *ids, int cnt,
                                              int base, float *out) {
   const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
   const Shape s{P.T, P.U, P.R, P.stride};
-  constexpr int B = rows_per_quarter<TMAX>();
   // groups of four rows (one per quarter) this round needs: a group wholly
   // past cnt is skipped (wave-uniform), so at most three slots are wasted
   const int groups = (cnt - base + 3) >> 2;
@@ -269,16 +410,69 @@ __device__ __forceinline__ void gather_round(const SearchArgs &P, const QLanes<T
   }
 }
 
+// bf16 rows of an exact layout (D = 32 * TMAX): B rows per quarter per
+// round held as the loaded words (four per row at D = 128) and widened only
+// as each product is formed, in the order of quarter_acc.
+template <int TMAX, int B>
+__device__ __forceinline__ void gather_round_bf16x(const SearchArgs &P, const QLanes<TMAX, false> &q,
+                                                   const uint16_t *rows, const uint32_t *ids,
+                                                   int cnt, int base, float *out) {
+  const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
+  constexpr int NB = TMAX / 4;
+  const int groups = (cnt - base + 3) >> 2;
+  uint2 ra[B][NB], rc[B][NB];
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const int row = base + 4 * b + quarter;
+    const uint32_t rid = ids[row < cnt ? row : cnt - 1];
+    gptr<uint16_t> p = G(rows) + (size_t)rid * P.stride;
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      if (b == 0 || b < groups) {
+        ra[b][k] = *(gptr<uint2>)(p + k * 128 + 4 * i);
+        rc[b][k] = *(gptr<uint2>)(p + k * 128 + 64 + 4 * i);
+      } else {
+        ra[b][k] = rc[b][k] = make_uint2(0u, 0u);
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    if (b > 0 && b >= groups) break;
+    const int row = base + 4 * b + quarter;
+    float lo = 0.0f, hi = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      const uint32_t wa[2] = {ra[b][k].x, ra[b][k].y}, wc[2] = {rc[b][k].x, rc[b][k].y};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const float xl = __uint_as_float(j & 1 ? wa[j >> 1] & 0xFFFF0000u : wa[j >> 1] << 16);
+        const float xh = __uint_as_float(j & 1 ? wc[j >> 1] & 0xFFFF0000u : wc[j >> 1] << 16);
+        if (P.metric == 0) {
+          const float dl = q.lo[4 * k + j] - xl, dh = q.hi[4 * k + j] - xh;
+          lo = __builtin_fmaf(dl, dl, lo);
+          hi = __builtin_fmaf(dh, dh, hi);
+        } else {
+          lo = __builtin_fmaf(q.lo[4 * k + j], xl, lo);
+          hi = __builtin_fmaf(q.hi[4 * k + j], xh, hi);
+        }
+      }
+    }
+    const float h = finish_metric(quarter_reduce<false>(lo, hi, 0.0f, 0, quarter), P.metric);
+    if (i == 0 && row < cnt) out[row] = h;
+  }
+}
+
 // The first round is straight-line code, so its loads issue right after
 // the caller's (no loop-header wait for earlier loads in flight).
-template <int TMAX, bool EXTRA, bool EXACT, class E>
+template <int TMAX, bool EXTRA, bool EXACT, class E, int B = rows_per_quarter<TMAX>()>
 __device__ __forceinline__ void gather_rows(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
                                             const E *rows, const uint32_t *ids, int cnt,
                                             float *out) {
-  constexpr int R = 4 * rows_per_quarter<TMAX>();
-  gather_round<TMAX, EXTRA, EXACT, E>(P, q, rows, ids, cnt, 0, out);
+  constexpr int R = 4 * B;
+  gather_round<TMAX, EXTRA, EXACT, E, B>(P, q, rows, ids, cnt, 0, out);
   for (int base = R; base < cnt; base += R)
-    gather_round<TMAX, EXTRA, EXACT, E>(P, q, rows, ids, cnt, base, out);
+    gather_round<TMAX, EXTRA, EXACT, E, B>(P, q, rows, ids, cnt, base, out);
 }
 
 // 128-byte code chunks a lane group holds: cstride / 128 for the exact
@@ -298,11 +492,14 @@ struct QCodeInfo {
   float sigma, qconst;
 };
 
-// Code rows a quarter gathers per round: in the code instances 8 (32 per
-// round) while the codes take at most 2 x 128 bytes per row, 4 beyond; in
-// the general instance 4 (its registers are shared with the f32 path).
+// Code rows a quarter gathers per round: in the code instances 16 (64 per
+// round, a beam step's usual batch) for rows of one 128-byte chunk, 8 for
+// two, 4 beyond; in the general instance 4 (its registers are shared with
+// the f32 path).
 template <int CC, int PC>
-constexpr int code_rows_per_quarter() { return PC == kPrecCodes && CC <= 2 ? 8 : 4; }
+constexpr int code_rows_per_quarter() {
+  return PC == kPrecCodes ? (CC <= 2 ? 8 : 4) : 4;
+}
 
 // Integer-code distances of rows ids[0..cnt): prec 1 the crate's codes
 // (exact sums, src/kernels/int8.rs:13-48, rescaled as
@@ -379,6 +576,77 @@ __device__ __forceinline__ void gather_codes(const SearchArgs &P, const uint2 (&
   }
 }
 
+// Integer-code distances of 128-byte code rows (dim <= 128) in the code
+// instances: four lanes per row, 32 bytes each (two 16-byte loads), so one
+// load instruction covers sixteen rows and B of them 16 * B rows per round;
+// a row's exact sum is its four lanes' sums (two quad-permute adds).  The
+// query code is held in the same layout: qv = bytes [32 t, 32 t + 32) for
+// lane t of each quad.  Same sums and rescaling as gather_codes.
+template <int B>
+__device__ __forceinline__ void gather_codes4(const SearchArgs &P, const uint4 (&qv)[2],
+                                              const QCodeInfo &qs, int prec, const uint32_t *ids,
+                                              int cnt, float *out) {
+  const int lane = lane_id(), t = lane & 3, r = lane >> 2;
+  const bool pd = prec == 3;
+  const int8_t *codes = pd ? P.pd_codes : P.codes;
+  const bool l2 = P.metric == 0;
+  const float s2 = __fmul_rn(P.sq_scale, P.sq_scale);
+  for (int base = 0; base < cnt; base += 16 * B) {
+    uint4 ca[B], cb[B];
+    float rn[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const int row = base + 16 * b + r;
+      const uint32_t id = ids[row < cnt ? row : cnt - 1];
+      gptr<uint4> cr = (gptr<uint4>)(codes + (size_t)id * 128 + 32 * t);
+      const bool need = b == 0 || base + 16 * b < cnt;
+      ca[b] = need ? (uint4)cr[0] : make_uint4(0u, 0u, 0u, 0u);
+      cb[b] = need ? (uint4)cr[1] : make_uint4(0u, 0u, 0u, 0u);
+      rn[b] = need && pd && l2 ? G(P.pd_rnorm)[id] : 0.0f;
+    }
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      if (b > 0 && base + 16 * b >= cnt) break;
+      const int row = base + 16 * b + r;
+      const int w[8] = {(int)ca[b].x, (int)ca[b].y, (int)ca[b].z, (int)ca[b].w,
+                        (int)cb[b].x, (int)cb[b].y, (int)cb[b].z, (int)cb[b].w};
+      const int qw[8] = {(int)qv[0].x, (int)qv[0].y, (int)qv[0].z, (int)qv[0].w,
+                         (int)qv[1].x, (int)qv[1].y, (int)qv[1].z, (int)qv[1].w};
+      int acc = 0;
+      if (!pd && l2) {
+        int ab = 0, bb = 0;
+#pragma unroll
+        for (int x = 0; x < 8; x++) {
+          ab = __builtin_amdgcn_sdot4(qw[x], w[x], ab, false);
+          bb = __builtin_amdgcn_sdot4(w[x], w[x], bb, false);
+        }
+        acc = bb - 2 * ab;
+      } else {
+#pragma unroll
+        for (int x = 0; x < 8; x++) acc = __builtin_amdgcn_sdot4(qw[x], w[x], acc, false);
+      }
+      acc += dppi<kXor1>(acc);
+      acc += dppi<kXor2>(acc);
+      if (t == 0 && row < cnt) {
+        float v;
+        if (pd) {
+          const float tt = __fmul_rn(qs.sigma, __int2float_rn(acc));
+          if (l2) v = __fsub_rn(__fadd_rn(qs.qconst, rn[b]), __fadd_rn(tt, tt));
+          else if (P.metric == 1) v = -__fadd_rn(qs.qconst, tt);
+          else v = __fsub_rn(1.0f, __fadd_rn(qs.qconst, tt));
+        } else if (l2) {
+          v = __fmul_rn(__uint2float_rn((uint32_t)(acc + qs.qaa)), s2);
+        } else if (P.metric == 1) {
+          v = __fmul_rn(-__int2float_rn(acc), s2);
+        } else {
+          v = __fsub_rn(1.0f, __fmul_rn(__int2float_rn(acc), s2));
+        }
+        out[row] = v;
+      }
+    }
+  }
+}
+
 // Distances from the query to rows ids[0..cnt) (ids in LDS), written to
 // out[0..cnt) (LDS), with traversal precision `prec` (0 exact f32, 1 the
 // crate's int8 codes, 2 bf16 rows, 3 per-dimension int8 codes).  All 64
@@ -386,11 +654,27 @@ __device__ __forceinline__ void gather_codes(const SearchArgs &P, const uint2 (&
 template <int TMAX, bool EXTRA, bool EXACT, int PC>
 __device__ __forceinline__ void gather_dists(const SearchArgs &P, const QLanes<TMAX, EXTRA> &q,
                                              const uint2 (&qc)[code_chunks<TMAX, EXACT>()],
-                                             const QCodeInfo &qs, int prec, const uint32_t *ids,
-                                             int cnt, float *out) {
+                                             const uint4 (&qv)[2], const QCodeInfo &qs, int prec,
+                                             const uint32_t *ids, int cnt, float *out) {
   if (prec == 1 || prec == 3) {
+    if constexpr (PC == kPrecCodes && TMAX <= 4) {
+      if (P.cstride == 128) {
+        gather_codes4<2>(P, qv, qs, prec, ids, cnt, out);
+        return;
+      }
+    }
     gather_codes<code_chunks<TMAX, EXACT>(), PC>(P, qc, qs, prec, ids, cnt, out, EXACT);
     return;
+  }
+  // the bf16 instance takes twice the rows per round at D <= 128 (32: the
+  // half-size rows of a beam step in one round trip)
+  if constexpr (PC == kPrecBf16 && EXACT && TMAX <= 4) {
+    if (prec == 2) {
+      gather_round_bf16x<TMAX, 8>(P, q, P.bf16, ids, cnt, 0, out);
+      for (int base = 32; base < cnt; base += 32)
+        gather_round_bf16x<TMAX, 8>(P, q, P.bf16, ids, cnt, base, out);
+      return;
+    }
   }
   if (prec == 2) gather_rows<TMAX, EXTRA, EXACT, uint16_t>(P, q, P.bf16, ids, cnt, out);
   else gather_rows<TMAX, EXTRA, EXACT, float>(P, q, P.rows, ids, cnt, out);
@@ -414,7 +698,7 @@ __device__ __forceinline__ void seed_query(const SearchArgs &P, uint32_t qi, uin
 // uniform bf16 or code batch runs the instance of its precision, so its
 // loop carries no code or registers for the others.
 template <int TMAX, bool EXTRA, bool EXACT, int SW, int PC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? 4 : 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? (PC == kPrecAny ? 4 : 3) : 2)))
 hnsw_search_kernel(const SearchArgs P) {
   extern __shared__ uint32_t lds[];
   const int lane = lane_id(), i16 = lane & 15;
@@ -422,8 +706,8 @@ hnsw_search_kernel(const SearchArgs P) {
   const uint32_t slot = blockIdx.x * (blockDim.x >> 6) + wave;
   uint32_t *table = lds + (size_t)wave * P.lds_words;
   uint32_t *st_id = table + P.hash_cap;
-  float *st_d = reinterpret_cast<float *>(st_id + 64);
-  uint64_t *Ck = reinterpret_cast<uint64_t *>(st_d + 64);  // 128 * SW candidate words
+  float *st_d = reinterpret_cast<float *>(st_id + P.st_cap);
+  uint64_t *Ck = reinterpret_cast<uint64_t *>(st_d + P.st_cap);  // 128 * SW candidate words
   uint64_t *Wk = Ck + 128 * SW;                              // 64 * SW result words
   uint32_t *spill = P.spill ? P.spill + (size_t)slot * P.spill_words : nullptr;
   gmut<uint32_t> slog = GM(P.spill_log) + (size_t)slot * P.spill_log_cap;
@@ -503,21 +787,42 @@ hnsw_search_kernel(const SearchArgs P) {
 #pragma unroll
     for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++) qc[c] = make_uint2(0, 0);
     QCodeInfo qs{0, 1.0f, 0.0f};
+    // the code instances hold 128-byte query codes four lanes per row
+    uint4 qv[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
     if (prec == 1 || prec == 3) {
-      gptr<uint2> cq = (gptr<uint2>)(P.qcode + (size_t)qi * P.cstride);
-      const int chunks = (int)(P.cstride >> 7);
+      bool quad = false;
+      if constexpr (PC == kPrecCodes && TMAX <= 4) quad = P.cstride == 128;
+      if (quad) {
+        gptr<uint4> cq = (gptr<uint4>)(P.qcode + (size_t)qi * 128 + 32 * (lane & 3));
+        qv[0] = cq[0];
+        qv[1] = cq[1];
+        if (prec == 1 && P.metric == 0) {
+          const int qw[8] = {(int)qv[0].x, (int)qv[0].y, (int)qv[0].z, (int)qv[0].w,
+                             (int)qv[1].x, (int)qv[1].y, (int)qv[1].z, (int)qv[1].w};
+          int aa = 0;
 #pragma unroll
-      for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++)
-        if (c < chunks) qc[c] = cq[c * 16 + i16];
-      if (prec == 1 && P.metric == 0) {
-        int aa = 0;
-#pragma unroll
-        for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++) {
-          aa = __builtin_amdgcn_sdot4((int)qc[c].x, (int)qc[c].x, aa, false);
-          aa = __builtin_amdgcn_sdot4((int)qc[c].y, (int)qc[c].y, aa, false);
+          for (int x = 0; x < 8; x++) aa = __builtin_amdgcn_sdot4(qw[x], qw[x], aa, false);
+          aa += dppi<kXor1>(aa);
+          aa += dppi<kXor2>(aa);
+          qs.qaa = aa;
         }
-        qs.qaa = quarter_sum_i(aa);
-      } else if (prec == 3) {
+      } else {
+        gptr<uint2> cq = (gptr<uint2>)(P.qcode + (size_t)qi * P.cstride);
+        const int chunks = (int)(P.cstride >> 7);
+#pragma unroll
+        for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++)
+          if (c < chunks) qc[c] = cq[c * 16 + i16];
+        if (prec == 1 && P.metric == 0) {
+          int aa = 0;
+#pragma unroll
+          for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++) {
+            aa = __builtin_amdgcn_sdot4((int)qc[c].x, (int)qc[c].x, aa, false);
+            aa = __builtin_amdgcn_sdot4((int)qc[c].y, (int)qc[c].y, aa, false);
+          }
+          qs.qaa = quarter_sum_i(aa);
+        }
+      }
+      if (prec == 3) {
         const float2 v = G(P.qpd)[qi];
         qs.sigma = __builtin_bit_cast(float, rl(__float_as_uint(v.x), 0));
         qs.qconst = __builtin_bit_cast(float, rl(__float_as_uint(v.y), 0));
@@ -544,7 +849,7 @@ hnsw_search_kernel(const SearchArgs P) {
       uint64_t cur_off = P.top >= 1 ? (uint64_t)G(P.upper_off)[cur] : 0;
       st_id[0] = cur;
       wave_sync();
-      gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qs, (int)prec, st_id, 1, st_d);
+      gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qv, qs, (int)prec, st_id, 1, st_d);
       wave_sync();
       best = rlf(st_d[0], 0);
       st.up++;
@@ -564,7 +869,7 @@ hnsw_search_kernel(const SearchArgs P) {
             const uint64_t noff = lane < c ? (uint64_t)G(P.upper_off)[nid] : 0;
             if (lane < c) st_id[lane] = nid;
             wave_sync();
-            gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qs, (int)prec, st_id, c, st_d);
+            gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qv, qs, (int)prec, st_id, c, st_d);
             wave_sync();
             const float dv = lane < c ? st_d[lane] : 0.0f;
             // first list position holding the IEEE-smallest distance
@@ -627,28 +932,146 @@ hnsw_search_kernel(const SearchArgs P) {
     wave_sync();
     const bool pf_ok = P.m0 <= 64;
     uint32_t pf_id = kInvalid, pf_adj = kInvalid, pf2_id = kInvalid, pf2_adj = kInvalid;
+    // refill C from the pool in chunks of 64, each merged like a neighbour
+    // batch; what does not fit goes back to the pool, which is compacted in
+    // place (a chunk writes at most as many entries as it read)
+    auto refill = [&]() {
+      const uint32_t total = pn;
+      const bool wfull = wsize >= ef;
+      pn = 0;
+      pmin = ~0ull;
+      for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t c = total - base < 64 ? total - base : 64;
+        const uint64_t x = lane < (int)c ? (uint64_t)pool[base + lane] : ~0ull;
+        const uint64_t live = __ballot(lane < (int)c && !(wfull && ckey_d(x) > worst));
+        uint64_t ce[2 * SW];
+        list_load<2 * SW>(Ck + head, csize, ce);
+        c_merge<2 * SW>(Ck, ce, head, csize, ccap, live, x, wfull, worst, pool, has_pool,
+                        P.cpool_cap, pn, pmin, q_ovf, q_cspill);
+      }
+    };
+    // Moves the LDS visited table into the wave's HBM bitset (the query
+    // continues there, logging what it marks).
+    auto leave_lds = [&]() {
+      for (uint32_t t = lane; t < hcap; t += 64) {
+        const uint32_t v = table[t];
+        if (v != kInvalid)
+          __hip_atomic_fetch_or(spill + (v >> 5), 1u << (v & 31), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __threadfence();
+      mode = 1;
+      q_spill++;
+    };
 
+    if ((prec == 2 || prec == 3) && P.beam > 1) {
+      // ---- beam steps (bf16 and per-dimension int8 traversals)
+      const uint32_t m0 = P.m0, beam = P.beam;
+      FrontFetch ff;
+      ff.adj0 = P.adj0;
+      ff.m0 = m0;
+      ff.n = beam;
+      ff.sj0 = (uint32_t)lane / m0;
+      ff.sp0 = (uint32_t)lane - ff.sj0 * m0;
+      ff.sj1 = (uint32_t)(lane + 64) / m0;
+      ff.sp1 = (uint32_t)(lane + 64) - ff.sj1 * m0;
+      ff.id0 = ff.id1 = ff.id2 = ff.id3 = kInvalid;
+      ff.v0 = ff.v1 = kInvalid;
+      for (;;) {
+        if (csize < beam && pn > 0) refill();
+        if (csize == 0) break;
+        // pop up to `beam` candidates while the stop rule does not hold
+        const uint32_t avail = csize < beam ? csize : beam;
+        const uint64_t fe = lane < (int)avail ? Ck[head + lane] : ~0ull;
+        const uint64_t stop =
+            __ballot(lane < (int)avail && wsize >= ef && ckey_d(fe) > worst);
+        const uint32_t ne = stop ? (uint32_t)(__ffsll((unsigned long long)stop) - 1) : avail;
+        if (ne == 0) break;
+        const uint32_t x0 = rl((uint32_t)fe, 0), x1 = rl((uint32_t)fe, 1);
+        const uint32_t x2 = rl((uint32_t)fe, 2), x3 = rl((uint32_t)fe, 3);
+        head += ne;
+        csize -= ne;
+        q_exp += ne;
+        // the table never fills: leave it before a step could pass its limit
+        if (mode == 0 && spill && vcount + ne * m0 > P.hash_limit) leave_lds();
+        // the popped lists in slot layout, from the prefetch where it matches
+        const uint32_t c0 = ff.sj0 < ne ? sel4(ff.sj0, x0, x1, x2, x3) : kInvalid;
+        const uint32_t c1 = ff.sj1 < ne ? sel4(ff.sj1, x0, x1, x2, x3) : kInvalid;
+        uint32_t nid0 = kInvalid, nid1 = kInvalid;
+        if (c0 != kInvalid)
+          nid0 = c0 == sel4(ff.sj0, ff.id0, ff.id1, ff.id2, ff.id3) ? ff.v0
+                                                                    : adj0[(size_t)c0 * m0 + ff.sp0];
+        if (c1 != kInvalid)
+          nid1 = c1 == sel4(ff.sj1, ff.id0, ff.id1, ff.id2, ff.id3) ? ff.v1
+                                                                    : adj0[(size_t)c1 * m0 + ff.sp1];
+        bool nw0 = false, nw1 = false;
+        if (mode == 0) {
+          lds_visit2(table, hcap, nid0, nid1, nw0, nw1);
+        } else {
+          if (nid0 != kInvalid) nw0 = bit_visit(spill, nid0);
+          if (nid1 != kInvalid) nw1 = bit_visit(spill, nid1);
+        }
+        const uint64_t nm0 = __ballot(nw0), nm1 = __ballot(nw1);
+        const uint32_t n0 = (uint32_t)popc64(nm0), nn = n0 + (uint32_t)popc64(nm1);
+        if (mode == 1 && nn > 0) {
+          if (slog_n + nn <= P.spill_log_cap) {
+            if (nw0) slog[slog_n + mbcnt(nm0)] = nid0;
+            if (nw1) slog[slog_n + n0 + mbcnt(nm1)] = nid1;
+          }
+          slog_n += nn;
+        }
+        vcount += nn;
+        q_dist += nn;
+        bool fetched = false;
+        if (nn > 0) {
+          if (nw0) st_id[mbcnt(nm0)] = nid0;
+          if (nw1) st_id[n0 + mbcnt(nm1)] = nid1;
+          wave_sync();
+          gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qv, qs, (int)prec, st_id, (int)nn, st_d);
+          wave_sync();
+          // every new neighbour against the results at the step's start
+          const bool wfull0 = wsize >= ef;
+          const float worst0 = worst;
+          for (uint32_t base = 0; base < nn; base += 64) {
+            const uint32_t i = base + (uint32_t)lane;
+            const uint32_t myid = i < nn ? st_id[i] : kInvalid;
+            const float myd = i < nn ? st_d[i] : 0.0f;
+            const bool surv = i < nn && (!wfull0 || myd < worst0);
+            const uint64_t amask = __ballot(surv);
+            if (!amask) continue;
+            const uint64_t am = __ballot(surv && admit(P, allow, myid));
+            const uint64_t akey = surv ? ckey(myd, myid) : ~0ull;
+            // one pass over the survivors ranks them in C and W; W is written
+            // first so C drops what the new worst makes dead
+            uint64_t ce[2 * SW], we[SW];
+            list_load<2 * SW>(Ck + head, csize, ce);
+            list_load<SW>(Wk, wsize, we);
+            uint32_t cp[2 * SW], wp[SW], apc, apw;
+            joint_ranks<2 * SW, SW>(ce, cp, csize, we, wp, wsize, amask, am, akey, apc, apw);
+            // C's new front is known: its lists load during the writes
+            front_capture(ff, ce[0], cp[0], apc, amask, akey, csize);
+            fetched = true;
+            if (am) worst = ckey_d(w_write<SW>(Wk, we, wp, apw, wsize, ef, am, akey));
+            c_write<2 * SW>(Ck, ce, cp, apc, head, csize, ccap, amask, akey, wsize >= ef, worst,
+                            pool, has_pool, P.cpool_cap, pn, pmin, q_ovf, q_cspill);
+          }
+        }
+        if (!fetched) {
+          // C's front is unchanged: fetch its lists for the next step
+          const uint64_t nf = lane < (int)beam && (uint32_t)lane < csize ? Ck[head + lane] : ~0ull;
+          ff.id0 = (uint32_t)rl((uint32_t)nf, 0);
+          ff.id1 = (uint32_t)rl((uint32_t)nf, 1);
+          ff.id2 = (uint32_t)rl((uint32_t)nf, 2);
+          ff.id3 = (uint32_t)rl((uint32_t)nf, 3);
+          front_fetch(ff);
+        }
+      }
+    } else
     for (;;) {
       if (csize == 0) {
         if (pn == 0) break;
-        // refill C from the pool in chunks of 64, each merged like a
-        // neighbour batch; what does not fit goes back to the pool, which
-        // is compacted in place (a chunk writes at most as many entries as
-        // it read)
-        const uint32_t total = pn;
-        const bool wfull = wsize >= ef;
-        pn = 0;
-        pmin = ~0ull;
         head = 0;
-        for (uint32_t base = 0; base < total; base += 64) {
-          const uint32_t c = total - base < 64 ? total - base : 64;
-          const uint64_t x = lane < (int)c ? (uint64_t)pool[base + lane] : ~0ull;
-          const uint64_t live = __ballot(lane < (int)c && !(wfull && ckey_d(x) > worst));
-          uint64_t ce[2 * SW];
-          list_load<2 * SW>(Ck + head, csize, ce);
-          c_merge<2 * SW>(Ck, ce, head, csize, ccap, live, x, wfull, worst, pool, has_pool,
-                          P.cpool_cap, pn, pmin, q_ovf, q_cspill);
-        }
+        refill();
         if (csize == 0) break;
       }
       const uint64_t top = Ck[head];
@@ -692,7 +1115,7 @@ hnsw_search_kernel(const SearchArgs P) {
         wave_sync();
         const uint32_t myid = lane < nn ? st_id[lane] : kInvalid;
         const bool adm = lane < nn && admit(P, allow, myid);
-        gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qs, (int)prec, st_id, nn, st_d);
+        gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qv, qs, (int)prec, st_id, nn, st_d);
         wave_sync();
         const float myd = lane < nn ? st_d[lane] : 0.0f;
         const uint64_t am = __ballot(adm);
@@ -763,17 +1186,7 @@ hnsw_search_kernel(const SearchArgs P) {
           c_merge<2 * SW>(Ck, ce, head, csize, ccap, amask, akey, wsize >= ef, worst, pool,
                           has_pool, P.cpool_cap, pn, pmin, q_ovf, q_cspill);
         }
-        if (mode == 0 && vcount > P.hash_limit && spill) {
-          for (uint32_t t = lane; t < hcap; t += 64) {
-            const uint32_t v = table[t];
-            if (v != kInvalid)
-              __hip_atomic_fetch_or(spill + (v >> 5), 1u << (v & 31), __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-          }
-          __threadfence();
-          mode = 1;
-          q_spill++;
-        }
+        if (mode == 0 && vcount > P.hash_limit && spill) leave_lds();
       }
     }
 
@@ -823,7 +1236,7 @@ hnsw_search_kernel(const SearchArgs P) {
     if (prec != 0) {
       for (int base = 0; base < wsize; base += 64) {
         const int c = wsize - base < 64 ? wsize - base : 64;
-        gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qs, 0, res_id + base, c, res_d + base);
+        gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qv, qs, 0, res_id + base, c, res_d + base);
         wave_sync();
       }
       st.rr += wsize;

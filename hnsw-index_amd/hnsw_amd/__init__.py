@@ -28,6 +28,8 @@ __all__ = [
 
 MAX_DIM = 1 << 16           # src/lib.rs:70
 MAX_NODES = (1 << 32) - 2   # src/lib.rs:74
+# candidates per beam step of the bf16 / per-dimension int8 traversals
+DEFAULT_BEAM_WIDTH = 4
 
 
 # ---------------------------------------------------------------- errors
@@ -242,6 +244,7 @@ class HnswIndex:
         self._stale = False
         self._bf16 = False
         self._perdim = False
+        self._beam = DEFAULT_BEAM_WIDTH
         self._servers = 0
 
     def __del__(self):
@@ -303,6 +306,7 @@ class HnswIndex:
         self._stale = False
         self._bf16 = False
         self._perdim = False
+        self._beam = DEFAULT_BEAM_WIDTH
         self._servers = 0
         return self
 
@@ -538,6 +542,8 @@ class HnswIndex:
             self._stale = False
             if self._perdim:
                 _check(lib.hh_index_quantize_perdim(h, None, None))
+            if self._beam != DEFAULT_BEAM_WIDTH:
+                _check(lib.hh_index_set_beam_width(h, self._beam))
         elif self._stale:
             # inserts and removals since the upload: move only the changes
             ids = self._take_dirty()
@@ -563,6 +569,22 @@ class HnswIndex:
         passes with the longest queries first (default), or in one pass in
         query order; results are the same either way."""
         _check(lib.hh_index_set_query_order(self.device_index(), int(bool(longest_first))))
+
+    def set_beam_width(self, width):
+        """Candidates per layer-0 step of the bf16 and per-dimension int8
+        traversals (1: the crate's one-at-a-time rule; 2-4: beam steps, at
+        most 128 neighbour slots per step).  f32 and the crate's int8 codes
+        always follow the crate's rule."""
+        width = int(width)
+        if not 1 <= width <= 4:
+            raise InvalidArgument("beam width must be 1..4")
+        if self._idx is not None and not self._dirty:
+            self._refuse_if_served()
+            _check(lib.hh_index_set_beam_width(self._idx, width))
+        self._beam = width
+
+    def beam_width(self):
+        return self._beam
 
     def enable_bf16(self):
         if not self._bf16:

@@ -82,6 +82,8 @@ _sig("hh_index_update_deleted", INT, P, P, U64)
 _sig("hh_index_enable_bf16", INT, P)
 _sig("hh_index_quantize", INT, P, P, ctypes.POINTER(ctypes.c_float))
 _sig("hh_index_set_query_order", INT, P, INT)
+_sig("hh_index_set_beam_width", INT, P, ctypes.c_uint32)
+_sig("hh_index_beam_steps", INT, P, P)
 _sig("hh_index_sync", INT, P, ctypes.POINTER(GraphDesc), P, U64)
 _sig("hh_graph_take_dirty", INT, P, P, U64, ctypes.POINTER(U64))
 _sig("hh_graph_append", INT, P, P, P, U64, ctypes.POINTER(U64), ctypes.POINTER(U64))

@@ -314,6 +314,24 @@ int hh_index_download_bf16(hh_index *idx, uint16_t *out);
  * order. */
 int hh_index_set_query_order(hh_index *idx, int longest_first);
 
+/* Beam steps of the bf16 and per-dimension int8 traversals (no counterpart
+ * in the crate, which has neither precision).  width 1: the crate's layer-0
+ * rule (search.rs:81-142), one candidate per expansion, each new neighbour
+ * tested against the results as the sequential loop leaves them.  width
+ * 2..4 (the default is 4): each step pops up to `width` candidates (each
+ * only while the crate's stop rule does not hold for it; at most 128
+ * neighbour slots per step, so m0 = 32 takes 4 and m0 = 64 takes 2; m0 >
+ * 64 keeps width 1), gathers every new neighbour of those lists in one
+ * batch and tests each against the results as they stood at the step's
+ * start: it becomes a candidate iff fewer than ef results were held or its
+ * distance is below their worst, and enters the results when admitted.  A
+ * step's outcome does not depend on the order of its batch.  f32 and the
+ * crate's int8 codes always follow the crate's rule.  Refused while a
+ * serving loop is open; plans captured before the change are refused. */
+int hh_index_set_beam_width(hh_index *idx, uint32_t width);
+/* Candidates per beam step this index's layout allows at its width. */
+int hh_index_beam_steps(const hh_index *idx, uint32_t *steps);
+
 /* Search statistics accumulated by the kernels since the last reset. */
 typedef struct hh_search_stats {
   uint64_t queries;

@@ -841,6 +841,79 @@ static size_t search_layer0(search_ctx *s, cand ep, size_t ef, cand **out) {
   return res.len;
 }
 
+/* Beam expansion of the bf16 and per-dimension int8 traversals (the device
+ * engine's own rule; the crate has neither precision).  Each step pops up
+ * to `width` candidates, each only while search.rs:107-110's stop rule does
+ * not hold for it (the first one failing it ends the search as there).  The
+ * step's batch is every neighbour of the popped candidates not yet visited
+ * (visited in pop order, list order), and each of them is tested against
+ * the results as they stood at the step's start: it becomes a candidate iff
+ * the results held fewer than ef or its distance is below their worst, and
+ * enters the results when admitted (the results keep their ef smallest).
+ * A step's outcome does not depend on the order of its batch. */
+static size_t search_layer0_beam(search_ctx *s, cand ep, size_t ef, size_t width, cand **out) {
+  cheap cands = {NULL, 0, 0, 0}, res = {NULL, 0, 0, 1};
+  const or_graph *g = s->g;
+  s->epoch++;
+  if (s->epoch == 0) {
+    memset(s->marks, 0, sizeof(uint32_t) * g->n);
+    s->epoch = 1;
+  }
+  if (visited_insert(s, ep.id)) {
+    cheap_push(&cands, ep);
+    if (admit(s, ep.id)) cheap_push(&res, ep);
+  }
+  uint32_t exp_ids[4];
+  cand *fresh = (cand *)malloc(sizeof(cand) * (width * g->m0 + 1));
+  while (cands.len) {
+    size_t ne = 0;
+    while (ne < width && cands.len) {
+      const cand top = cands.v[0];
+      if (res.len >= ef && res.len > 0 && top.d > res.v[0].d) break;
+      exp_ids[ne++] = cheap_pop(&cands).id;
+    }
+    if (ne == 0) break;
+    s->cnt.n_expand += ne;
+    size_t nf = 0;
+    for (size_t j = 0; j < ne; j++) {
+      uint32_t cnt;
+      const uint32_t *nb = neighbors(g, exp_ids[j], 0, &cnt);
+      for (uint32_t i = 0; i < cnt; i++) {
+        if (!visited_insert(s, nb[i])) continue;
+        fresh[nf].id = nb[i];
+        fresh[nf].d = trav_dist(s, nb[i]);
+        nf++;
+      }
+    }
+    const int full = res.len >= ef && res.len > 0;
+    const float worst = res.len ? res.v[0].d : 0.0f;
+    for (size_t i = 0; i < nf; i++) {
+      if (full && !(fresh[i].d < worst)) continue;
+      cheap_push(&cands, fresh[i]);
+      if (cands.len > s->cnt.max_cands) s->cnt.max_cands = cands.len;
+      if (admit(s, fresh[i].id)) {
+        cheap_push(&res, fresh[i]);
+        if (res.len > ef) cheap_pop(&res);
+      }
+    }
+  }
+  free(fresh);
+  free(cands.v);
+  qsort(res.v, res.len, sizeof(cand), cand_qcmp);
+  *out = res.v;
+  return res.len;
+}
+
+/* Candidates a beam step of the device engine expands: the requested width
+ * (at most 4), at most 128 neighbour slots per step; lists of more than 64
+ * slots keep the crate's rule (width 1). */
+static size_t beam_steps(const or_graph *g) {
+  if (g->beam_width <= 1 || g->m0 > 64 || g->m0 == 0) return 1;
+  size_t w = g->beam_width > 4 ? 4 : g->beam_width;
+  const size_t fit = 128 / g->m0;
+  return w < fit ? w : fit;
+}
+
 typedef struct {
   float d;
   uint64_t key;
@@ -887,7 +960,10 @@ static long search_one(search_ctx *s, const float *query, size_t qlen, size_t k,
     cand ep = {trav_dist(s, g->entry_id), g->entry_id};
     s->cnt.n_upper++;
     for (int layer = g->top; layer >= 1; layer--) ep = greedy_closest(s, ep, layer);
-    nf = search_layer0(s, ep, ef, &found);
+    if ((quantized == OR_PREC_BF16 || quantized == OR_PREC_INT8_PERDIM) && beam_steps(g) > 1)
+      nf = search_layer0_beam(s, ep, ef, beam_steps(g), &found);
+    else
+      nf = search_layer0(s, ep, ef, &found);
   }
   hit *hits = (hit *)malloc(sizeof(hit) * (nf ? nf : 1));
   for (size_t i = 0; i < nf; i++) {

@@ -102,6 +102,10 @@ typedef struct {
    * code norms n (L2), or NULL */
   const int8_t *pd_codes;
   const float *pd_mid, *pd_scale, *pd_rnorm;
+  /* beam width of the bf16 and per-dimension int8 traversals (no crate
+   * counterpart): 0 or 1 keeps the crate's one-candidate-at-a-time rule,
+   * 2..4 expands that many candidates per step (search_layer0_beam) */
+  uint32_t beam_width;
 } or_graph;
 
 /* Traversal precisions (same numbering as hh_precision). */

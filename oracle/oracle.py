@@ -47,6 +47,7 @@ class Graph(ctypes.Structure):
         ("upper_off", P), ("upper", P), ("deleted", P),
         ("codes", P), ("sq_offset", P), ("sq_scale", ctypes.c_float),
         ("bf16_rows", P), ("pd_codes", P), ("pd_mid", P), ("pd_scale", P), ("pd_rnorm", P),
+        ("beam_width", ctypes.c_uint32),
     ]
 
 
@@ -321,10 +322,12 @@ class OracleGraph:
 
     def __init__(self, dim, metric, m, m0, vectors, keys, levels, layer0, upper_off, upper,
                  entry_id, top, deleted=None, codes=None, sq_offset=None, sq_scale=1.0,
-                 bf16=False, perdim=None):
+                 bf16=False, perdim=None, beam_width=1):
         """bf16: also traverse bf16-rounded rows (precision 2).  perdim:
         (mid, scale) of a per-dimension quantizer; its codes and row norms
-        are computed here from the rows (precision 3)."""
+        are computed here from the rows (precision 3).  beam_width: the
+        candidates expanded per step by those two traversals (1: the crate's
+        rule)."""
         vectors = f32(vectors)
         bf16_rows = bf16_round(vectors) if bf16 else None
         pd_codes = pd_mid = pd_scale = pd_rnorm = None
@@ -348,8 +351,12 @@ class OracleGraph:
                        _p(k["vectors"]), _p(k["keys"]), _p(k["levels"]), _p(k["layer0"]),
                        _p(k["upper_off"]), _p(k["upper"]), _p(k["deleted"]), _p(k["codes"]),
                        _p(k["sq_offset"]), float(sq_scale), _p(k["bf16_rows"]),
-                       _p(k["pd_codes"]), _p(k["pd_mid"]), _p(k["pd_scale"]), _p(k["pd_rnorm"]))
+                       _p(k["pd_codes"]), _p(k["pd_mid"]), _p(k["pd_scale"]), _p(k["pd_rnorm"]),
+                       int(beam_width))
         self.dim = dim
+
+    def set_beam_width(self, width):
+        self.g.beam_width = int(width)
 
     @staticmethod
     def _prec(quantized, precision):

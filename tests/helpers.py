@@ -6,12 +6,13 @@ import hnsw_amd as H
 import oracle as O
 
 
-def oracle_graph(idx, bf16=False, perdim=False):
+def oracle_graph(idx, bf16=False, perdim=False, beam_width=None):
     """The oracle's view of the index's current graph (arrays copied, so the
     view outlives later mutation of the index).  bf16: the oracle also
     traverses bf16-rounded rows; perdim: True trains the per-dimension
     quantizer on the rows (the oracle's restatement), a (mid, scale) tuple
-    uses those parameters, for precision 3."""
+    uses those parameters, for precision 3.  beam_width: the beam steps of
+    those two traversals (default: the index's)."""
     v = {k: (np.array(x, copy=True) if isinstance(x, np.ndarray) else x)
          for k, x in idx.graph_arrays().items()}
     if isinstance(perdim, tuple):
@@ -22,7 +23,8 @@ def oracle_graph(idx, bf16=False, perdim=False):
                          v["levels"], v["layer0"], v["upper_off"], v["upper"], v["entry_id"],
                          v["top"], deleted=v["deleted"], codes=v["codes"],
                          sq_offset=v["sq_offset"], sq_scale=v["sq_scale"], bf16=bf16,
-                         perdim=pd)
+                         perdim=pd,
+                         beam_width=idx.beam_width() if beam_width is None else beam_width)
 
 
 def build(base, metric=H.Metric.L2, m=16, keys=None, threads=1, params=None):

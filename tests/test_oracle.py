@@ -421,3 +421,94 @@ def test_perdim_distance_error_bound(metric):
             tol = 1e-5 * (abs(exact_h) + float(np.abs(w).sum()) * 127 + 1.0)
             assert abs(approx - exact_h) <= bound_q + tol, (i, approx, exact_h)
             assert abs(approx - exact) <= bound_q + bound_x + tol, (i, approx, exact)
+
+
+def _line_graph(xs, lists, m0):
+    """A one-level graph over 1-D points with the given layer-0 lists."""
+    n = len(xs)
+    layer0 = np.full((n, m0 + 1), 0xFFFFFFFF, np.uint32)
+    for i, nb in enumerate(lists):
+        layer0[i, 0] = len(nb)
+        layer0[i, 1:1 + len(nb)] = nb
+    vec = np.asarray(xs, np.float32).reshape(n, 1)
+    return O.OracleGraph(1, O.L2, m0 // 2, m0, vec, np.arange(n, dtype=np.uint64),
+                         np.zeros(n, np.uint8), layer0.reshape(-1), np.zeros(n, np.uint64),
+                         np.zeros(1, np.uint32), 0, 0, bf16=True)
+
+
+def test_beam_steps_hand_worked_graph():
+    """Six points on a line, query 0, entry x=10, ef=2 (distances are exact
+    squares).  The crate's rule expands 0, 1, 3, 4 (node 2 is never
+    popped: after node 1's step the worst result is 9 < 16).  A beam of
+    two pops 1 and 2 together (the worst is then 16, so node 2 passes the
+    stop rule), visits 5 from node 2, then pops 3 alone (5 at 12.25 is
+    beyond the worst, 9) and 4 alone: five expansions, six distances, the
+    same two results."""
+    xs = [10.0, 3.0, 4.0, 1.0, 2.0, 3.5]
+    lists = [[1, 2], [0, 3], [0, 5], [1, 4], [3], [2]]
+    g = _line_graph(xs, lists, 4)
+    q = np.zeros(1, np.float32)
+    want_keys = [3, 4]
+    for width, n_expand, n_dist in ((1, 4, 5), (2, 5, 6), (4, 5, 6)):
+        g.set_beam_width(width)
+        keys, dists, _, cnt = g.search(q, 2, 2, precision=O.PREC_BF16)
+        assert list(keys) == want_keys and list(dists) == [1.0, 4.0], width
+        assert (cnt.n_expand, cnt.n_dist) == (n_expand, n_dist), (width, cnt.n_expand, cnt.n_dist)
+    # f32 traversal ignores the beam width (the crate's rule)
+    g.set_beam_width(2)
+    _, _, _, cnt = g.search(q, 2, 2)
+    assert (cnt.n_expand, cnt.n_dist) == (4, 5)
+
+
+@pytest.mark.parametrize("width", [2, 3, 4])
+def test_beam_steps_on_a_complete_graph_equal_brute_force(width):
+    """n <= m0 with every node linked to every other: any search reaches
+    every node, so the results are the exact top-k (bf16 traversal, exact
+    re-rank)."""
+    rng = np.random.default_rng(width)
+    n, dim = 24, 16
+    vec = rng.standard_normal((n, dim)).astype(np.float32)
+    m0 = 32
+    layer0 = np.full((n, m0 + 1), 0xFFFFFFFF, np.uint32)
+    for i in range(n):
+        nb = [j for j in range(n) if j != i]
+        layer0[i, 0] = len(nb)
+        layer0[i, 1:1 + len(nb)] = nb
+    g = O.OracleGraph(dim, O.L2, 16, m0, vec, np.arange(n, dtype=np.uint64), np.zeros(n, np.uint8),
+                      layer0.reshape(-1), np.zeros(n, np.uint64), np.zeros(1, np.uint32), 5, 0,
+                      bf16=True, beam_width=width)
+    for qi in range(10):
+        q = rng.standard_normal(dim).astype(np.float32)
+        keys, dists, _, _ = g.search(q, 5, 5, precision=O.PREC_BF16)
+        bk, bd = O.brute_search(O.L2, np.arange(n, dtype=np.uint64), vec, q, 5)
+        assert np.array_equal(keys, bk) and np.array_equal(dists, bd)
+
+
+def test_beam_steps_keep_recall_and_add_few_distances():
+    """On a 10k gaussian graph built by the crate's sequential insert, beam
+    steps of 2 and 4 (bf16 and per-dimension codes) lose no recall against
+    the crate's rule (a beam explores more, never less: at most 0.01 below
+    it, where a different path may end elsewhere) at ef 32 and 128, with at
+    most 15% more distance evaluations."""
+    import hnsw_amd as H
+    base, queries = O.dataset_gaussian(10000, 200, 64, 42)
+    idx = H.HnswIndex(64, H.Metric.L2, H.HnswParams(16))
+    idx.insert_batch(np.arange(10000), base, threads=1)
+    v = idx.graph_arrays()
+    tk, _, tc = O.brute_search_batch(O.L2, v["keys"], v["vectors"], queries, 10)
+    mid, scale = O.pd_train(v["vectors"])
+    g = O.OracleGraph(v["dim"], v["metric"], v["m"], v["m0"], v["vectors"], v["keys"],
+                      v["levels"], v["layer0"], v["upper_off"], v["upper"], v["entry_id"],
+                      v["top"], bf16=True, perdim=(mid, scale))
+    for prec in (O.PREC_BF16, O.PREC_INT8_PERDIM):
+        for ef in (32, 128):
+            g.set_beam_width(1)
+            ok, _, oc, c1 = g.search_batch(queries, 10, ef, precision=prec)
+            r1 = O.recall_at_k(tk, tc, ok, oc, 10)
+            for width in (2, 4):
+                g.set_beam_width(width)
+                ok, _, oc, cw = g.search_batch(queries, 10, ef, precision=prec)
+                rw = O.recall_at_k(tk, tc, ok, oc, 10)
+                assert rw >= r1 - 0.01, (prec, ef, width, r1, rw)
+                assert cw.n_dist <= 1.15 * c1.n_dist, (prec, ef, width)
+                assert cw.n_expand >= c1.n_expand  # every step expands at least as much
