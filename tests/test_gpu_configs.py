@@ -11,9 +11,10 @@ C3  U(0,1) rows L2-normalised, cosine, D=960, M=32, k 10/100, ef 64/256, in
 C4  N(0,1) rows with lognormal norms, inner product, D=256, M=24, one mixed
     batch (k 1/10/100, ef 32..512, 1-3 of 20 categories per query) replayed
     from a HIP graph: the D=256 kernel at the 512-entry list bucket.
-C5  8 Gaussian clusters, sigma 0.3, L2, D=96, M=16, bf16 + int8 rows,
-    served by the continuous-batching loop with Poisson arrivals, k 1/10/50,
-    ef 16/64/256 and an optional 10% key bitmap.
+C5  8 Gaussian clusters, sigma 0.3, L2, D=96, M=16, bf16 rows + int8 codes
+    (the crate's and the per-dimension ones), served by the continuous-
+    batching loop with Poisson arrivals, k 1/10/50, ef 16/64/256 and an
+    optional 10% key bitmap.
 """
 import ctypes
 import json
@@ -182,14 +183,16 @@ def test_c5_serving_bf16_and_int8_matches_oracle():
     idx = build(b, H.Metric.L2, m=16, threads=16)
     idx.enable_quantization()
     idx.enable_bf16()
-    g = oracle_graph(idx, bf16=True)
+    pd = idx.enable_quantization_perdim()
+    g = oracle_graph(idx, bf16=True, perdim=pd)
     keys = idx.graph_arrays()["keys"]
     mask = (keys % 10) == 3  # a 10% key bitmap
     rng = np.random.default_rng(9)
     n = q.shape[0]
     ks = rng.choice([1, 10, 50], n)
     efs = rng.choice([16, 64, 256], n)
-    precs = rng.choice([int(H.Precision.BF16), int(H.Precision.INT8)], n)
+    precs = rng.choice([int(H.Precision.BF16), int(H.Precision.INT8),
+                        int(H.Precision.INT8_PERDIM)], n)
     fsel = np.where(rng.random(n) < 0.3, 0, -1)
     arrival = np.cumsum(rng.exponential(1e6 / 200000.0, n))  # 200k requests/s
     with H.Server(idx, k_max=50, max_inflight=2) as srv:
