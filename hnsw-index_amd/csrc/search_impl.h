@@ -336,6 +336,58 @@ __device__ __forceinline__ void joint_ranks_n(const uint64_t (&ce)[CPL], uint32_
   apc = rc + bc;
   apw = rw + bw;
 }
+// Ranks of a batch (amask, words akey) against the single list U of a
+// beam step (entries e, expanded marks f, size entries): every element's
+// merged position (ep for U's, the return value for the batch's), and for
+// each batch element its rank among the merged order's unexpanded entries
+// (ua: unexpanded U entries and batch entries before it).  N: the chunks of
+// U that hold entries.
+template <int SW, int N>
+__device__ __forceinline__ uint32_t u_ranks_n(const uint64_t (&e)[SW], const uint32_t (&f)[SW],
+                                              uint32_t (&ep)[SW], uint32_t size, uint64_t amask,
+                                              uint64_t akey, uint32_t &ua) {
+  const int lane = lane_id();
+  uint64_t vc[N], uc[N];
+#pragma unroll
+  for (int c = 0; c < N; c++) {
+    vc[c] = __ballot(c * 64 + lane < (int)size);
+    uc[c] = __ballot(c * 64 + lane < (int)size && f[c] == 0u);
+  }
+#pragma unroll
+  for (int c = 0; c < SW; c++) ep[c] = c * 64 + lane;
+  uint32_t b = 0, rl_ = 0, ru = 0;
+  for (uint64_t m = amask; m; m &= m - 1) {
+    const int j = __ffsll((unsigned long long)m) - 1;
+    const uint64_t kj = rl64(akey, j);
+    uint32_t r = 0, u = 0;
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      const uint64_t lt = __ballot(e[c] < kj);
+      r += (uint32_t)popc64(lt & vc[c]);
+      u += (uint32_t)popc64(lt & uc[c]);
+      ep[c] += (e[c] < kj) ? 0u : 1u;
+    }
+    b += kj < akey ? 1u : 0u;
+    rl_ = lane == j ? r : rl_;
+    ru = lane == j ? u : ru;
+  }
+  ua = ru + b;
+  return rl_ + b;
+}
+template <int SW>
+__device__ __forceinline__ uint32_t u_ranks(const uint64_t (&e)[SW], const uint32_t (&f)[SW],
+                                            uint32_t (&ep)[SW], uint32_t size, uint64_t amask,
+                                            uint64_t akey, uint32_t &ua) {
+  if (size <= 64) return u_ranks_n<SW, 1>(e, f, ep, size, amask, akey, ua);
+  if constexpr (SW > 2) {
+    if (size <= 128) return u_ranks_n<SW, 2>(e, f, ep, size, amask, akey, ua);
+  }
+  if constexpr (SW > 4) {
+    if (size <= 256) return u_ranks_n<SW, 4>(e, f, ep, size, amask, akey, ua);
+  }
+  return u_ranks_n<SW, SW>(e, f, ep, size, amask, akey, ua);
+}
+
 template <int CPL, int WPL>
 __device__ __forceinline__ void joint_ranks(const uint64_t (&ce)[CPL], uint32_t (&cp)[CPL],
                                             uint32_t csize, const uint64_t (&we)[WPL],
@@ -698,7 +750,7 @@ __device__ __forceinline__ void seed_query(const SearchArgs &P, uint32_t qi, uin
 // uniform bf16 or code batch runs the instance of its precision, so its
 // loop carries no code or registers for the others.
 template <int TMAX, bool EXTRA, bool EXACT, int SW, int PC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? (PC == kPrecAny ? 4 : 3) : 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? 3 : 2)))
 hnsw_search_kernel(const SearchArgs P) {
   extern __shared__ uint32_t lds[];
   const int lane = lane_id(), i16 = lane & 15;
@@ -977,6 +1029,161 @@ hnsw_search_kernel(const SearchArgs P) {
       ff.sp1 = (uint32_t)(lane + 64) - ff.sj1 * m0;
       ff.id0 = ff.id1 = ff.id2 = ff.id3 = kInvalid;
       ff.v0 = ff.v1 = kInvalid;
+      if (allow == nullptr && P.deleted_bits == nullptr) {
+        // -- every visited node is admissible: one list U (= W) of the ef
+        // smallest seen, each entry marked once expanded; the candidates are
+        // its unexpanded entries (C and the pool stay unused)
+        uint32_t *F = reinterpret_cast<uint32_t *>(Ck);  // marks, by U position
+        uint64_t ue[SW];
+        uint32_t uf[SW];
+        list_load<SW>(Wk, wsize, ue);
+#pragma unroll
+        for (int c = 0; c < SW; c++) uf[c] = 0u;
+        // the first `beam` unexpanded entries, marked: their ids
+        auto take_front = [&](uint32_t &y0, uint32_t &y1, uint32_t &y2, uint32_t &y3) {
+          uint32_t got = 0;
+          y0 = y1 = y2 = y3 = kInvalid;
+#pragma unroll
+          for (int c = 0; c < SW; c++) {
+            if (got >= beam || c * 64 >= (int)wsize) break;
+            const bool cand = c * 64 + lane < (int)wsize && uf[c] == 0u;
+            const uint32_t r = got + (uint32_t)mbcnt(__ballot(cand));
+            const bool pick = cand && r < beam;
+            for (uint64_t m = __ballot(pick); m; m &= m - 1) {
+              const uint32_t id = rl((uint32_t)ue[c], __ffsll((unsigned long long)m) - 1);
+              if (got == 0) y0 = id;
+              else if (got == 1) y1 = id;
+              else if (got == 2) y2 = id;
+              else y3 = id;
+              got++;
+            }
+            uf[c] = pick ? 1u : uf[c];
+          }
+          return got;
+        };
+        uint32_t y0, y1, y2, y3;
+        uint32_t ne = take_front(y0, y1, y2, y3);
+        while (ne > 0) {
+          q_exp += ne;
+          // the table never fills: leave it before a step could pass its limit
+          if (mode == 0 && spill && vcount + ne * m0 > P.hash_limit) leave_lds();
+          const uint32_t c0 = ff.sj0 < ne ? sel4(ff.sj0, y0, y1, y2, y3) : kInvalid;
+          const uint32_t c1 = ff.sj1 < ne ? sel4(ff.sj1, y0, y1, y2, y3) : kInvalid;
+          uint32_t nid0 = kInvalid, nid1 = kInvalid;
+          if (c0 != kInvalid)
+            nid0 = c0 == sel4(ff.sj0, ff.id0, ff.id1, ff.id2, ff.id3) ? ff.v0
+                                                                      : adj0[(size_t)c0 * m0 + ff.sp0];
+          if (c1 != kInvalid)
+            nid1 = c1 == sel4(ff.sj1, ff.id0, ff.id1, ff.id2, ff.id3) ? ff.v1
+                                                                      : adj0[(size_t)c1 * m0 + ff.sp1];
+          bool nw0 = false, nw1 = false;
+          if (mode == 0) {
+            lds_visit2(table, hcap, nid0, nid1, nw0, nw1);
+          } else {
+            if (nid0 != kInvalid) nw0 = bit_visit(spill, nid0);
+            if (nid1 != kInvalid) nw1 = bit_visit(spill, nid1);
+          }
+          const uint64_t nm0 = __ballot(nw0), nm1 = __ballot(nw1);
+          const uint32_t n0 = (uint32_t)popc64(nm0), nn = n0 + (uint32_t)popc64(nm1);
+          if (mode == 1 && nn > 0) {
+            if (slog_n + nn <= P.spill_log_cap) {
+              if (nw0) slog[slog_n + mbcnt(nm0)] = nid0;
+              if (nw1) slog[slog_n + n0 + mbcnt(nm1)] = nid1;
+            }
+            slog_n += nn;
+          }
+          vcount += nn;
+          q_dist += nn;
+          bool fetched = false;
+          if (nn > 0) {
+            if (nw0) st_id[mbcnt(nm0)] = nid0;
+            if (nw1) st_id[n0 + mbcnt(nm1)] = nid1;
+            wave_sync();
+            gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qv, qs, (int)prec, st_id, (int)nn, st_d);
+            wave_sync();
+            // every new neighbour against U as it stood at the step's start
+            const bool wfull0 = wsize >= ef;
+            const float worst0 = worst;
+            for (uint32_t base = 0; base < nn; base += 64) {
+              const uint32_t i = base + (uint32_t)lane;
+              const uint32_t myid = i < nn ? st_id[i] : kInvalid;
+              const float myd = i < nn ? st_d[i] : 0.0f;
+              const bool surv = i < nn && (!wfull0 || myd < worst0);
+              const uint64_t amask = __ballot(surv);
+              if (!amask) continue;
+              const uint64_t akey = surv ? ckey(myd, myid) : ~0ull;
+              uint32_t wp[SW], ua;
+              const uint32_t ap = u_ranks<SW>(ue, uf, wp, wsize, amask, akey, ua);
+              const uint32_t total = wsize + (uint32_t)popc64(amask);
+              const uint32_t nsize = total < ef ? total : ef;
+              if (base + 64 >= nn) {
+                // the merged order's first `beam` unexpanded entries are the
+                // next step's pops: their lists load during the writes.  A U
+                // entry's unexpanded rank counts the unexpanded U entries and
+                // the batch entries before it; a batch entry's is ua.
+                uint32_t id[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
+                uint32_t uprev = 0;
+#pragma unroll
+                for (int c = 0; c < SW; c++) {
+                  if (c > 0 && c * 64 >= (int)wsize) break;
+                  const bool un = c * 64 + lane < (int)wsize && uf[c] == 0u;
+                  const uint64_t um = __ballot(un);
+                  const uint32_t ur =
+                      uprev + (uint32_t)mbcnt(um) + (wp[c] - (uint32_t)(c * 64 + lane));
+#pragma unroll
+                  for (int p = 0; p < 4; p++) {
+                    const uint64_t hit =
+                        __ballot(un && wp[c] < nsize && ur == (uint32_t)p && p < (int)beam);
+                    if (hit) id[p] = rl((uint32_t)ue[c], __ffsll((unsigned long long)hit) - 1);
+                  }
+                  uprev += (uint32_t)popc64(um);
+                }
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                  const uint64_t hit = __ballot(surv && ap < nsize && ua == (uint32_t)p && p < (int)beam);
+                  if (hit) id[p] = rl((uint32_t)akey, __ffsll((unsigned long long)hit) - 1);
+                }
+                ff.id0 = id[0];
+                ff.id1 = id[1];
+                ff.id2 = id[2];
+                ff.id3 = id[3];
+                front_fetch(ff);
+                fetched = true;
+              }
+#pragma unroll
+              for (int c = 0; c < SW; c++) {
+                if (c > 0 && c * 64 >= (int)wsize) break;
+                if (c * 64 + lane < (int)wsize && wp[c] < nsize) {
+                  Wk[wp[c]] = ue[c];
+                  F[wp[c]] = uf[c];
+                }
+              }
+              if (surv && ap < nsize) {
+                Wk[ap] = akey;
+                F[ap] = 0u;
+              }
+              wave_sync();
+              wsize = nsize;
+              list_load<SW>(Wk, wsize, ue);
+#pragma unroll
+              for (int c = 0; c < SW; c++) {
+                const uint32_t x = c * 64 + lane;
+                uf[c] = x < wsize ? F[x] : 1u;
+              }
+              worst = ckey_d(Wk[wsize - 1]);
+              wave_sync();
+            }
+          }
+          ne = take_front(y0, y1, y2, y3);
+          if (!fetched) {
+            ff.id0 = y0;
+            ff.id1 = y1;
+            ff.id2 = y2;
+            ff.id3 = y3;
+            front_fetch(ff);
+          }
+        }
+      } else
       for (;;) {
         if (csize < beam && pn > 0) refill();
         if (csize == 0) break;
@@ -1180,11 +1387,15 @@ hnsw_search_kernel(const SearchArgs P) {
         const uint64_t amask = __ballot(acc);
         const uint64_t akey = acc ? ckey(myd, myid) : ~0ull;
         if (amask) {
+          // one pass over the accepted neighbours ranks them in C and (the
+          // admitted ones) in W
           uint64_t ce[2 * SW];
           list_load<2 * SW>(Ck + head, csize, ce);
-          if (amask & am) worst = ckey_d(w_merge<SW>(Wk, we, wsize, ef, amask & am, akey));
-          c_merge<2 * SW>(Ck, ce, head, csize, ccap, amask, akey, wsize >= ef, worst, pool,
-                          has_pool, P.cpool_cap, pn, pmin, q_ovf, q_cspill);
+          uint32_t cp[2 * SW], wp[SW], apc, apw;
+          joint_ranks<2 * SW, SW>(ce, cp, csize, we, wp, wsize, amask, amask & am, akey, apc, apw);
+          if (amask & am) worst = ckey_d(w_write<SW>(Wk, we, wp, apw, wsize, ef, amask & am, akey));
+          c_write<2 * SW>(Ck, ce, cp, apc, head, csize, ccap, amask, akey, wsize >= ef, worst,
+                          pool, has_pool, P.cpool_cap, pn, pmin, q_ovf, q_cspill);
         }
         if (mode == 0 && vcount > P.hash_limit && spill) leave_lds();
       }

@@ -715,7 +715,9 @@ class Server:
     def register_filter(self, mask):
         """mask: bool array over node ids. Returns the filter id."""
         fid = ctypes.c_uint32()
-        _check(lib.hh_serve_register_filter(self._h, _ptr(_bits_from_mask(mask)), ctypes.byref(fid)))
+        # the bitmap stays referenced until the library has copied it
+        bits = _bits_from_mask(mask)
+        _check(lib.hh_serve_register_filter(self._h, _ptr(bits), ctypes.byref(fid)))
         return fid.value
 
     def submit(self, query, k, ef, precision=Precision.F32, filter_id=-1):

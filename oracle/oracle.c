@@ -904,6 +904,79 @@ static size_t search_layer0_beam(search_ctx *s, cand ep, size_t ef, size_t width
   return res.len;
 }
 
+/* Beam steps of a query with no filter on an index with no tombstones:
+ * every visited node is admissible, so the candidates worth expanding are
+ * exactly the results not yet expanded.  One sorted list U holds the ef
+ * smallest (distance, id) seen, each with an expanded mark.  Each step pops
+ * the first `width` unexpanded entries of U (none left ends the search),
+ * visits their lists like search_layer0_beam, and merges every new
+ * neighbour that passes the step-start test (U held fewer than ef, or its
+ * distance is below U's worst) into U, which keeps its ef smallest.  It
+ * differs from search_layer0_beam only where a distance equals the worst
+ * result's: a neighbour that falls out of the results then stays a
+ * candidate there and is dropped here. */
+static size_t search_layer0_unified(search_ctx *s, cand ep, size_t ef, size_t width,
+                                    cand **out) {
+  const or_graph *g = s->g;
+  s->epoch++;
+  if (s->epoch == 0) {
+    memset(s->marks, 0, sizeof(uint32_t) * g->n);
+    s->epoch = 1;
+  }
+  cand *u = (cand *)malloc(sizeof(cand) * (ef + width * g->m0 + 1));
+  uint8_t *done = (uint8_t *)malloc(ef + width * g->m0 + 1);
+  cand *fresh = (cand *)malloc(sizeof(cand) * (width * g->m0 + 1));
+  size_t un = 0;
+  if (visited_insert(s, ep.id)) {
+    u[0] = ep;
+    done[0] = 0;
+    un = 1;
+  }
+  uint32_t exp_ids[4];
+  for (;;) {
+    size_t ne = 0;
+    for (size_t i = 0; i < un && ne < width; i++)
+      if (!done[i]) {
+        done[i] = 1;
+        exp_ids[ne++] = u[i].id;
+      }
+    if (ne == 0) break;
+    s->cnt.n_expand += ne;
+    size_t nf = 0;
+    for (size_t j = 0; j < ne; j++) {
+      uint32_t cnt;
+      const uint32_t *nb = neighbors(g, exp_ids[j], 0, &cnt);
+      for (uint32_t i = 0; i < cnt; i++) {
+        if (!visited_insert(s, nb[i])) continue;
+        fresh[nf].id = nb[i];
+        fresh[nf].d = trav_dist(s, nb[i]);
+        nf++;
+      }
+    }
+    const int full = un >= ef;
+    const float worst = un ? u[un - 1].d : 0.0f;
+    for (size_t i = 0; i < nf; i++) {
+      if (full && !(fresh[i].d < worst)) continue;
+      /* insertion into the sorted list, then keep the ef smallest */
+      size_t p = un;
+      while (p > 0 && cand_less(fresh[i], u[p - 1])) {
+        u[p] = u[p - 1];
+        done[p] = done[p - 1];
+        p--;
+      }
+      u[p] = fresh[i];
+      done[p] = 0;
+      un++;
+      if (un > s->cnt.max_cands) s->cnt.max_cands = un;
+    }
+    if (un > ef) un = ef;
+  }
+  free(fresh);
+  free(done);
+  *out = u;
+  return un;
+}
+
 /* Candidates a beam step of the device engine expands: the requested width
  * (at most 4), at most 128 neighbour slots per step; lists of more than 64
  * slots keep the crate's rule (width 1). */
@@ -960,7 +1033,11 @@ static long search_one(search_ctx *s, const float *query, size_t qlen, size_t k,
     cand ep = {trav_dist(s, g->entry_id), g->entry_id};
     s->cnt.n_upper++;
     for (int layer = g->top; layer >= 1; layer--) ep = greedy_closest(s, ep, layer);
-    if ((quantized == OR_PREC_BF16 || quantized == OR_PREC_INT8_PERDIM) && beam_steps(g) > 1)
+    const int beam = (quantized == OR_PREC_BF16 || quantized == OR_PREC_INT8_PERDIM) &&
+                     beam_steps(g) > 1;
+    if (beam && !s->allow && live == g->n)
+      nf = search_layer0_unified(s, ep, ef, beam_steps(g), &found);
+    else if (beam)
       nf = search_layer0_beam(s, ep, ef, beam_steps(g), &found);
     else
       nf = search_layer0(s, ep, ef, &found);

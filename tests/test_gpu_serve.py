@@ -89,3 +89,31 @@ def test_launches_take_one_ef_bucket_at_a_time():
         ok, od, _, _ = g.search(q[i], 10, int(efs[i]))
         c = int(sc[i])
         assert np.array_equal(sk[i, :c], ok) and same_bits(sd[i, :c], od), i
+
+
+def test_registered_filters_are_copied_whole_and_respected():
+    """The filter table keeps exactly the bitmaps registered, whatever the
+    host allocator does with the caller's memory afterwards (the binding
+    keeps its bitmap alive until the library has copied it), and every
+    served result of a filtered request passes its filter and equals the
+    oracle's: several servers in a row, each with every request filtered."""
+    b, q = O.dataset_gaussian(6000, 300, 32, 31)
+    idx = build(b, threads=4)
+    g = oracle_graph(idx)
+    keys = idx.graph_arrays()["keys"]
+    mask = (keys % 10) == 3
+    allow = mask.astype(np.uint8)
+    want = [g.search(q[i], 10, 64, allow=allow)[0] for i in range(len(q))]
+    for rep in range(3):
+        with H.Server(idx, k_max=10, max_inflight=2) as srv:
+            fid = srv.register_filter(mask)
+            junk = [np.random.default_rng(rep).random(5000) for _ in range(20)]  # reuse freed memory
+            del junk
+            arrival = np.cumsum(np.full(len(q), 5.0))
+            lat, sk, sd, sc, sst = srv.replay(q, np.full(len(q), 10), np.full(len(q), 64), arrival,
+                                              filter_ids=np.full(len(q), fid))
+        assert (sst == 0).all()
+        for i in range(len(q)):
+            c = int(sc[i])
+            assert mask[sk[i, :c].astype(np.int64)].all(), (rep, i)
+            assert np.array_equal(sk[i, :c], want[i]), (rep, i)
