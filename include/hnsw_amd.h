@@ -11,7 +11,7 @@
  * Each entry point cites the reference interface it replaces
  * (aozmekik/hnsw-index, file:line).
  *
- * Status codes: 0 on success, otherwise one of hh_status.  Codes 1..11 map
+ * Status codes: 0 on success, otherwise one of hh_status.  Codes 1..10 map
  * one-to-one onto the variants of the crate's `Error` (src/error.rs:9-51);
  * codes >= 100 are device-side conditions the crate has no variant for.
  * hh_last_error() returns a message for the calling thread's last failure.
