@@ -216,7 +216,7 @@ int tmax_for(const hh::RowLayout &lay) {
 // id and distance staging (st entries each: 64, or 128 for beam steps), C
 // (128 * sw candidate words) and W (64 * sw result words), candidate words
 // being 2 words each.
-uint32_t lds_words_for(uint32_t hash_cap, int sw, uint32_t st = 128) {
+uint32_t lds_words_for(uint32_t hash_cap, int sw, uint32_t st = 256) {
   return hash_cap + 2 * st + 384u * (uint32_t)sw;
 }
 
@@ -624,12 +624,12 @@ int first_failure(const std::vector<int32_t> &status, uint64_t *first_bad) {
   return HH_OK;
 }
 
-// Candidates a beam step expands: the index's width, at most 4 and at most
-// 128 neighbour slots per step; lists longer than 64 slots keep the crate's
-// one-candidate rule.
+// Candidates a beam step expands: the index's width, at most 8 and at most
+// 256 neighbour slots per step; lists longer than 128 slots keep the
+// crate's one-candidate rule.
 uint32_t beam_steps(const hh_index *idx) {
-  if (idx->beam_width <= 1 || idx->m0 == 0 || idx->m0 > 64) return 1;
-  return std::min<uint32_t>(std::min<uint32_t>(idx->beam_width, 4u), 128u / idx->m0);
+  if (idx->beam_width <= 1 || idx->m0 == 0 || idx->m0 > 128) return 1;
+  return std::min<uint32_t>(std::min<uint32_t>(idx->beam_width, 8u), 256u / idx->m0);
 }
 
 // Launch shape of one search: kernel instance, visited-set capacity,
@@ -637,6 +637,7 @@ uint32_t beam_steps(const hh_index *idx) {
 struct SearchConfig {
   void *kern = nullptr;
   uint32_t hash_cap = 0, lds_words = 0, st_cap = 64;
+  uint32_t beam = 1;  // candidates per layer-0 step of bf16 / per-dimension queries
   int wpb = 1;
   uint64_t blocks = 0, spill_words = 0;
 };
@@ -699,16 +700,21 @@ int check_call(hh_index *idx, const SearchCall &c) {
 int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
   const uint32_t efmax = std::max<uint32_t>(spec_ef_bound(c), 1);
   const int sw = slots_for(efmax);
-  // visited-set capacity: 2048 entries (8 KiB of LDS per wavefront, which
-  // keeps the register-limited 16 wavefronts per CU resident, and holds the
-  // result arrays afterwards); past 3/4 full a query continues on its HBM
-  // bitset.  hh_index_autotune trades this against occupancy per ef.
+  // visited-set capacity: 2560 entries (10 KiB of LDS per wavefront; with
+  // the lists and staging at ef <= 64, 12-12.5 KiB, which keeps the
+  // register-limited 12 wavefronts per CU resident, and holds the result
+  // arrays afterwards); past 3/4 full a query continues on its HBM bitset.
+  // Measured at 1M x 128, ef=64: 2048 entries sent 65 of 10,000 f32 queries
+  // and 611 per-dimension beam queries to the bitset, 2560 sent 11 and 177,
+  // +1% f32 and +9% per-dimension QPS.  hh_index_autotune trades this
+  // against occupancy per ef.
   const int bucket = sw == 1 ? 0 : sw == 2 ? 1 : sw == 4 ? 2 : 3;
-  cfg.hash_cap = idx->tuned_hash_cap[bucket] ? idx->tuned_hash_cap[bucket] : 2048u;
+  cfg.hash_cap = idx->tuned_hash_cap[bucket] ? idx->tuned_hash_cap[bucket] : 2560u;
   cfg.hash_cap = std::max<uint32_t>(cfg.hash_cap, 256u * (uint32_t)sw);
-  // beam steps stage up to 128 new neighbours
-  cfg.st_cap = beam_steps(idx) > 1 && !c.build && (c.specs || c.precision == 2 || c.precision == 3)
-                   ? 128u : 64u;
+  // beam steps stage every neighbour slot of a step (up to 256)
+  const uint32_t bs = beam_steps(idx);
+  cfg.beam = bs > 1 && !c.build && (c.specs || c.precision == 2 || c.precision == 3) ? bs : 1u;
+  cfg.st_cap = cfg.beam > 1 ? std::max<uint32_t>(64u, (cfg.beam * idx->m0 + 63) / 64 * 64) : 64u;
   cfg.lds_words = lds_words_for(cfg.hash_cap, sw, cfg.st_cap);
   cfg.kern = pick_search_kernel(idx->lay, sw, precision_class(c));
   // waves per block: the tuned value, else whichever of 1, 2, 4 keeps the
@@ -865,7 +871,7 @@ int enqueue_search(hh_index *idx, Workspace &ws, const SearchCall &c, const Sear
   a.hash_limit = cfg.hash_cap / 2 + cfg.hash_cap / 4;
   a.lds_words = cfg.lds_words;
   a.st_cap = cfg.st_cap;
-  a.beam = cfg.st_cap == 128 ? beam_steps(idx) : 1u;
+  a.beam = cfg.beam;
   a.seed = ws.seed;
   a.order = ws.order;
   // a pool smaller than the node count can drop a candidate: such queries
@@ -1738,7 +1744,7 @@ int hh_index_set_query_order(hh_index *idx, int longest_first) {
 
 int hh_index_set_beam_width(hh_index *idx, uint32_t width) {
   if (!idx) return fail(HH_INVALID_ARGUMENT, "null index");
-  if (width < 1 || width > 4) return fail(HH_INVALID_ARGUMENT, "beam width must be 1..4");
+  if (width < 1 || width > 8) return fail(HH_INVALID_ARGUMENT, "beam width must be 1..8");
   std::unique_lock<std::shared_mutex> wr(idx->mu);
   if (int st = refuse_if_served(idx)) return st;
   int st = drain(idx);
@@ -1827,7 +1833,7 @@ int hh_index_autotune(hh_index *idx, const float *queries, uint64_t nq, uint32_t
   float best_ms = 1e30f;
   uint32_t best_cap = 0;
   int best_wpb = 0;
-  const uint32_t caps[] = {1024, 1536, 1792, 2048, 3072, 4096, 6144, 8192, 12288, 16384};
+  const uint32_t caps[] = {1024, 1536, 1792, 2048, 2560, 3072, 4096, 6144, 8192, 12288, 16384};
   int st = HH_OK;
   // best of three timed launches after a warm-up, in the current shape and order
   auto time_launch = [&](float &ms_best) {

@@ -151,58 +151,117 @@ __device__ __forceinline__ uint64_t w_merge(uint64_t *W, const uint64_t (&e)[WPL
   return w_write<WPL>(W, e, ep, ap, size, cap, amask, akey);
 }
 
-// One of four wave-uniform values, chosen per lane by j (explicit selects:
+// Beam steps expand up to kBeamMax lists, up to 256 neighbour slots: slot
+// s = lane + 64 g (lane group g < kBeamGroups) holds list s / m0, position
+// s % m0.
+constexpr int kBeamMax = 8, kBeamGroups = 4;
+
+// One of eight wave-uniform values, chosen per lane by j (explicit selects:
 // a lane-indexed private array would live in scratch memory).
-__device__ __forceinline__ uint32_t sel4(uint32_t j, uint32_t a, uint32_t b, uint32_t c,
-                                         uint32_t d) {
-  return j == 0 ? a : j == 1 ? b : j == 2 ? c : d;
+__device__ __forceinline__ uint32_t sel8(uint32_t j, const uint32_t (&x)[kBeamMax]) {
+  uint32_t r = kInvalid;
+#pragma unroll
+  for (int t = 0; t < kBeamMax; t++) r = j == (uint32_t)t ? x[t] : r;
+  return r;
 }
 
-// Adjacency prefetch of a beam step: the candidates at C's front after a
+// Adjacency prefetch of a beam step: the candidates at the front after a
 // merge are the next step's pops (unless the pool refills C first), so their
 // layer-0 lists are loaded as soon as the merge ranks are known, in the
-// step's slot layout (slot s = lane + 64 g holds list s / m0, position
-// s % m0).  A hint only: the next step checks each list's id.
+// step's slot layout.  A hint only: the next step checks each list's id.
 struct FrontFetch {
   const uint32_t *adj0;
-  uint32_t m0, n;        // n: front positions captured (the beam)
-  uint32_t sj0, sp0, sj1, sp1;
-  uint32_t id0, id1, id2, id3;
-  uint32_t v0, v1;
+  uint32_t m0, n, groups;  // lists per step, lane groups they take
+  uint32_t sj[kBeamGroups], sp[kBeamGroups];  // this lane's list and position, per group
+  uint32_t id[kBeamMax];
+  uint32_t v[kBeamGroups];
 };
 
 __device__ __forceinline__ void front_fetch(FrontFetch &f) {
   gptr<uint32_t> adj = G(f.adj0);
-  const uint32_t a = sel4(f.sj0, f.id0, f.id1, f.id2, f.id3);
-  const uint32_t b = sel4(f.sj1, f.id0, f.id1, f.id2, f.id3);
-  f.v0 = f.sj0 < f.n && a != kInvalid ? adj[(size_t)a * f.m0 + f.sp0] : kInvalid;
-  f.v1 = f.sj1 < f.n && b != kInvalid ? adj[(size_t)b * f.m0 + f.sp1] : kInvalid;
+#pragma unroll
+  for (int g = 0; g < kBeamGroups; g++) {
+    f.v[g] = kInvalid;
+    if ((uint32_t)g < f.groups) {
+      const uint32_t a = sel8(f.sj[g], f.id);
+      if (f.sj[g] < f.n && a != kInvalid) f.v[g] = adj[(size_t)a * f.m0 + f.sp[g]];
+    }
+  }
 }
 
 // The merged order's first f.n entries, given C's first chunk (e0, its
 // merged positions ep0, size entries) and the batch (amask, words akey at
-// positions ap), and their lists fetched: list entries at positions below 4
-// come from the first chunk, since positions never fall below indices.
+// positions ap), and their lists fetched: list entries at positions below
+// kBeamMax come from the first chunk, since positions never fall below
+// indices.
 __device__ __forceinline__ void front_capture(FrontFetch &f, uint64_t e0, uint32_t ep0, uint32_t ap,
                                               uint64_t amask, uint64_t akey, uint32_t size) {
   const int lane = lane_id();
   const bool mine = (amask >> lane) & 1;
-  uint32_t id[4];
 #pragma unroll
-  for (int p = 0; p < 4; p++) {
-    id[p] = kInvalid;
+  for (int p = 0; p < kBeamMax; p++) {
+    f.id[p] = kInvalid;
     if (p < (int)f.n) {
       const uint64_t la = __ballot(mine && ap == (uint32_t)p);
       const uint64_t lb = __ballot(lane < (int)size && ep0 == (uint32_t)p);
-      if (la) id[p] = rl((uint32_t)akey, __ffsll((unsigned long long)la) - 1);
-      else if (lb) id[p] = rl((uint32_t)e0, __ffsll((unsigned long long)lb) - 1);
+      if (la) f.id[p] = rl((uint32_t)akey, __ffsll((unsigned long long)la) - 1);
+      else if (lb) f.id[p] = rl((uint32_t)e0, __ffsll((unsigned long long)lb) - 1);
     }
   }
-  f.id0 = id[0];
-  f.id1 = id[1];
-  f.id2 = id[2];
-  f.id3 = id[3];
   front_fetch(f);
+}
+
+// One beam step's visit: the neighbours of the popped lists x[0..ne) in slot
+// layout (each slot's id from the prefetch where its list matches, else
+// loaded), probed in the visited set (LDS table or, in mode 1, the HBM
+// bitset with the spill log), and the new ones staged in st_id[0..nn) in
+// slot order.  Returns nn.  All 64 lanes must call it.
+__device__ __forceinline__ uint32_t beam_visit(gptr<uint32_t> adj0, const uint32_t (&x)[kBeamMax],
+                                               uint32_t ne, const FrontFetch &ff, uint32_t *table,
+                                               uint32_t hcap, int mode, uint32_t *spill,
+                                               gmut<uint32_t> slog, uint32_t slog_cap,
+                                               uint32_t &slog_n, uint32_t *st_id) {
+  const uint32_t groups = (ne * ff.m0 + 63) / 64;
+  uint32_t nid[kBeamGroups];
+  bool nw[kBeamGroups];
+#pragma unroll
+  for (int g = 0; g < kBeamGroups; g++) {
+    nid[g] = kInvalid;
+    nw[g] = false;
+    if ((uint32_t)g < groups) {
+      const uint32_t c = ff.sj[g] < ne ? sel8(ff.sj[g], x) : kInvalid;
+      if (c != kInvalid)
+        nid[g] = c == sel8(ff.sj[g], ff.id) ? ff.v[g] : adj0[(size_t)c * ff.m0 + ff.sp[g]];
+    }
+  }
+  if (mode == 0) {
+    lds_visit2(table, hcap, nid[0], nid[1], nw[0], nw[1]);
+    if (groups > 2) lds_visit2(table, hcap, nid[2], nid[3], nw[2], nw[3]);
+  } else {
+#pragma unroll
+    for (int g = 0; g < kBeamGroups; g++)
+      if (nid[g] != kInvalid) nw[g] = bit_visit(spill, nid[g]);
+  }
+  uint64_t nm[kBeamGroups];
+  uint32_t pre[kBeamGroups], nn = 0;
+#pragma unroll
+  for (int g = 0; g < kBeamGroups; g++) {
+    nm[g] = __ballot(nw[g]);
+    pre[g] = nn;
+    nn += (uint32_t)popc64(nm[g]);
+  }
+  if (mode == 1 && nn > 0) {
+    if (slog_n + nn <= slog_cap) {
+#pragma unroll
+      for (int g = 0; g < kBeamGroups; g++)
+        if (nw[g]) slog[slog_n + pre[g] + mbcnt(nm[g])] = nid[g];
+    }
+    slog_n += nn;
+  }
+#pragma unroll
+  for (int g = 0; g < kBeamGroups; g++)
+    if (nw[g]) st_id[pre[g] + mbcnt(nm[g])] = nid[g];
+  return nn;
 }
 
 // C[head..head+size) (held in e[]) merged with the batch elements in
@@ -1023,12 +1082,16 @@ hnsw_search_kernel(const SearchArgs P) {
       ff.adj0 = P.adj0;
       ff.m0 = m0;
       ff.n = beam;
-      ff.sj0 = (uint32_t)lane / m0;
-      ff.sp0 = (uint32_t)lane - ff.sj0 * m0;
-      ff.sj1 = (uint32_t)(lane + 64) / m0;
-      ff.sp1 = (uint32_t)(lane + 64) - ff.sj1 * m0;
-      ff.id0 = ff.id1 = ff.id2 = ff.id3 = kInvalid;
-      ff.v0 = ff.v1 = kInvalid;
+      ff.groups = (beam * m0 + 63) / 64;
+#pragma unroll
+      for (int g = 0; g < kBeamGroups; g++) {
+        ff.sj[g] = (uint32_t)(lane + 64 * g) / m0;
+        ff.sp[g] = (uint32_t)(lane + 64 * g) - ff.sj[g] * m0;
+        ff.v[g] = kInvalid;
+      }
+#pragma unroll
+      for (int t = 0; t < kBeamMax; t++) ff.id[t] = kInvalid;
+      uint32_t x[kBeamMax];
       if (allow == nullptr && P.deleted_bits == nullptr) {
         // -- every visited node is admissible: one list U (= W) of the ef
         // smallest seen, each entry marked once expanded; the candidates are
@@ -1039,10 +1102,11 @@ hnsw_search_kernel(const SearchArgs P) {
         list_load<SW>(Wk, wsize, ue);
 #pragma unroll
         for (int c = 0; c < SW; c++) uf[c] = 0u;
-        // the first `beam` unexpanded entries, marked: their ids
-        auto take_front = [&](uint32_t &y0, uint32_t &y1, uint32_t &y2, uint32_t &y3) {
+        // the first `beam` unexpanded entries, marked: their ids in x
+        auto take_front = [&]() {
           uint32_t got = 0;
-          y0 = y1 = y2 = y3 = kInvalid;
+#pragma unroll
+          for (int t = 0; t < kBeamMax; t++) x[t] = kInvalid;
 #pragma unroll
           for (int c = 0; c < SW; c++) {
             if (got >= beam || c * 64 >= (int)wsize) break;
@@ -1051,53 +1115,25 @@ hnsw_search_kernel(const SearchArgs P) {
             const bool pick = cand && r < beam;
             for (uint64_t m = __ballot(pick); m; m &= m - 1) {
               const uint32_t id = rl((uint32_t)ue[c], __ffsll((unsigned long long)m) - 1);
-              if (got == 0) y0 = id;
-              else if (got == 1) y1 = id;
-              else if (got == 2) y2 = id;
-              else y3 = id;
+#pragma unroll
+              for (int t = 0; t < kBeamMax; t++) x[t] = got == (uint32_t)t ? id : x[t];
               got++;
             }
             uf[c] = pick ? 1u : uf[c];
           }
           return got;
         };
-        uint32_t y0, y1, y2, y3;
-        uint32_t ne = take_front(y0, y1, y2, y3);
+        uint32_t ne = take_front();
         while (ne > 0) {
           q_exp += ne;
           // the table never fills: leave it before a step could pass its limit
           if (mode == 0 && spill && vcount + ne * m0 > P.hash_limit) leave_lds();
-          const uint32_t c0 = ff.sj0 < ne ? sel4(ff.sj0, y0, y1, y2, y3) : kInvalid;
-          const uint32_t c1 = ff.sj1 < ne ? sel4(ff.sj1, y0, y1, y2, y3) : kInvalid;
-          uint32_t nid0 = kInvalid, nid1 = kInvalid;
-          if (c0 != kInvalid)
-            nid0 = c0 == sel4(ff.sj0, ff.id0, ff.id1, ff.id2, ff.id3) ? ff.v0
-                                                                      : adj0[(size_t)c0 * m0 + ff.sp0];
-          if (c1 != kInvalid)
-            nid1 = c1 == sel4(ff.sj1, ff.id0, ff.id1, ff.id2, ff.id3) ? ff.v1
-                                                                      : adj0[(size_t)c1 * m0 + ff.sp1];
-          bool nw0 = false, nw1 = false;
-          if (mode == 0) {
-            lds_visit2(table, hcap, nid0, nid1, nw0, nw1);
-          } else {
-            if (nid0 != kInvalid) nw0 = bit_visit(spill, nid0);
-            if (nid1 != kInvalid) nw1 = bit_visit(spill, nid1);
-          }
-          const uint64_t nm0 = __ballot(nw0), nm1 = __ballot(nw1);
-          const uint32_t n0 = (uint32_t)popc64(nm0), nn = n0 + (uint32_t)popc64(nm1);
-          if (mode == 1 && nn > 0) {
-            if (slog_n + nn <= P.spill_log_cap) {
-              if (nw0) slog[slog_n + mbcnt(nm0)] = nid0;
-              if (nw1) slog[slog_n + n0 + mbcnt(nm1)] = nid1;
-            }
-            slog_n += nn;
-          }
+          const uint32_t nn = beam_visit(adj0, x, ne, ff, table, hcap, mode, spill, slog,
+                                         P.spill_log_cap, slog_n, st_id);
           vcount += nn;
           q_dist += nn;
           bool fetched = false;
           if (nn > 0) {
-            if (nw0) st_id[mbcnt(nm0)] = nid0;
-            if (nw1) st_id[n0 + mbcnt(nm1)] = nid1;
             wave_sync();
             gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qv, qs, (int)prec, st_id, (int)nn, st_d);
             wave_sync();
@@ -1121,7 +1157,8 @@ hnsw_search_kernel(const SearchArgs P) {
                 // next step's pops: their lists load during the writes.  A U
                 // entry's unexpanded rank counts the unexpanded U entries and
                 // the batch entries before it; a batch entry's is ua.
-                uint32_t id[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
+#pragma unroll
+                for (int t = 0; t < kBeamMax; t++) ff.id[t] = kInvalid;
                 uint32_t uprev = 0;
 #pragma unroll
                 for (int c = 0; c < SW; c++) {
@@ -1131,22 +1168,18 @@ hnsw_search_kernel(const SearchArgs P) {
                   const uint32_t ur =
                       uprev + (uint32_t)mbcnt(um) + (wp[c] - (uint32_t)(c * 64 + lane));
 #pragma unroll
-                  for (int p = 0; p < 4; p++) {
+                  for (int t = 0; t < kBeamMax; t++) {
                     const uint64_t hit =
-                        __ballot(un && wp[c] < nsize && ur == (uint32_t)p && p < (int)beam);
-                    if (hit) id[p] = rl((uint32_t)ue[c], __ffsll((unsigned long long)hit) - 1);
+                        __ballot(un && wp[c] < nsize && ur == (uint32_t)t && t < (int)beam);
+                    if (hit) ff.id[t] = rl((uint32_t)ue[c], __ffsll((unsigned long long)hit) - 1);
                   }
                   uprev += (uint32_t)popc64(um);
                 }
 #pragma unroll
-                for (int p = 0; p < 4; p++) {
-                  const uint64_t hit = __ballot(surv && ap < nsize && ua == (uint32_t)p && p < (int)beam);
-                  if (hit) id[p] = rl((uint32_t)akey, __ffsll((unsigned long long)hit) - 1);
+                for (int t = 0; t < kBeamMax; t++) {
+                  const uint64_t hit = __ballot(surv && ap < nsize && ua == (uint32_t)t && t < (int)beam);
+                  if (hit) ff.id[t] = rl((uint32_t)akey, __ffsll((unsigned long long)hit) - 1);
                 }
-                ff.id0 = id[0];
-                ff.id1 = id[1];
-                ff.id2 = id[2];
-                ff.id3 = id[3];
                 front_fetch(ff);
                 fetched = true;
               }
@@ -1167,24 +1200,24 @@ hnsw_search_kernel(const SearchArgs P) {
               list_load<SW>(Wk, wsize, ue);
 #pragma unroll
               for (int c = 0; c < SW; c++) {
-                const uint32_t x = c * 64 + lane;
-                uf[c] = x < wsize ? F[x] : 1u;
+                const uint32_t xi = c * 64 + lane;
+                uf[c] = xi < wsize ? F[xi] : 1u;
               }
               worst = ckey_d(Wk[wsize - 1]);
               wave_sync();
             }
           }
-          ne = take_front(y0, y1, y2, y3);
+          ne = take_front();
           if (!fetched) {
-            ff.id0 = y0;
-            ff.id1 = y1;
-            ff.id2 = y2;
-            ff.id3 = y3;
+#pragma unroll
+            for (int t = 0; t < kBeamMax; t++) ff.id[t] = x[t];
             front_fetch(ff);
           }
         }
       } else
       for (;;) {
+        // -- two lists (filters or tombstones: candidates that fail the
+        // filter stay traversable)
         if (csize < beam && pn > 0) refill();
         if (csize == 0) break;
         // pop up to `beam` candidates while the stop rule does not hold
@@ -1194,45 +1227,19 @@ hnsw_search_kernel(const SearchArgs P) {
             __ballot(lane < (int)avail && wsize >= ef && ckey_d(fe) > worst);
         const uint32_t ne = stop ? (uint32_t)(__ffsll((unsigned long long)stop) - 1) : avail;
         if (ne == 0) break;
-        const uint32_t x0 = rl((uint32_t)fe, 0), x1 = rl((uint32_t)fe, 1);
-        const uint32_t x2 = rl((uint32_t)fe, 2), x3 = rl((uint32_t)fe, 3);
+#pragma unroll
+        for (int t = 0; t < kBeamMax; t++) x[t] = rl((uint32_t)fe, t);
         head += ne;
         csize -= ne;
         q_exp += ne;
         // the table never fills: leave it before a step could pass its limit
         if (mode == 0 && spill && vcount + ne * m0 > P.hash_limit) leave_lds();
-        // the popped lists in slot layout, from the prefetch where it matches
-        const uint32_t c0 = ff.sj0 < ne ? sel4(ff.sj0, x0, x1, x2, x3) : kInvalid;
-        const uint32_t c1 = ff.sj1 < ne ? sel4(ff.sj1, x0, x1, x2, x3) : kInvalid;
-        uint32_t nid0 = kInvalid, nid1 = kInvalid;
-        if (c0 != kInvalid)
-          nid0 = c0 == sel4(ff.sj0, ff.id0, ff.id1, ff.id2, ff.id3) ? ff.v0
-                                                                    : adj0[(size_t)c0 * m0 + ff.sp0];
-        if (c1 != kInvalid)
-          nid1 = c1 == sel4(ff.sj1, ff.id0, ff.id1, ff.id2, ff.id3) ? ff.v1
-                                                                    : adj0[(size_t)c1 * m0 + ff.sp1];
-        bool nw0 = false, nw1 = false;
-        if (mode == 0) {
-          lds_visit2(table, hcap, nid0, nid1, nw0, nw1);
-        } else {
-          if (nid0 != kInvalid) nw0 = bit_visit(spill, nid0);
-          if (nid1 != kInvalid) nw1 = bit_visit(spill, nid1);
-        }
-        const uint64_t nm0 = __ballot(nw0), nm1 = __ballot(nw1);
-        const uint32_t n0 = (uint32_t)popc64(nm0), nn = n0 + (uint32_t)popc64(nm1);
-        if (mode == 1 && nn > 0) {
-          if (slog_n + nn <= P.spill_log_cap) {
-            if (nw0) slog[slog_n + mbcnt(nm0)] = nid0;
-            if (nw1) slog[slog_n + n0 + mbcnt(nm1)] = nid1;
-          }
-          slog_n += nn;
-        }
+        const uint32_t nn = beam_visit(adj0, x, ne, ff, table, hcap, mode, spill, slog,
+                                       P.spill_log_cap, slog_n, st_id);
         vcount += nn;
         q_dist += nn;
         bool fetched = false;
         if (nn > 0) {
-          if (nw0) st_id[mbcnt(nm0)] = nid0;
-          if (nw1) st_id[n0 + mbcnt(nm1)] = nid1;
           wave_sync();
           gather_dists<TMAX, EXTRA, EXACT, PC>(P, q, qc, qv, qs, (int)prec, st_id, (int)nn, st_d);
           wave_sync();
@@ -1266,10 +1273,8 @@ hnsw_search_kernel(const SearchArgs P) {
         if (!fetched) {
           // C's front is unchanged: fetch its lists for the next step
           const uint64_t nf = lane < (int)beam && (uint32_t)lane < csize ? Ck[head + lane] : ~0ull;
-          ff.id0 = (uint32_t)rl((uint32_t)nf, 0);
-          ff.id1 = (uint32_t)rl((uint32_t)nf, 1);
-          ff.id2 = (uint32_t)rl((uint32_t)nf, 2);
-          ff.id3 = (uint32_t)rl((uint32_t)nf, 3);
+#pragma unroll
+          for (int t = 0; t < kBeamMax; t++) ff.id[t] = rl((uint32_t)nf, t);
           front_fetch(ff);
         }
       }

@@ -71,11 +71,12 @@ struct SearchArgs {
   // construction candidates: node ids instead of keys in out_keys
   uint32_t ids_out;
   uint32_t hash_cap, hash_limit, lds_words;
-  // id/distance staging entries per wave (64, or 128 for beam steps)
+  // id/distance staging entries per wave (64, or a beam step's slots
+  // rounded up to 64, at most 256)
   uint32_t st_cap;
   // candidates a layer-0 step of the bf16 and per-dimension int8
-  // traversals expands (1: the crate's one-at-a-time rule; 2..4 with
-  // beam * m0 <= 128: one batch per step, every new neighbour tested against
+  // traversals expands (1: the crate's one-at-a-time rule; 2..8 with
+  // beam * m0 <= 256: one batch per step, every new neighbour tested against
   // the results as they stood at the step's start)
   uint32_t beam;
 };

@@ -542,8 +542,7 @@ class HnswIndex:
             self._stale = False
             if self._perdim:
                 _check(lib.hh_index_quantize_perdim(h, None, None))
-            if self._beam != DEFAULT_BEAM_WIDTH:
-                _check(lib.hh_index_set_beam_width(h, self._beam))
+            _check(lib.hh_index_set_beam_width(h, self._beam))
         elif self._stale:
             # inserts and removals since the upload: move only the changes
             ids = self._take_dirty()
@@ -572,12 +571,12 @@ class HnswIndex:
 
     def set_beam_width(self, width):
         """Candidates per layer-0 step of the bf16 and per-dimension int8
-        traversals (1: the crate's one-at-a-time rule; 2-4: beam steps, at
-        most 128 neighbour slots per step).  f32 and the crate's int8 codes
+        traversals (1: the crate's one-at-a-time rule; 2-8: beam steps, at
+        most 256 neighbour slots per step).  f32 and the crate's int8 codes
         always follow the crate's rule."""
         width = int(width)
-        if not 1 <= width <= 4:
-            raise InvalidArgument("beam width must be 1..4")
+        if not 1 <= width <= 8:
+            raise InvalidArgument("beam width must be 1..8")
         if self._idx is not None and not self._dirty:
             self._refuse_if_served()
             _check(lib.hh_index_set_beam_width(self._idx, width))

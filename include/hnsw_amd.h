@@ -318,16 +318,20 @@ int hh_index_set_query_order(hh_index *idx, int longest_first);
  * in the crate, which has neither precision).  width 1: the crate's layer-0
  * rule (search.rs:81-142), one candidate per expansion, each new neighbour
  * tested against the results as the sequential loop leaves them.  width
- * 2..4 (the default is 4): each step pops up to `width` candidates (each
- * only while the crate's stop rule does not hold for it; at most 128
- * neighbour slots per step, so m0 = 32 takes 4 and m0 = 64 takes 2; m0 >
- * 64 keeps width 1), gathers every new neighbour of those lists in one
- * batch and tests each against the results as they stood at the step's
- * start: it becomes a candidate iff fewer than ef results were held or its
- * distance is below their worst, and enters the results when admitted.  A
- * step's outcome does not depend on the order of its batch.  f32 and the
- * crate's int8 codes always follow the crate's rule.  Refused while a
- * serving loop is open; plans captured before the change are refused. */
+ * 2..8 (the default is 4): each step pops up to `width` candidates (each
+ * only while the crate's stop rule does not hold for it; at most 256
+ * neighbour slots per step, so m0 = 32 takes 8, m0 = 48 five and m0 = 64
+ * four; m0 > 128 keeps width 1), gathers every new neighbour of those lists
+ * in one batch and tests each against the results as they stood at the
+ * step's start: it becomes a candidate iff fewer than ef results were held
+ * or its distance is below their worst, and enters the results when
+ * admitted.  A query with no filter on an index with no tombstones keeps
+ * the results alone, each marked once expanded, and pops their first
+ * unexpanded entries (the same outcome except where a distance equals the
+ * worst result's).  A step's outcome does not depend on the order of its
+ * batch.  f32 and the crate's int8 codes always follow the crate's rule.
+ * Refused while a serving loop is open; plans captured before the change
+ * are refused. */
 int hh_index_set_beam_width(hh_index *idx, uint32_t width);
 /* Candidates per beam step this index's layout allows at its width. */
 int hh_index_beam_steps(const hh_index *idx, uint32_t *steps);

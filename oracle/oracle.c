@@ -863,7 +863,7 @@ static size_t search_layer0_beam(search_ctx *s, cand ep, size_t ef, size_t width
     cheap_push(&cands, ep);
     if (admit(s, ep.id)) cheap_push(&res, ep);
   }
-  uint32_t exp_ids[4];
+  uint32_t exp_ids[8];
   cand *fresh = (cand *)malloc(sizeof(cand) * (width * g->m0 + 1));
   while (cands.len) {
     size_t ne = 0;
@@ -932,7 +932,7 @@ static size_t search_layer0_unified(search_ctx *s, cand ep, size_t ef, size_t wi
     done[0] = 0;
     un = 1;
   }
-  uint32_t exp_ids[4];
+  uint32_t exp_ids[8];
   for (;;) {
     size_t ne = 0;
     for (size_t i = 0; i < un && ne < width; i++)
@@ -978,12 +978,12 @@ static size_t search_layer0_unified(search_ctx *s, cand ep, size_t ef, size_t wi
 }
 
 /* Candidates a beam step of the device engine expands: the requested width
- * (at most 4), at most 128 neighbour slots per step; lists of more than 64
+ * (at most 8), at most 256 neighbour slots per step; lists of more than 128
  * slots keep the crate's rule (width 1). */
 static size_t beam_steps(const or_graph *g) {
-  if (g->beam_width <= 1 || g->m0 > 64 || g->m0 == 0) return 1;
-  size_t w = g->beam_width > 4 ? 4 : g->beam_width;
-  const size_t fit = 128 / g->m0;
+  if (g->beam_width <= 1 || g->m0 > 128 || g->m0 == 0) return 1;
+  size_t w = g->beam_width > 8 ? 8 : g->beam_width;
+  const size_t fit = 256 / g->m0;
   return w < fit ? w : fit;
 }
 

@@ -104,7 +104,8 @@ typedef struct {
   const float *pd_mid, *pd_scale, *pd_rnorm;
   /* beam width of the bf16 and per-dimension int8 traversals (no crate
    * counterpart): 0 or 1 keeps the crate's one-candidate-at-a-time rule,
-   * 2..4 expands that many candidates per step (search_layer0_beam) */
+   * 2..8 expands that many candidates per step (search_layer0_beam,
+   * search_layer0_unified), at most 256 neighbour slots per step */
   uint32_t beam_width;
 } or_graph;
 

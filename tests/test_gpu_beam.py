@@ -32,15 +32,16 @@ def _oracle(idx, prec, width):
 @pytest.mark.parametrize("prec", list(PRECS))
 @pytest.mark.parametrize("m,metric,dim", [(16, H.Metric.L2, 128), (24, H.Metric.InnerProduct, 100),
                                           (32, H.Metric.Cosine, 128), (48, H.Metric.L2, 64),
-                                          (8, H.Metric.L2, 37)])
+                                          (8, H.Metric.L2, 37), (72, H.Metric.L2, 32)])
 def test_beam_steps_match_the_oracle(prec, m, metric, dim):
-    """Widths 1-4 on lists of 16, 32, 48, 64 and 96 slots (96: the crate's
-    rule whatever the width): results, distance bits, expansions and
-    distance counts equal the oracle's."""
+    """Widths 1-8 on lists of 16, 32, 48, 64, 96 and 144 slots (at most
+    256 slots per step, so 96-slot lists take two per step and lists of 144
+    keep the crate's rule whatever the width): results, distance bits,
+    expansions and distance counts equal the oracle's."""
     b, q = O.dataset_gaussian(3000, 64, dim, 3 * m + dim)
     idx = build(b, metric, m=m, threads=8)
     _ready(idx, prec)
-    for width in (1, 2, 3, 4):
+    for width in (1, 2, 3, 4, 5, 8):
         idx.set_beam_width(width)
         g = _oracle(idx, prec, width)
         for k, ef in ((10, 64), (1, 16), (40, 150)):
@@ -113,7 +114,7 @@ def test_beam_queries_return_the_same_alone_and_in_a_mixed_batch():
 @pytest.mark.parametrize("prec", list(PRECS))
 def test_beam_visited_set_leaves_lds_without_filling_it(prec):
     """A 256-entry visited table (the smallest launch shape at ef <= 64)
-    leaves LDS before a step of up to 128 new neighbours could fill it; the
+    leaves LDS before a step of up to 256 new neighbours could fill it; the
     search continues on the HBM bitset with the oracle's results."""
     b, q = O.dataset_gaussian(6000, 96, 64, 15)
     idx = build(b, H.Metric.L2, m=16, threads=8)
@@ -171,12 +172,12 @@ def test_beam_two_pass_equals_one_pass(prec):
 def test_beam_width_is_validated_and_refused_while_served():
     b, _ = O.dataset_gaussian(500, 1, 16, 3)
     idx = build(b)
-    for w in (0, 5):
+    for w in (0, 9):
         with pytest.raises(H.InvalidArgument):
             idx.set_beam_width(w)
     steps = ctypes.c_uint32()
     assert H.lib.hh_index_beam_steps(idx.device_index(), ctypes.byref(steps)) == 0
-    assert steps.value == 4  # m0 = 32: four lists per step
+    assert steps.value == 8  # m0 = 32: eight lists per step
     with H.Server(idx, k_max=10):
         with pytest.raises(H.InvalidArgument):
             idx.set_beam_width(2)
