@@ -177,7 +177,10 @@ def test_beam_width_is_validated_and_refused_while_served():
             idx.set_beam_width(w)
     steps = ctypes.c_uint32()
     assert H.lib.hh_index_beam_steps(idx.device_index(), ctypes.byref(steps)) == 0
-    assert steps.value == 8  # m0 = 32: eight lists per step
+    assert steps.value == 4  # the default width
+    idx.set_beam_width(8)
+    assert H.lib.hh_index_beam_steps(idx.device_index(), ctypes.byref(steps)) == 0
+    assert steps.value == 8  # m0 = 32: eight lists fill a step's 256 slots
     with H.Server(idx, k_max=10):
         with pytest.raises(H.InvalidArgument):
             idx.set_beam_width(2)
