@@ -95,6 +95,10 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="search measurement only: no precision variants, serving or CPU "
                          "baseline (for profiler runs)")
+    ap.add_argument("--quick-precision", choices=["bf16", "int8", "int8_perdim"], default="",
+                    help="with --quick: after the f32 measurement, enable this precision and "
+                         "time --steps launches of it at the same ef and launch shape (for "
+                         "profiler runs of the bf16 and int8 traversals)")
     ap.add_argument("--brute-only", action="store_true",
                     help="time only the exact brute force (MFMA shortlist and streaming "
                          "kernel) over the workload's rows (for profiler runs) and print its "
@@ -336,6 +340,26 @@ def main():
 
     variants, serve, cpu = {}, None, None
     if args.quick:
+        if args.quick_precision:
+            name = args.quick_precision
+            prec = {"bf16": H.Precision.BF16, "int8": H.Precision.INT8,
+                    "int8_perdim": H.Precision.INT8_PERDIM}[name]
+            if name == "bf16":
+                idx.enable_bf16()
+            elif name == "int8":
+                idx.enable_quantization(on_device=True)
+            else:
+                idx.enable_quantization_perdim()
+            h = idx.device_index()
+            run(ef, prec)
+            r = recall()
+            for _ in range(args.warmup):
+                run(ef, prec)
+            sec = timed(ef, prec, args.steps)
+            variants[name] = {"recall": round(r, 4), "qps": round(nq / sec),
+                              "vs_f32": round(nq / sec / qps, 3)}
+            log(f"{name} ef={ef}: recall={r:.4f} qps={nq / sec:,.0f} ({nq / sec / qps:.2f}x "
+                f"the timed f32 steps)")
         emit(args, locals())
         return
 
