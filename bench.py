@@ -356,10 +356,19 @@ def main():
             for _ in range(args.warmup):
                 run(ef, prec)
             sec = timed(ef, prec, args.steps)
+            idx.search_stats(reset=True)
+            run(ef, prec)
+            synchronize()
+            pst = idx.search_stats()
             variants[name] = {"recall": round(r, 4), "qps": round(nq / sec),
-                              "vs_f32": round(nq / sec / qps, 3)}
+                              "vs_f32": round(nq / sec / qps, 3),
+                              "dist_evals_per_query": round((pst["n_dist"] + pst["n_upper"]) / nq, 1),
+                              "expansions_per_query": round(pst["n_expand"] / nq, 1),
+                              "reranked_per_query": round(pst["n_rerank"] / nq, 1)}
             log(f"{name} ef={ef}: recall={r:.4f} qps={nq / sec:,.0f} ({nq / sec / qps:.2f}x "
-                f"the timed f32 steps)")
+                f"the timed f32 steps); {variants[name]['dist_evals_per_query']} distance "
+                f"evaluations, {variants[name]['expansions_per_query']} layer-0 expansions, "
+                f"{variants[name]['reranked_per_query']} f32 reranks per query")
         emit(args, locals())
         return
 
