@@ -297,9 +297,9 @@ struct Workspace {
   int init() {
     HH_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     int st = dalloc(&counter, 4);
-    if (!st) st = dalloc(&stats, kStatCount);
+    if (!st) st = dalloc(&stats, (size_t)hhk::kStatStride * hhk::kStatLines);
     if (st) return st;
-    HH_HIP(hipMemset(stats, 0, kStatCount * sizeof(unsigned long long)));
+    HH_HIP(hipMemset(stats, 0, (size_t)hhk::kStatStride * hhk::kStatLines * sizeof(unsigned long long)));
     HH_HIP(hipMemset(counter, 0, 4 * sizeof(uint32_t)));
     return HH_OK;
   }
@@ -1720,8 +1720,12 @@ int hh_index_stats(hh_index *idx, hh_search_stats *out) {
   HH_HIP(hipSetDevice(idx->device));
   int st = drain(idx);
   if (st) return st;
+  std::vector<unsigned long long> lines((size_t)hhk::kStatStride * hhk::kStatLines);
+  HH_HIP(hipMemcpy(lines.data(), idx->stats_ws.stats, lines.size() * sizeof(unsigned long long),
+                   hipMemcpyDeviceToHost));
   unsigned long long v[kStatCount] = {0};
-  HH_HIP(hipMemcpy(v, idx->stats_ws.stats, sizeof v, hipMemcpyDeviceToHost));
+  for (uint32_t l = 0; l < hhk::kStatLines; l++)
+    for (int c = 0; c < kStatCount; c++) v[c] += lines[(size_t)l * hhk::kStatStride + c];
   out->queries = v[0];
   out->n_dist = v[1];
   out->n_upper = v[2];
@@ -1917,7 +1921,8 @@ int hh_index_reset_stats(hh_index *idx) {
   HH_HIP(hipSetDevice(idx->device));
   int st = drain(idx);
   if (st) return st;
-  HH_HIP(hipMemset(idx->stats_ws.stats, 0, kStatCount * sizeof(unsigned long long)));
+  HH_HIP(hipMemset(idx->stats_ws.stats, 0,
+                   (size_t)hhk::kStatStride * hhk::kStatLines * sizeof(unsigned long long)));
   return HH_OK;
 }
 

@@ -306,8 +306,15 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
       GM(P.seed)[qi] = ((uint64_t)__float_as_uint(best) << 32) | cur;
     }
   }
-  // one count per query: its first lane adds the query's total
-  if (il == 0 && up && P.stats) atomicAdd(&P.stats[2], up);
+  // one count per query (its first lane holds the query's total), summed
+  // over the wave and added by lane 0 into the wave's line of the counters
+  uint32_t wsum = il == 0 ? (uint32_t)up : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) wsum += (uint32_t)__shfl_xor((int)wsum, off, 64);
+  if ((threadIdx.x & 63) == 0 && wsum && P.stats) {
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    atomicAdd(&P.stats[(size_t)(wave % kStatLines) * kStatStride + 2], (unsigned long long)wsum);
+  }
 }
 
 // Longest-first query order for the two-pass launch: a histogram of the

@@ -833,6 +833,10 @@ hnsw_search_kernel(const SearchArgs P) {
   gmut<uint64_t> pool = GM(P.cpool) + (size_t)slot * P.cpool_cap;
   const bool has_pool = P.cpool != nullptr;
 
+  // The re-run launch follows every batch that may overflow a pool; with
+  // nothing listed its waves leave before touching the shared counter.
+  if (P.phase == 3 && rl(G(P.redo_count)[0], 0) == 0) return;
+
   // Phase 1 (descent only, nearly uniform work per query) takes queries
   // statically, without the shared counter.
   uint32_t static_next = slot;
@@ -1487,16 +1491,18 @@ hnsw_search_kernel(const SearchArgs P) {
     GM(P.out_count)[qi] = (uint32_t)(wsize < (int)k ? wsize : (int)k);
     wave_sync();
   }
+  // this wave's line of the counters; zero counts are not added
   if (lane == 0 && P.stats) {
-    atomicAdd(&P.stats[0], st.q);
-    atomicAdd(&P.stats[1], st.dist);
-    atomicAdd(&P.stats[2], st.up);
-    atomicAdd(&P.stats[3], st.exp);
-    atomicAdd(&P.stats[4], st.rr);
-    atomicAdd(&P.stats[5], st.ovf);
-    atomicAdd(&P.stats[6], st.spill);
-    atomicAdd(&P.stats[7], st.cspill);
-    atomicAdd(&P.stats[8], st.redo);
+    unsigned long long *sl = P.stats + (size_t)(slot % kStatLines) * kStatStride;
+    if (st.q) atomicAdd(&sl[0], st.q);
+    if (st.dist) atomicAdd(&sl[1], st.dist);
+    if (st.up) atomicAdd(&sl[2], st.up);
+    if (st.exp) atomicAdd(&sl[3], st.exp);
+    if (st.rr) atomicAdd(&sl[4], st.rr);
+    if (st.ovf) atomicAdd(&sl[5], st.ovf);
+    if (st.spill) atomicAdd(&sl[6], st.spill);
+    if (st.cspill) atomicAdd(&sl[7], st.cspill);
+    if (st.redo) atomicAdd(&sl[8], st.redo);
   }
 }
 

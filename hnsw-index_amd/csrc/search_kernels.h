@@ -52,7 +52,7 @@ struct SearchArgs {
   uint32_t cpool_cap;
   uint32_t *spill_log;          // per-wave ids marked in the bitset (spill_log_cap each)
   uint32_t spill_log_cap;
-  unsigned long long *stats;    // 9 counters
+  unsigned long long *stats;    // kStatLines lines of kStatStride counters (9 used)
   // Queries whose candidate pool dropped an entry are listed here (count in
   // redo_count) and run again by a phase-3 launch whose per-wave pools hold
   // cpool_cap >= n entries, so no query returns other results than the
@@ -102,6 +102,12 @@ struct PrepArgs {
 };
 
 constexpr uint32_t kOrderBins = 4096;
+
+// The search counters are kept in kStatLines copies, one 128-byte line
+// each, summed when read: a wave adds into line (its global index mod
+// kStatLines), so the atomics of a draining launch do not all meet on one
+// cache line.
+constexpr uint32_t kStatStride = 16, kStatLines = 64;
 
 // Precision classes of the search kernel instances: bf16 rows, int8 codes
 // (the crate's or the per-dimension ones), or every precision.
