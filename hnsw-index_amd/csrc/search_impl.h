@@ -841,6 +841,7 @@ hnsw_search_kernel(const SearchArgs P) {
   // statically, without the shared counter.
   uint32_t static_next = slot;
   const uint32_t total_waves = gridDim.x * (blockDim.x >> 6);
+  bool first = true;
   for (;;) {
     // Side effects in this loop run on every lane with wave-uniform values
     // (never under `lane == 0` branches): correlated lane-0 branches around
@@ -853,8 +854,12 @@ hnsw_search_kernel(const SearchArgs P) {
     } else if (P.phase == 3) {
       qi = rl(atomicAdd(P.counter + 1, lane == 0 ? 1u : 0u), 0);
     } else {
-      qi = rl(atomicAdd(P.counter, lane == 0 ? 1u : 0u), 0);
+      // each wave's first query is its slot; later ones come from the
+      // shared counter, numbered past those (so the launch does not start
+      // with every wave's atomic on one address)
+      qi = first ? slot : total_waves + rl(atomicAdd(P.counter, lane == 0 ? 1u : 0u), 0);
     }
+    first = false;
     if (P.phase == 3) {
       const uint32_t nredo = rl(G(P.redo_count)[0], 0);
       if (qi >= (nredo < P.nq ? nredo : (uint32_t)P.nq)) break;
