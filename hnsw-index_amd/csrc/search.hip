@@ -12,140 +12,15 @@ namespace hhk {
 
 // ------------------------------------------------------ descent pass
 // The first pass of a two-pass launch when m <= 32: one query per 16 lanes
-// (m <= 16, four per wavefront) or 32 lanes (two per wavefront), one
-// neighbour row per lane.  A lane computes a whole distance with the AVX2
-// kernel's 32 accumulators (value 32t + j into accumulator j, one fused
-// multiply-add per element in t order, the 8-wide remainder into
-// accumulators 0..7, x86.rs:47-77 / 96-129) and its lane combination
-// s[l] = (a_l + a_l+8) + (a_l+16 + a_l+24),
-// h = ((s0+s4)+(s1+s5))+((s2+s6)+(s3+s7)) (x86.rs:21-27), then the tail
-// products, so every distance has the bits of the search kernel's quarter
-// reduction.  The greedy steps are those of search_graph / greedy_closest
+// (m <= 16, four per wavefront) or 32 lanes (two per wavefront).  f32 and
+// bf16 rows are read and reduced by 16-lane quarters exactly as the search
+// kernel's gather does (the AVX2 kernel's accumulators and lane
+// combination, x86.rs:21-27, 47-77, 96-129), so every distance has the
+// search kernel's bits; int8 code rows are summed one per lane (exact
+// integers).  Without a per-query copy of the query in LDS, more blocks
+// stay resident.  The greedy steps are those of search_graph / greedy_closest
 // (src/hnsw/search.rs:49-68,146-161): the first list position holding the
 // smallest distance, taken when strictly below the best.
-
-// Accumulator j's four fused multiply-adds of one full 4-block (values
-// 32t + j, t in 4b..4b+3, at j*4 + t%4), in t order.
-template <bool L2, class E>
-__device__ __forceinline__ float acc4(float acc, const float *q, gptr<E> blk, int j) {
-  float4 x;
-  if constexpr (sizeof(E) == 4) {
-    x = *(gptr<float4>)(blk + 4 * j);
-  } else {
-    const uint2 h = *(gptr<uint2>)(blk + 4 * j);
-    x = make_float4(__uint_as_float(h.x << 16), __uint_as_float(h.x & 0xFFFF0000u),
-                    __uint_as_float(h.y << 16), __uint_as_float(h.y & 0xFFFF0000u));
-  }
-  const float4 y = *reinterpret_cast<const float4 *>(q + 4 * j);
-  if (L2) {
-    float d = y.x - x.x;
-    acc = __builtin_fmaf(d, d, acc);
-    d = y.y - x.y;
-    acc = __builtin_fmaf(d, d, acc);
-    d = y.z - x.z;
-    acc = __builtin_fmaf(d, d, acc);
-    d = y.w - x.w;
-    return __builtin_fmaf(d, d, acc);
-  }
-  acc = __builtin_fmaf(y.x, x.x, acc);
-  acc = __builtin_fmaf(y.y, x.y, acc);
-  acc = __builtin_fmaf(y.z, x.z, acc);
-  return __builtin_fmaf(y.w, x.w, acc);
-}
-
-// The crate's combination of the 32 accumulators: s[l] = (a_l + a_{l+8}) +
-// (a_{l+16} + a_{l+24}), h = ((s0+s4)+(s1+s5))+((s2+s6)+(s3+s7))
-// (src/kernels/x86.rs:21-27, 70-73).
-#define HH_COMBINE32(acc, out)                                                                 \
-  do {                                                                                         \
-    float sl_[8];                                                                              \
-    _Pragma("unroll") for (int l_ = 0; l_ < 8; l_++) sl_[l_] =                                 \
-        __fadd_rn(__fadd_rn(acc[l_], acc[l_ + 8]), __fadd_rn(acc[l_ + 16], acc[l_ + 24]));    \
-    out = __fadd_rn(__fadd_rn(__fadd_rn(sl_[0], sl_[4]), __fadd_rn(sl_[1], sl_[5])),           \
-                    __fadd_rn(__fadd_rn(sl_[2], sl_[6]), __fadd_rn(sl_[3], sl_[7])));          \
-  } while (0)
-
-// One lane's whole distance sum of an exact-layout row (D = 32 * TMAX) in
-// the crate's order, the same bits as the quarter kernel's.
-template <int TMAX, bool L2, class E>
-__device__ __forceinline__ float lane_distance(const float *q, gptr<E> row) {
-  float acc[32];
-#pragma unroll
-  for (int j = 0; j < 32; j++) acc[j] = 0.0f;
-#pragma unroll
-  for (int b = 0; b < TMAX / 4; b++) {
-#pragma unroll
-    for (int j = 0; j < 32; j++) acc[j] = acc4<L2, E>(acc[j], q + b * 128, row + b * 128, j);
-  }
-  float h;
-  HH_COMBINE32(acc, h);
-  return h;
-}
-
-// The last, partial block of TW < 4 values per accumulator (value 32t + j
-// at j*TW + t%4).
-template <int TW, bool L2, class E>
-__device__ __forceinline__ void acc_partial(float (&acc)[32], const float *q, gptr<E> blk) {
-#pragma unroll
-  for (int j = 0; j < 32; j++) {
-#pragma unroll
-    for (int tt = 0; tt < TW; tt++) {
-      const float x = to_f<E>(blk[j * TW + tt]), y = q[j * TW + tt];
-      if (L2) {
-        const float d = y - x;
-        acc[j] = __builtin_fmaf(d, d, acc[j]);
-      } else {
-        acc[j] = __builtin_fmaf(y, x, acc[j]);
-      }
-    }
-  }
-}
-
-// Any layout: the full 4-blocks, the last partial block (value 32t + j at
-// j*tw + t%4, tw = T mod 4), the 8-wide remainder into accumulators 0..7,
-// then the unfused tail products added after the combination, in order
-// (x86.rs:70-77, 121-129) -- the quarter kernel's order, element by element.
-template <bool L2, class E>
-__device__ __forceinline__ float lane_distance_any(const float *q, gptr<E> row, const Shape &s) {
-  float acc[32];
-#pragma unroll
-  for (int j = 0; j < 32; j++) acc[j] = 0.0f;
-  const int full = s.T >> 2;
-  for (int b = 0; b < full; b++) {
-#pragma unroll
-    for (int j = 0; j < 32; j++) acc[j] = acc4<L2, E>(acc[j], q + b * 128, row + b * 128, j);
-  }
-  const int tw = s.T & 3;
-  if (tw == 1) acc_partial<1, L2, E>(acc, q + full * 128, row + full * 128);
-  if (tw == 2) acc_partial<2, L2, E>(acc, q + full * 128, row + full * 128);
-  if (tw == 3) acc_partial<3, L2, E>(acc, q + full * 128, row + full * 128);
-  const int rest = 32 * s.T;
-  for (int u = 0; u < s.U; u++) {
-#pragma unroll
-    for (int l = 0; l < 8; l++) {
-      const float x = to_f<E>(row[rest + 8 * u + l]), y = q[rest + 8 * u + l];
-      if (L2) {
-        const float d = y - x;
-        acc[l] = __builtin_fmaf(d, d, acc[l]);
-      } else {
-        acc[l] = __builtin_fmaf(y, x, acc[l]);
-      }
-    }
-  }
-  float h;
-  HH_COMBINE32(acc, h);
-  for (int r = 0; r < s.R; r++) {
-    const int pos = rest + 8 * s.U + r;
-    const float x = to_f<E>(row[pos]), y = q[pos];
-    if (L2) {
-      const float d = y - x;
-      h = __fadd_rn(h, __fmul_rn(d, d));
-    } else {
-      h = __fadd_rn(h, __fmul_rn(y, x));
-    }
-  }
-  return h;
-}
 
 // Integer codes: the exact integer sums over the whole code row (zero
 // padded to cstride on both sides) in any order, rescaled as the search
@@ -194,44 +69,57 @@ __device__ __forceinline__ float lane_distance_i8(const SearchArgs &P, uint32_t 
   return __fsub_rn(1.0f, __fmul_rn(__int2float_rn(acc), s2));
 }
 
-// The traversal distance of the batch's precision (0 f32, 1 the crate's
-// int8 codes, 2 bf16 rows, 3 per-dimension codes), with the bits the search
-// kernel computes.
-template <int TMAX, bool EXACT>
-__device__ __forceinline__ float descent_distance(const SearchArgs &P, uint32_t prec,
-                                                  const Shape &shp, const float *q,
-                                                  const uint2 *qc, const QCodeInfo &qs,
-                                                  uint32_t id) {
-  if (prec == 1 || prec == 3) return lane_distance_i8<EXACT ? 4 * TMAX : 0>(P, prec, qs, qc, id);
-  float s;
-  if (prec == 2) {
-    gptr<uint16_t> row = G(P.bf16) + (size_t)id * P.stride;
-    if constexpr (EXACT)
-      s = P.metric == 0 ? lane_distance<TMAX, true, uint16_t>(q, row)
-                        : lane_distance<TMAX, false, uint16_t>(q, row);
-    else
-      s = P.metric == 0 ? lane_distance_any<true, uint16_t>(q, row, shp)
-                        : lane_distance_any<false, uint16_t>(q, row, shp);
-  } else {
-    gptr<float> row = G(P.rows) + (size_t)id * P.stride;
-    if constexpr (EXACT)
-      s = P.metric == 0 ? lane_distance<TMAX, true, float>(q, row)
-                        : lane_distance<TMAX, false, float>(q, row);
-    else
-      s = P.metric == 0 ? lane_distance_any<true, float>(q, row, shp)
-                        : lane_distance_any<false, float>(q, row, shp);
+// f32 or bf16 distances from one query to the rows ids[0..cnt) (this
+// query's LDS slots), written to out[0..cnt): the query's QPQ quarters take
+// rows qq, qq + QPQ, ..., B at a time with all of a round's loads issued
+// first, each row read by a quarter's 16 lanes in the search kernel's lane
+// layout and reduced as it reduces (the same bits as its gather_round).
+// Sixteen lanes reading one row keep a load instruction on 4 rows per wave
+// instead of 64.
+template <int TMAX, bool EXTRA, bool EXACT, class E, int QPQ>
+__device__ __forceinline__ void descent_rows(const SearchArgs &P, const QLanes<TMAX, EXTRA> &qv,
+                                             const E *rows, const uint32_t *ids, int cnt, int qq,
+                                             float *out) {
+  constexpr int B = rows_per_quarter<TMAX>();
+  const int lane = lane_id(), i = lane & 15, quarter = lane >> 4;
+  const Shape s{P.T, P.U, P.R, P.stride};
+  for (int base = qq; base < cnt; base += B * QPQ) {
+    QLanes<TMAX, EXTRA> x[B];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const int row = base + b * QPQ;
+      const uint32_t rid = ids[row < cnt ? row : base];
+      if constexpr (!EXACT) {
+#pragma unroll
+        for (int t = 0; t < TMAX; t++) x[b].lo[t] = x[b].hi[t] = 0.0f;
+        x[b].e[0] = x[b].r = 0.0f;
+        if constexpr (EXTRA) x[b].e[1] = x[b].e[2] = 0.0f;
+      }
+      load_q<TMAX, EXTRA, E, EXACT>(rows + (size_t)rid * P.stride, s, i, x[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      const int row = base + b * QPQ;
+      if (row >= cnt) break;
+      float lo, hi, tail;
+      if (P.metric == 0) quarter_acc<TMAX, EXTRA, true, EXACT>(qv, x[b], s, i, lo, hi, tail);
+      else quarter_acc<TMAX, EXTRA, false, EXACT>(qv, x[b], s, i, lo, hi, tail);
+      const float h = finish_metric(quarter_reduce<EXTRA>(lo, hi, tail, s.R, quarter), P.metric);
+      if (i == 0) out[row] = h;
+    }
   }
-  return finish_metric(s, P.metric);
 }
 
 template <int TMAX, int LPQ, bool EXACT>
 __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const SearchArgs P) {
   static_assert(!EXACT || TMAX % 4 == 0, "exact layout");
-  constexpr int kStride = descent_stride<TMAX>();
   constexpr int kCodeWords = descent_code_words<TMAX>();
   constexpr int kQueries = descent_threads<TMAX>() / LPQ;
-  __shared__ __align__(16) float qs[kQueries][kStride];
+  constexpr bool kExtra = !EXACT;
+  constexpr int kQpq = LPQ / 16;  // quarters per query
   __shared__ __align__(16) uint2 qcs[kQueries][kCodeWords];
+  __shared__ uint32_t sid[kQueries][LPQ];  // a step's neighbour ids
+  __shared__ float sdist[kQueries][LPQ];   // and their distances
   const int il = lane_id() & (LPQ - 1);
   const int qslot = (int)(threadIdx.x / LPQ);
   const uint64_t qi = (uint64_t)blockIdx.x * kQueries + qslot;
@@ -248,10 +136,17 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
     if (status != 0 || k == 0 || P.live == 0 || P.top < 0) {
       GM(P.seed)[qi] = ((uint64_t)__float_as_uint(-INFINITY) << 32) | kInvalid;
     } else {
-      // the query (prepared, row layout) and its codes into this slot
-      gptr<float4> src = (gptr<float4>)(P.qprep + qi * P.stride);
-      float4 *dst = reinterpret_cast<float4 *>(qs[qslot]);
-      for (int c = il; c < (int)(P.stride / 4); c += LPQ) dst[c] = src[c];
+      // the query (prepared, row layout) in the quarter-lane layout, and
+      // its codes into this slot
+      QLanes<TMAX, kExtra> qv;
+      if constexpr (!EXACT) {
+#pragma unroll
+        for (int t = 0; t < TMAX; t++) qv.lo[t] = qv.hi[t] = 0.0f;
+        qv.e[0] = qv.r = 0.0f;
+        if constexpr (kExtra) qv.e[1] = qv.e[2] = 0.0f;
+      }
+      load_q<TMAX, kExtra, float, EXACT>(P.qprep + qi * P.stride, shp, il & 15, qv);
+      const bool codes = prec == 1 || prec == 3;
       QCodeInfo qk{0, 1.0f, 0.0f};
       if (prec == 1 || prec == 3) {
         gptr<uint2> csrc = (gptr<uint2>)(P.qcode + qi * P.cstride);
@@ -263,11 +158,28 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
         }
       }
       wave_sync();
-      const float *q = qs[qslot];
       const uint2 *qc = qcs[qslot];
+      uint32_t *ids = sid[qslot];
+      float *dists = sdist[qslot];
+      // f32 / bf16 distances of rows ids[0..cnt) into dists (quarter lanes)
+      auto rows_dist = [&](int cnt) {
+        if (prec == 2)
+          descent_rows<TMAX, kExtra, EXACT, uint16_t, kQpq>(P, qv, P.bf16, ids, cnt, il >> 4, dists);
+        else
+          descent_rows<TMAX, kExtra, EXACT, float, kQpq>(P, qv, P.rows, ids, cnt, il >> 4, dists);
+        wave_sync();
+      };
       uint32_t cur = P.entry;
       uint64_t cur_off = G(P.upper_off)[cur];
-      float best = descent_distance<TMAX, EXACT>(P, prec, shp, q, qc, qk, cur);
+      float best;
+      if (codes) {
+        best = lane_distance_i8<EXACT ? 4 * TMAX : 0>(P, prec, qk, qc, cur);
+      } else {
+        ids[il] = cur;
+        wave_sync();
+        rows_dist(1);
+        best = dists[0];
+      }
       up++;
       for (int layer = P.top; layer >= 1; layer--) {
         for (;;) {
@@ -277,7 +189,15 @@ __global__ void __launch_bounds__(descent_threads<TMAX>()) descent_kernel(const 
           const bool valid = (uint32_t)il < cnt;
           const uint32_t nid = valid ? blk[1 + il] : kInvalid;
           const uint64_t noff = valid ? (uint64_t)G(P.upper_off)[nid] : 0;
-          float md = valid ? descent_distance<TMAX, EXACT>(P, prec, shp, q, qc, qk, nid) : 0.0f;
+          float md;
+          if (codes) {
+            md = valid ? lane_distance_i8<EXACT ? 4 * TMAX : 0>(P, prec, qk, qc, nid) : 0.0f;
+          } else {
+            ids[il] = nid;
+            wave_sync();
+            rows_dist((int)cnt);
+            md = valid ? dists[il] : 0.0f;
+          }
           int mi = valid ? il : LPQ;
 #pragma unroll
           for (int m = 1; m < LPQ; m <<= 1) {
