@@ -6,12 +6,18 @@
  * twice, as Rust (which never contracts) does; fused operations are written
  * as fmaf() exactly where the reference calls an FMA intrinsic.
  *
- * How each part is pinned (tests/test_oracle.py):
- *   SplitMix64 ............ Vigna KAT (src/synthetic.rs:182-187)
+ * How each part is pinned (tests/test_oracle.py, tests/test_host.py):
+ *   SplitMix64 ............ Vigna KAT (src/synthetic.rs:182-187) and its
+ *                           closed form
  *   gaussian / Box-Muller . first components of gaussian(10000,200,128,42)
- *                           (tests/recall.rs:130-139); moments test
- *   clustered ............. determinism/shape (src/synthetic.rs:210-220) and
- *                           centre/spread statistics
+ *                           (tests/recall.rs:130-139); Box-Muller of the
+ *                           stream; moments test
+ *   next_below ............ Lemire's multiply-shift over the raw stream
+ *                           (a modulo reduction fails the test)
+ *   clustered ............. the crate's draw order (src/synthetic.rs:102-124)
+ *                           by an independent restatement; both published
+ *                           100k recall tables (docs/benchmarks.md:117-121,
+ *                           f32 and int8 columns) reproduced exactly
  *   kernels ............... scalar KATs (src/kernels/scalar.rs:28-33); exact on
  *                           small integers (tests/kernels.rs:83-96); every
  *                           backend within the scalar tolerance
@@ -21,13 +27,26 @@
  *   int8 kernels .......... naive i64 sums for len 0..80 and MAX_DIM extremes
  *                           (src/kernels/int8.rs:68-85)
  *   prepare ............... src/vector.rs:106-179 KATs
- *   quantizer ............. src/quantization.rs:101-145 KATs
+ *   quantizer ............. src/quantization.rs:101-145 KATs; rounding half
+ *                           away from zero; approximate distance rounding
+ *                           (src/quantization.rs:85-92) against a numpy
+ *                           restatement (an f64 product fails the test)
+ *   per-dimension codes ... closed forms of the min/max parameters; the
+ *                           distance against float64 within its stated
+ *                           error bound (no crate counterpart)
+ *   bf16 rows ............. round-to-nearest-even against torch's bfloat16
  *   brute force ........... src/brute.rs:218-262 KATs; FP64 variant vs numpy
  *   HNSW search ........... n <= M complete graph == brute force
- *                           (src/hnsw/tests.rs:113-127); recall table of
- *                           docs/benchmarks.md:162-169; k/ef edge cases;
- *                           filter KATs (src/hnsw/tests.rs:293-313)
- *   recall ................ src/eval.rs:59-72 KATs
+ *                           (src/hnsw/tests.rs:113-127); the 10k recall
+ *                           table of docs/benchmarks.md:162-169 exactly;
+ *                           k/ef edge cases; filter KATs
+ *                           (src/hnsw/tests.rs:293-313)
+ *   beam steps ............ a hand-worked six-node graph (expansions and
+ *                           distances per width); complete graphs equal
+ *                           brute force; recall and cost against width 1
+ *                           (no crate counterpart)
+ *   recall ................ src/eval.rs:59-72 KATs; hh_recall_at_k against
+ *                           them and the oracle
  */
 #include "oracle.h"
 
