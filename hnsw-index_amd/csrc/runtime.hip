@@ -579,6 +579,7 @@ void *search_kernel_ptr(int pc) {
   switch (pc) {
     case hhk::kPrecBf16: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecBf16>);
     case hhk::kPrecCodes: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecCodes>);
+    case hhk::kPrecF32: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecF32>);
     default: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecAny>);
   }
 }
@@ -673,7 +674,7 @@ int precision_class(const SearchCall &c) {
   if (c.specs || c.build) return hhk::kPrecAny;
   if (c.precision == 2) return hhk::kPrecBf16;
   if (c.precision == 1 || c.precision == 3) return hhk::kPrecCodes;
-  return hhk::kPrecAny;
+  return hhk::kPrecF32;
 }
 
 // Largest max(ef, k) a launch must serve: the call's own for a uniform

@@ -1,7 +1,7 @@
 // search.hip — the query preparation, the descent pass and the query
 // order of the batched HNSW search on gfx950 (the layer-0 search kernel is
-// in search_impl.h, instantiated by search_bf16.hip, search_codes.hip and
-// search_any.hip).
+// in search_impl.h, instantiated by search_f32.hip, search_bf16.hip,
+// search_codes.hip and search_any.hip).
 #include <hip/hip_runtime.h>
 
 #include "search_impl.h"

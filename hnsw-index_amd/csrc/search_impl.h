@@ -805,8 +805,8 @@ __device__ __forceinline__ void seed_query(const SearchArgs &P, uint32_t qi, uin
 
 // EXACT: D == 32 * TMAX (branch-free row loads, see load_q).  PC: the
 // precisions the instance handles (kPrecBf16, kPrecCodes for the crate's
-// and the per-dimension int8 codes, kPrecAny for every precision); a
-// uniform bf16 or code batch runs the instance of its precision, so its
+// and the per-dimension int8 codes, kPrecF32, kPrecAny for every
+// precision); a uniform batch runs the instance of its precision, so its
 // loop carries no code or registers for the others.
 template <int TMAX, bool EXTRA, bool EXACT, int SW, int PC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? 3 : 2)))
@@ -879,6 +879,7 @@ hnsw_search_kernel(const SearchArgs P) {
     k = rl(k, 0); ef = rl(ef, 0); prec = rl(prec, 0); filt = rl(filt, 0);
     if constexpr (PC == kPrecBf16) prec = 2;
     if constexpr (PC == kPrecCodes) prec = prec == 3 ? 3u : 1u;
+    if constexpr (PC == kPrecF32) prec = 0;
     const uint64_t obase = (uint64_t)qi * P.k_stride;
     const int status = (int)rl((uint32_t)G(P.status)[qi], 0);
     if (status != 0 || k == 0 || P.live == 0 || P.top < 0) {

@@ -110,8 +110,9 @@ constexpr uint32_t kOrderBins = 4096;
 constexpr uint32_t kStatStride = 16, kStatLines = 64;
 
 // Precision classes of the search kernel instances: bf16 rows, int8 codes
-// (the crate's or the per-dimension ones), or every precision.
-constexpr int kPrecBf16 = 1, kPrecCodes = 2, kPrecAny = 3;
+// (the crate's or the per-dimension ones), every precision (mixed batches
+// and construction), or f32 rows alone (uniform f32 batches).
+constexpr int kPrecBf16 = 1, kPrecCodes = 2, kPrecAny = 3, kPrecF32 = 4;
 
 template <int TMAX, bool EXTRA, bool EXACT, int SW, int PC>
 __global__ void hnsw_search_kernel(const SearchArgs P);
