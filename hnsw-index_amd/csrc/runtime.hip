@@ -742,7 +742,11 @@ int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
 
 // The descent pass as descent_kernel: rows of T 4-blocks take the TMAX
 // instance that holds them, the exact layouts (D = 32 * TMAX) the unrolled
-// one, m <= 16 sixteen lanes per query and m <= 32 thirty-two.
+// one, m <= 16 sixteen lanes per query and m <= 32 thirty-two.  Uniform
+// f32 and bf16 batches take thirty-two lanes at any m: two quarters share
+// a step's rows, so a step waits for half as many load rounds and twice
+// the waves hide the latency (+1% QPS at 1M x 128, m = 16; int8 code rows,
+// one per lane, lose 2-3% with idle lanes and keep sixteen).
 int launch_descent(const hh_index *idx, uint64_t nq, const hhk::SearchArgs &a,
                    hipStream_t stream) {
   using K = void (*)(const hhk::SearchArgs);
@@ -751,7 +755,8 @@ int launch_descent(const hh_index *idx, uint64_t nq, const hhk::SearchArgs &a,
   // 128, which the exact instances take as a compile-time length)
   const bool exact = idx->lay.U == 0 && idx->lay.R == 0 && (int)idx->lay.T == tmax &&
                      (idx->cstride == 0 || idx->cstride == 32u * (uint32_t)tmax);
-  const int wide = idx->m > 16;
+  const bool row_batch = !a.specs && (a.precision == 0 || a.precision == 2);
+  const int wide = idx->m > 16 || row_batch;
   const int ti = tmax == 4 ? 0 : tmax == 8 ? 1 : 2;
   static const K table[3][2][2] = {
       {{hhk::descent_kernel<4, 16, false>, hhk::descent_kernel<4, 16, true>},
