@@ -580,6 +580,7 @@ void *search_kernel_ptr(int pc) {
     case hhk::kPrecBf16: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecBf16>);
     case hhk::kPrecCodes: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecCodes>);
     case hhk::kPrecF32: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecF32>);
+    case hhk::kPrecI8: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecI8>);
     default: return reinterpret_cast<void *>(&hhk::hnsw_search_kernel<TMAX, EX, XA, SW, hhk::kPrecAny>);
   }
 }
@@ -673,7 +674,8 @@ struct SearchCall {
 int precision_class(const SearchCall &c) {
   if (c.specs || c.build) return hhk::kPrecAny;
   if (c.precision == 2) return hhk::kPrecBf16;
-  if (c.precision == 1 || c.precision == 3) return hhk::kPrecCodes;
+  if (c.precision == 3) return hhk::kPrecCodes;
+  if (c.precision == 1) return hhk::kPrecI8;
   return hhk::kPrecF32;
 }
 

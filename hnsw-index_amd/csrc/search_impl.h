@@ -609,7 +609,7 @@ struct QCodeInfo {
 // the f32 path).
 template <int CC, int PC>
 constexpr int code_rows_per_quarter() {
-  return PC == kPrecCodes ? (CC <= 2 ? 8 : 4) : 4;
+  return codes_class(PC) ? (CC <= 2 ? 8 : 4) : 4;
 }
 
 // Integer-code distances of rows ids[0..cnt): prec 1 the crate's codes
@@ -768,7 +768,7 @@ __device__ __forceinline__ void gather_dists(const SearchArgs &P, const QLanes<T
                                              const uint4 (&qv)[2], const QCodeInfo &qs, int prec,
                                              const uint32_t *ids, int cnt, float *out) {
   if (prec == 1 || prec == 3) {
-    if constexpr (PC == kPrecCodes && TMAX <= 4) {
+    if constexpr (codes_class(PC) && TMAX <= 4) {
       if (P.cstride == 128) {
         gather_codes4<2>(P, qv, qs, prec, ids, cnt, out);
         return;
@@ -804,10 +804,10 @@ __device__ __forceinline__ void seed_query(const SearchArgs &P, uint32_t qi, uin
 }
 
 // EXACT: D == 32 * TMAX (branch-free row loads, see load_q).  PC: the
-// precisions the instance handles (kPrecBf16, kPrecCodes for the crate's
-// and the per-dimension int8 codes, kPrecF32, kPrecAny for every
-// precision); a uniform batch runs the instance of its precision, so its
-// loop carries no code or registers for the others.
+// precisions the instance handles (kPrecBf16, kPrecCodes for the
+// per-dimension int8 codes, kPrecI8 for the crate's, kPrecF32, kPrecAny for
+// every precision); a uniform batch runs the instance of its precision, so
+// its loop carries no code or registers for the others.
 template <int TMAX, bool EXTRA, bool EXACT, int SW, int PC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TMAX <= 8 ? 3 : 2)))
 hnsw_search_kernel(const SearchArgs P) {
@@ -878,7 +878,8 @@ hnsw_search_kernel(const SearchArgs P) {
     }
     k = rl(k, 0); ef = rl(ef, 0); prec = rl(prec, 0); filt = rl(filt, 0);
     if constexpr (PC == kPrecBf16) prec = 2;
-    if constexpr (PC == kPrecCodes) prec = prec == 3 ? 3u : 1u;
+    if constexpr (PC == kPrecCodes) prec = 3;
+    if constexpr (PC == kPrecI8) prec = 1;
     if constexpr (PC == kPrecF32) prec = 0;
     const uint64_t obase = (uint64_t)qi * P.k_stride;
     const int status = (int)rl((uint32_t)G(P.status)[qi], 0);
@@ -903,7 +904,7 @@ hnsw_search_kernel(const SearchArgs P) {
     for (int t = 0; t < TMAX; t++) q.lo[t] = q.hi[t] = 0.0f;
     q.e[0] = q.r = 0.0f;
     if constexpr (EXTRA) q.e[1] = q.e[2] = 0.0f;
-    if constexpr (PC != kPrecCodes) load_q<TMAX, EXTRA, float>(P.qprep + (size_t)qi * P.stride, shp, i16, q);
+    if constexpr (!codes_class(PC)) load_q<TMAX, EXTRA, float>(P.qprep + (size_t)qi * P.stride, shp, i16, q);
     uint2 qc[code_chunks<TMAX, EXACT>()];
 #pragma unroll
     for (int c = 0; c < code_chunks<TMAX, EXACT>(); c++) qc[c] = make_uint2(0, 0);
@@ -912,7 +913,7 @@ hnsw_search_kernel(const SearchArgs P) {
     uint4 qv[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
     if (prec == 1 || prec == 3) {
       bool quad = false;
-      if constexpr (PC == kPrecCodes && TMAX <= 4) quad = P.cstride == 128;
+      if constexpr (codes_class(PC) && TMAX <= 4) quad = P.cstride == 128;
       if (quad) {
         gptr<uint4> cq = (gptr<uint4>)(P.qcode + (size_t)qi * 128 + 32 * (lane & 3));
         qv[0] = cq[0];
@@ -1458,7 +1459,7 @@ hnsw_search_kernel(const SearchArgs P) {
       res_id[idx] = (uint32_t)w;
     }
     wave_sync();
-    if constexpr (PC == kPrecCodes) load_q<TMAX, EXTRA, float>(P.qprep + (size_t)qi * P.stride, shp, i16, q);
+    if constexpr (codes_class(PC)) load_q<TMAX, EXTRA, float>(P.qprep + (size_t)qi * P.stride, shp, i16, q);
     if (prec != 0) {
       for (int base = 0; base < wsize; base += 64) {
         const int c = wsize - base < 64 ? wsize - base : 64;

@@ -109,10 +109,13 @@ constexpr uint32_t kOrderBins = 4096;
 // cache line.
 constexpr uint32_t kStatStride = 16, kStatLines = 64;
 
-// Precision classes of the search kernel instances: bf16 rows, int8 codes
-// (the crate's or the per-dimension ones), every precision (mixed batches
-// and construction), or f32 rows alone (uniform f32 batches).
-constexpr int kPrecBf16 = 1, kPrecCodes = 2, kPrecAny = 3, kPrecF32 = 4;
+// Precision classes of the search kernel instances: bf16 rows, the
+// per-dimension int8 codes, every precision (mixed batches and
+// construction), f32 rows alone, or the crate's int8 codes alone (the last
+// four for uniform batches).
+constexpr int kPrecBf16 = 1, kPrecCodes = 2, kPrecAny = 3, kPrecF32 = 4, kPrecI8 = 5;
+// Instances that traverse on int8 code rows only.
+constexpr bool codes_class(int pc) { return pc == kPrecCodes || pc == kPrecI8; }
 
 template <int TMAX, bool EXTRA, bool EXACT, int SW, int PC>
 __global__ void hnsw_search_kernel(const SearchArgs P);
