@@ -666,15 +666,18 @@ struct SearchCall {
   uint64_t filter_words = 0;
 };
 
-// The precision class of a call: a uniform bf16 or int8-code batch runs
-// the instance of its precision (no registers or code for the others in
-// its loop); f32 batches, construction searches and mixed batches run the
-// general instance, which measured faster for f32 than an f32-only one
-// (7.6 vs 6.7 M QPS at 1M x 128, ef=64).
-int precision_class(const SearchCall &c) {
+// The precision class of a call: a uniform batch runs the instance of its
+// precision (no registers or code for the others in its loop; at 1M x 128,
+// ef=64, the f32-only instance ran 9.4 M QPS against 8.95 M for the general
+// one, the crate-int8 one 11.05 M against 10.6 M).  The bf16 and
+// per-dimension instances take beam steps only: at beam width 1 (or lists
+// too long for a step) those batches run the general instance, as do
+// construction searches and mixed batches.
+int precision_class(const hh_index *idx, const SearchCall &c) {
   if (c.specs || c.build) return hhk::kPrecAny;
-  if (c.precision == 2) return hhk::kPrecBf16;
-  if (c.precision == 3) return hhk::kPrecCodes;
+  const bool beam = beam_steps(idx) > 1;
+  if (c.precision == 2) return beam ? hhk::kPrecBf16 : hhk::kPrecAny;
+  if (c.precision == 3) return beam ? hhk::kPrecCodes : hhk::kPrecAny;
   if (c.precision == 1) return hhk::kPrecI8;
   return hhk::kPrecF32;
 }
@@ -719,7 +722,7 @@ int search_config(hh_index *idx, const SearchCall &c, SearchConfig &cfg) {
   cfg.beam = bs > 1 && !c.build && (c.specs || c.precision == 2 || c.precision == 3) ? bs : 1u;
   cfg.st_cap = cfg.beam > 1 ? std::max<uint32_t>(64u, (cfg.beam * idx->m0 + 63) / 64 * 64) : 64u;
   cfg.lds_words = lds_words_for(cfg.hash_cap, sw, cfg.st_cap);
-  cfg.kern = pick_search_kernel(idx->lay, sw, precision_class(c));
+  cfg.kern = pick_search_kernel(idx->lay, sw, precision_class(idx, c));
   // waves per block: the tuned value, else whichever of 1, 2, 4 keeps the
   // most waves resident
   int blocks_per_cu = 1, best_waves = 0;

@@ -1086,7 +1086,7 @@ hnsw_search_kernel(const SearchArgs P) {
       q_spill++;
     };
 
-    if ((prec == 2 || prec == 3) && P.beam > 1) {
+    if (beam_class(PC) || ((prec == 2 || prec == 3) && P.beam > 1)) {
       // ---- beam steps (bf16 and per-dimension int8 traversals)
       const uint32_t m0 = P.m0, beam = P.beam;
       FrontFetch ff;

@@ -109,13 +109,16 @@ constexpr uint32_t kOrderBins = 4096;
 // cache line.
 constexpr uint32_t kStatStride = 16, kStatLines = 64;
 
-// Precision classes of the search kernel instances: bf16 rows, the
-// per-dimension int8 codes, every precision (mixed batches and
-// construction), f32 rows alone, or the crate's int8 codes alone (the last
-// four for uniform batches).
+// Precision classes of the search kernel instances: bf16 rows and the
+// per-dimension int8 codes (beam steps), every precision (mixed batches,
+// construction, beam width 1), f32 rows alone, or the crate's int8 codes
+// alone.
 constexpr int kPrecBf16 = 1, kPrecCodes = 2, kPrecAny = 3, kPrecF32 = 4, kPrecI8 = 5;
 // Instances that traverse on int8 code rows only.
 constexpr bool codes_class(int pc) { return pc == kPrecCodes || pc == kPrecI8; }
+// Instances launched only with beam steps (the bf16 and per-dimension
+// traversals at beam width 2 or more): the crate-rule loop is not compiled.
+constexpr bool beam_class(int pc) { return pc == kPrecBf16 || pc == kPrecCodes; }
 
 template <int TMAX, bool EXTRA, bool EXACT, int SW, int PC>
 __global__ void hnsw_search_kernel(const SearchArgs P);
